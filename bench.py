@@ -1,0 +1,181 @@
+"""Benchmark of the Chebyshev moment engine on one MI355X.
+
+Workload (the headline configuration): kpm_1d DOS moments of a disordered
+graphene flake, 1024 x 1024 honeycomb cells x 2 orbitals (NH = 2,097,152,
+nnz = 8,388,608 stored: 3 bonds per site each way + the disorder diagonal),
+t = 1, on-site disorder U[-0.5, 0.5], rescaled by a fixed a = 3.6; NC = 2048,
+NR = 64 unit random-phase probes.  One step = one full kpm_1d call (NC/2 =
+1024 fused recurrence passes) with the probe block already resident in HBM.
+The metric is moments/s = NC * NR * nnz / seconds, for the FP32-recurrence
+form (ComplexF32 vectors, FP64 arithmetic and FP64 moment accumulation); the
+plain FP64 form is timed beside it and the max moment error of FP32 against
+FP64 is reported.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "kpm.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def step_bytes(nh: int, nnz: int, nrp: int, elem: int, val_bytes: int) -> int:
+    """Algorithmic HBM bytes of one fused pass: H once (values + int32 column
+    indices + row pointers) and each vector row once: read v_n, read v_{n-1},
+    write v_{n+1}."""
+    return nnz * (val_bytes + 4) + (nh + 1) * 4 + 3 * nh * nrp * elem
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cells", type=int, default=1024, help="honeycomb cells per side")
+    ap.add_argument("--nc", type=int, default=2048)
+    ap.add_argument("--nr", type=int, default=64)
+    ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline leg")
+    ap.add_argument("--skip-fp64", action="store_true", help="omit the FP64 comparison leg")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--kernel-launches", type=int, default=100)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine is measured on one GPU; multi-GPU runs are "
+                                   "not part of this build (--gpus must be 1)"}))
+        return 2
+
+    import numpy as np
+
+    import kpmhip as K
+    from kpmhip import models
+    from kpmhip.rng import Xoshiro256pp
+
+    if not K.activate(0):
+        raise SystemExit("no HIP device visible")
+    ctx = K.get_context()
+    L, NC, NR = args.cells, args.nc, args.nr
+    print(f"[bench] building graphene {L}x{L}x2 ...", file=sys.stderr, flush=True)
+    H = (models.graphene(L, L, disorder=0.5, seed=42) / 3.6).tocsc()
+    NH, nnz = H.shape[0], H.nnz
+    print(f"[bench] NH={NH} nnz={nnz}; probes NR={NR} ...", file=sys.stderr, flush=True)
+    psi = K.random_phase_vectors(Xoshiro256pp(42), NH, NR)
+    Hd = K.to_device(H)
+
+    def run_leg(precision: str):
+        blk = K.DeviceBlock.from_host(psi, precision)
+        mu = np.zeros((NR, NC), dtype=np.complex128, order="F")
+        lib = ctx.lib
+        for _ in range(args.warmup):
+            K._lib.check(lib.kpm_1d_block(ctx.handle, Hd.handle, NC, blk.handle,
+                                          K._lib.dptr(mu)), "kpm_1d_block")
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.event_record(0)
+        for _ in range(args.steps):
+            K._lib.check(lib.kpm_1d_block(ctx.handle, Hd.handle, NC, blk.handle,
+                                          K._lib.dptr(mu)), "kpm_1d_block")
+        ctx.event_record(1)
+        ctx.synchronize()
+        wall = time.perf_counter() - t0
+        ev_ms = ctx.event_elapsed(0, 1)
+        # the dominant kernel alone: back-to-back fused passes on the same stream
+        x = K.DeviceBlock.from_host(psi, precision)
+        z = K.DeviceBlock.from_host(psi, precision)
+        ms = ctypes.c_float(0.0)
+        K._lib.check(lib.kpm_bench_step(ctx.handle, Hd.handle, x.handle, z.handle,
+                                        args.kernel_launches, ctypes.byref(ms)), "bench_step")
+        del x, z, blk
+        return mu.copy(), wall / args.steps, ev_ms / 1e3 / args.steps, ms.value / args.kernel_launches
+
+    print("[bench] FP32-recurrence leg ...", file=sys.stderr, flush=True)
+    mu32, wall32, ev32, kms32 = run_leg("f32")
+    mu64 = None
+    if not args.skip_fp64:
+        print("[bench] FP64 leg ...", file=sys.stderr, flush=True)
+        mu64, wall64, ev64, kms64 = run_leg("f64")
+
+    units = NC * NR * nnz
+    value = units / wall32
+    nrp32 = K.moment._padded_nr(NR, 1)
+    val_bytes = 8 if Hd.is_real else 16
+    b32 = step_bytes(NH, nnz, nrp32, 8, val_bytes)
+    achieved32 = b32 / (kms32 * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved32, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved32 / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_step<FP32,LPR=32,MODE_DBL>", "bytes_per_launch": b32,
+                "avg_launch_ms": round(kms32, 5)}
+    tfile = os.path.join(ROOT, "profiles", "traffic_graphene2M_fp32.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            roofline["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+
+    extra = {"ms_per_step_event": round(ev32 * 1e3, 3)}
+    if mu64 is not None:
+        nrp64 = K.moment._padded_nr(NR, 0)
+        b64 = step_bytes(NH, nnz, nrp64, 16, val_bytes)
+        avg32 = mu32.mean(axis=0)
+        avg64 = mu64.mean(axis=0)
+        extra.update({
+            "fp64_value": units / wall64,
+            "fp64_ms_per_step": round(wall64 * 1e3, 3),
+            "fp32_speedup_vs_fp64": round(wall64 / wall32, 3),
+            "fp64_kernel_gbs": round(b64 / (kms64 * 1e-3) / 1e9, 1),
+            "fp64_kernel_frac": round(b64 / (kms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "max_abs_dmu_fp32_vs_fp64": float(np.max(np.abs(avg32 - avg64))),
+            "max_abs_dmu_per_probe": float(np.max(np.abs(mu32 - mu64))),
+        })
+
+    cpu = None
+    if not args.skip_cpu:
+        print("[bench] CPU baseline (oracle restatement) ...", file=sys.stderr, flush=True)
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        threads = min(os.cpu_count() or 1, 16)
+        oracle.set_threads(threads)
+        runner = oracle.StepRunner(H, psi)
+        runner.run(1)
+        done, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < args.cpu_seconds:
+            runner.run(1)
+            done += 1
+        tcpu = time.perf_counter() - t0
+        cpu = {"value": 2 * done * NR * nnz / tcpu, "unit": "moments/s", "cores": threads,
+               "kind": "port",
+               "sample": f"{done} doubling steps (2 moments each) of the same graphene "
+                         f"NH={NH} NR={NR} workload, FP64, {threads} threads, {tcpu:.1f} s"}
+
+    out = {
+        "metric": "moments/s (NC*NR*nnz/s), FP32-recurrence kpm_1d",
+        "value": value,
+        "unit": "moments/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall32 * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c32-storage/f64-accumulate",
+        "data": "synthetic",
+        "config": {"workload": f"graphene {L}x{L}x2 DOS kpm_1d", "NH": NH, "nnz": nnz, "NC": NC,
+                   "NR": NR, "disorder": "U[-0.5,0.5]", "a": 3.6},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        **extra,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

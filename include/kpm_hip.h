@@ -1,0 +1,186 @@
+/*
+ * kpm_hip.h -- C ABI of libkpm_hip.so, the MI355X (gfx950) Chebyshev moment
+ * engine that replaces KPM.jl's CUDA package extension (ext/KPMCUDAExt.jl).
+ *
+ * Conventions
+ *   - Every entry point returns an int status (KPM_OK = 0, negative on error);
+ *     kpm_last_error() returns a thread-local message for the last failure.
+ *   - Host blocks use the Julia caller's layout: probe blocks are NH x NR
+ *     column-major, interleaved complex double (ComplexF64); mu_all is NR x NC
+ *     column-major; the 2-D table mu is NC x NC column-major with
+ *     mu[n + NC*m] = mu[n,m]; the 3-D table is NC x NC x NC column-major.
+ *   - Sparse inputs are Julia SparseMatrixCSC arrays (colptr, rowval, nzval),
+ *     int64 indices with index_base 0 or 1, interleaved complex double values.
+ *   - One HIP stream per context; calls on one context must be serialised by
+ *     the caller; different contexts are independent.
+ *   - The library owns all device memory it allocates; handles are released
+ *     with the matching *_destroy.
+ */
+#ifndef KPM_HIP_H
+#define KPM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------
+ * KPM_EINVAL   <-> ArgumentError / AssertionError in the reference
+ *                  (src/moment.jl:431,434,438,512,515-519,767-807,913-973)
+ * KPM_EUNIMPL  <-> throw("unimplemented") (src/moment.jl:734,882)
+ * KPM_EUNSTABLE<-> error() of the recurrence stability guard
+ *                  (src/utils/chebyshev_iteration.jl:100-132)             */
+#define KPM_OK 0
+#define KPM_EINVAL (-1)
+#define KPM_ENOMEM (-2)
+#define KPM_EHIP (-3)
+#define KPM_EUNSTABLE (-4)
+#define KPM_EUNIMPL (-5)
+
+/* recurrence-vector storage: FP64 = ComplexF64 throughout (the reference's
+ * dt_cplx, src/KPM.jl:5-7); FP32 = ComplexF32 vectors, FP64 arithmetic and
+ * FP64 moment accumulation. */
+#define KPM_VEC_F64 0
+#define KPM_VEC_F32 1
+
+/* kpm_2d contraction arithmetic */
+#define KPM_GEMM_F64 0
+#define KPM_GEMM_F32 1
+
+/* moment_parity of kpm_2d! (src/moment.jl:913-915,1067-1069) */
+#define KPM_PARITY_NONE 0
+#define KPM_PARITY_ODD 1
+#define KPM_PARITY_EVEN 2
+
+typedef struct kpm_ctx kpm_ctx;     /* device + stream + cached workspaces */
+typedef struct kpm_csr kpm_csr;     /* device CSR operator (H, J_alpha, ...) */
+typedef struct kpm_block kpm_block; /* device NH x NR recurrence block */
+
+/* ---- errors / version ---------------------------------------------------- */
+const char* kpm_last_error(void);
+int kpm_abi_version(void);
+
+/* ---- device layer: src/device.jl (ACTIVE_DEVICE, whichcore, device_zeros,
+ *      device_free_memory) and ext/KPMCUDAExt.jl:13-20,43-45 ------------- */
+/* number of visible GPUs (0 -> the caller stays on the CPU device,
+ * the equivalent of CUDA.functional() == false at ext/KPMCUDAExt.jl:16) */
+int kpm_device_count(int* count);
+int kpm_ctx_create(int device, kpm_ctx** out);
+int kpm_ctx_destroy(kpm_ctx* ctx);
+int kpm_ctx_synchronize(kpm_ctx* ctx);
+/* device_free_memory(::Dev) -- ext/KPMCUDAExt.jl:45, src/device.jl:66-67 */
+int kpm_device_free_memory(kpm_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
+/* name of the device, e.g. for logs */
+int kpm_device_name(kpm_ctx* ctx, char* buf, size_t len);
+
+/* ---- operators: to_device(::Dev, ::SparseMatrixCSC, T) -- ext:26-27 ------
+ * Builds the device CSR of A (not of its transpose) from Julia CSC arrays.
+ * Values are stored complex FP64, or real FP64 when every imaginary part is
+ * zero (the moments are unchanged; it halves the operator's bytes). */
+int kpm_csr_from_csc(kpm_ctx* ctx, int64_t n, const int64_t* colptr, const int64_t* rowval,
+                     const double* nzval, int index_base, kpm_csr** out);
+/* Device CSR of adjoint(A) from the CSC arrays of A: the operator of
+ * mul!(Jα_psi, Jα', ψ) at src/moment.jl:680. */
+int kpm_csr_adjoint_from_csc(kpm_ctx* ctx, int64_t n, const int64_t* colptr,
+                             const int64_t* rowval, const double* nzval, int index_base,
+                             kpm_csr** out);
+/* Device CSR from CSR arrays (row pointers / column indices), e.g. when the
+ * caller already holds H in row-major sparse form. */
+int kpm_csr_from_csr(kpm_ctx* ctx, int64_t n, const int64_t* rowptr, const int64_t* colind,
+                     const double* vals, int index_base, kpm_csr** out);
+int kpm_csr_destroy(kpm_csr* A);
+int kpm_csr_info(const kpm_csr* A, int64_t* n, int64_t* nnz, int* is_real);
+
+/* ---- recurrence blocks: maybe_on_device_zeros(dt_cplx, NH, NR) and
+ *      maybe_to_device/maybe_to_host of probe blocks -- ext:28,36,43 ------- */
+int kpm_block_create(kpm_ctx* ctx, int64_t NH, int64_t NR, int vec_prec, kpm_block** out);
+int kpm_block_destroy(kpm_block* b);
+/* host NH x NR column-major ComplexF64 -> device block (rounded for FP32) */
+int kpm_block_upload(kpm_block* b, const double* host);
+/* device block -> host NH x NR column-major ComplexF64 */
+int kpm_block_download(const kpm_block* b, double* host);
+int kpm_block_copy(kpm_block* dst, const kpm_block* src);
+int kpm_block_zero(kpm_block* b);
+
+/* ---- recurrence primitives: ext/KPMCUDAExt.jl:84-121 ---------------------
+ * chebyshev_iter_single(H, V_pp, V_p[, V_out]) (chebyshev_iteration.jl:39-91,
+ * 135-167): V_out <- 2 H V_p - V_pp.  V_out may be V_pp (in place). */
+int kpm_cheb_step(kpm_ctx* ctx, const kpm_csr* H, const kpm_block* Vpp, const kpm_block* Vp,
+                  kpm_block* Vout);
+/* mul!(Y, A, X, alpha, beta) with real alpha, beta: Y <- alpha A X + beta Y */
+int kpm_spmm(kpm_ctx* ctx, const kpm_csr* A, double alpha, const kpm_block* X, double beta,
+             kpm_block* Y);
+/* columnwise_dot!(target, n, A, B; alpha, beta_col, beta_scale)
+ * (src/utils/vectors.jl:103-117, ext/KPMCUDAExt.jl:128-145):
+ * target[i,n] = alpha <A_i|B_i> + beta_scale target[i,beta_col]; the target
+ * is a host NR x NC column-major ComplexF64 matrix, n and beta_col are
+ * 1-based (beta_col == 0: no beta term).  FP64 accumulation. */
+int kpm_colwise_dot(kpm_ctx* ctx, double* target, int64_t NC, int64_t n, const kpm_block* A,
+                    const kpm_block* B, double alpha, int64_t beta_col, double beta_scale);
+
+/* ---- moment engine: src/moment.jl ----------------------------------------
+ * kpm_1d! equal-vector doubling form (src/moment.jl:420-492): NC even,
+ * NC/2 fused SpMM+dot passes; mu_all NR x NC column-major ComplexF64 out. */
+int kpm_1d(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t NR, int64_t NH,
+           const double* psi, double* mu_all, int vec_prec);
+/* kpm_1d! bra/ket form (src/moment.jl:498-556): NC >= 2, NC-1 passes. */
+int kpm_1d_lr(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t NR, int64_t NH,
+              const double* psi_l, const double* psi_r, double* mu_all, int vec_prec);
+/* kpm_1d_current! (src/moment.jl:656-714): Jadj is the device adjoint of
+ * J_alpha (kpm_csr_adjoint_from_csc). */
+int kpm_1d_current(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Jadj, int64_t NC, int64_t NR,
+                   int64_t NH, const double* psi, double* mu_all, int vec_prec);
+/* Same engines on probe blocks already resident in device memory (the
+ * block is read, not modified). */
+int kpm_1d_block(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, const kpm_block* psi,
+                 double* mu_all);
+int kpm_1d_lr_block(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, const kpm_block* psi_l,
+                    const kpm_block* psi_r, double* mu_all);
+
+/* kpm_2d_blocking (src/moment.jl:759-810).  budget < 0 means "auto" (50% of
+ * free device memory, src/moment.jl:741); arr_size/right_block <= 0 mean :auto. */
+int kpm_2d_blocking(kpm_ctx* ctx, int64_t NH, int64_t NR, int64_t NC, int64_t budget,
+                    int64_t arr_size, int64_t right_block, int64_t* K_left, int64_t* K_right,
+                    int64_t* bytes);
+/* kpm_2d! / _kpm_2d_blocked! (src/moment.jl:887-1088):
+ * mu[n,m] = (1/NR) sum_r <T_{m-1} psi_l_r | Jb T_{n-1} Ja psi_r_r>,
+ * blocked as K_left left slots x K_right right slots, contraction on the
+ * MFMA cores.  mu is NC x NC column-major ComplexF64 (overwritten). */
+int kpm_2d(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, const kpm_csr* Jb, int64_t NC,
+           int64_t NR, int64_t NH, const double* psi_l, const double* psi_r, int64_t K_left,
+           int64_t K_right, int parity, int mn_sym, int vec_prec, int gemm_mode, double* mu);
+/* kpm_3d! (src/moment.jl:1106-1227): mu NC x NC x NC column-major, one
+ * kpm_2d table per outer order n1, mu[:,:,n1] with Jgamma as the inner Ja. */
+int kpm_3d(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, const kpm_csr* Jb,
+           const kpm_csr* Jg, int64_t NC, int64_t NR, int64_t NH, const double* psi_l,
+           const double* psi_r, int64_t K_left, int64_t K_right, int vec_prec, int gemm_mode,
+           double* mu);
+
+/* ---- measurement ---------------------------------------------------------
+ * Launch the fused recurrence+doubling-dot pass `nsteps` times on the
+ * context stream over blocks X (v_n) and Z (v_{n-1}, overwritten), bracketed
+ * by HIP events on that stream; *ms is the elapsed time of all passes.
+ * The output is not meaningful (timing only). */
+int kpm_bench_step(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z, int64_t nsteps,
+                   float* ms);
+/* HIP-event timer on the context stream: record into slot 0..15, read the
+ * elapsed milliseconds between two recorded slots. */
+int kpm_event_record(kpm_ctx* ctx, int slot);
+int kpm_event_elapsed(kpm_ctx* ctx, int slot_a, int slot_b, float* ms);
+
+/* ---- launch-shape tuner ---------------------------------------------------
+ * No counterpart in the reference (it has no tuning knob).  Times every
+ * candidate geometry of the fused pass (rows in flight per lane x workgroups
+ * per CU) on the caller's operator and blocks X, Z (Z is overwritten),
+ * records the fastest for this shape and returns it. */
+int kpm_autotune_step(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z, int reps,
+                      int* best_unroll, int* best_grid, float* best_ms_per_step);
+/* Set the table entry for a shape directly (unroll 1, 2 or 4; grid 0 = auto). */
+int kpm_tune_set(kpm_ctx* ctx, const kpm_csr* H, int64_t NR, int vec_prec, int unroll, int grid);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KPM_HIP_H */

@@ -1,0 +1,250 @@
+// engine2d.hip -- kpm_2d (src/moment.jl:887-1088) and kpm_3d
+// (src/moment.jl:1106-1227) on the device.
+//
+// Blocking follows _kpm_2d_blocked! exactly: a K_left-slot left array
+// L = T_m psi_l filled by chebyshev_iter and continued by chebyshev_iter_wrap
+// (src/utils/chebyshev_iteration.jl:10-36), a 3-slot right ring
+// T_n Ja psi_r restarted every rep (:1039-1050), K_right slots
+// R = Jb T_n Ja psi_r (:1051) and one contraction per (rep, right block)
+// accumulated on the device (no per-block D2H sync).  The 3-address steps
+// (V_out <- 2 H V_p - V_pp) are single fused launches: V_pp is read and
+// V_out written by the same pass, no copy.
+//
+// Slots are stored back to back with a stride of NH*NRp complex values
+// rounded up to 16 (zero tail), so the contraction sees dense vectors.
+#include <algorithm>
+#include <vector>
+
+#include "kpm_internal.h"
+
+namespace kpm {
+
+StepConfig tuned_config(kpm_ctx* ctx, const kpm_csr* A, int64_t NR, int prec, int mode);
+
+static int spmm(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* x, const kpm_block* z,
+                kpm_block* out, double alpha, double beta) {
+  StepLaunch s{A, x, z, out, nullptr, alpha, beta, MODE_NONE};
+  return launch_step(ctx, s, tuned_config(ctx, A, x->NR, x->prec, MODE_NONE), nullptr);
+}
+
+static void slot_view(void* base, int64_t stride_bytes, int64_t idx, int64_t NH, int64_t NR,
+                      int prec, kpm_ctx* ctx, kpm_block* out) {
+  out->ctx = ctx;
+  out->NH = NH;
+  out->NR = NR;
+  out->NRp = padded_nr(NR, prec);
+  out->prec = prec;
+  out->d = (char*)base + stride_bytes * idx;
+  out->owned = false;
+}
+
+struct Plan2d {
+  int64_t NC, NR, NH, K_left, K_right;
+  int parity, mn_sym, prec, gemm_mode;
+};
+
+// mu_dev: NC x NC column-major double2, overwritten
+static int run_2d(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, const kpm_csr* Jb,
+                  const Plan2d& p, const kpm_block* psi_l, const kpm_block* psi_r, void* mu_dev) {
+  const int64_t NH = p.NH, NR = p.NR, NC = p.NC;
+  const int prec = p.prec;
+  const int64_t NRp = padded_nr(NR, prec);
+  const int64_t used = NH * NRp;                 // complex values per block
+  const int64_t Kp = (used + 15) / 16 * 16;      // slot stride in complex values
+  const int64_t eb = elem_bytes(prec);
+  const int64_t stride = Kp * eb;
+  const int64_t KL = p.K_left, KR = p.K_right;
+  KPM_TRY(ensure(ctx, ctx->slots_l, (size_t)(stride * KL)));
+  KPM_TRY(ensure(ctx, ctx->slots_r, (size_t)(stride * KR)));
+  if (Kp > used) {
+    KPM_HIP_TRY(hipMemset2DAsync((char*)ctx->slots_l.ptr + used * eb, (size_t)stride, 0,
+                                 (size_t)((Kp - used) * eb), (size_t)KL, ctx->stream));
+    KPM_HIP_TRY(hipMemset2DAsync((char*)ctx->slots_r.ptr + used * eb, (size_t)stride, 0,
+                                 (size_t)((Kp - used) * eb), (size_t)KR, ctx->stream));
+  }
+  kpm_block rr[3], jpsi0r;
+  for (int i = 0; i < 3; ++i) KPM_TRY(work_block(ctx, ctx->ring[i], NH, NR, prec, &rr[i]));
+  KPM_TRY(work_block(ctx, ctx->ring[3], NH, NR, prec, &jpsi0r));
+  std::vector<kpm_block> L(KL), R(KR);
+  for (int64_t i = 0; i < KL; ++i) slot_view(ctx->slots_l.ptr, stride, i, NH, NR, prec, ctx, &L[i]);
+  for (int64_t i = 0; i < KR; ++i) slot_view(ctx->slots_r.ptr, stride, i, NH, NR, prec, ctx, &R[i]);
+
+  KPM_HIP_TRY(hipMemsetAsync(mu_dev, 0, (size_t)(NC * NC) * sizeof(double2), ctx->stream));
+  // left starter (:1013-1018) and right starter (:1021)
+  KPM_TRY(kpm_block_copy(&L[0], psi_l));
+  KPM_TRY(spmm(ctx, H, psi_l, nullptr, &L[1], 1.0, 0.0));
+  KPM_TRY(spmm(ctx, Ja, psi_r, nullptr, &jpsi0r, 1.0, 0.0));
+
+  const int64_t reps = (NC + KL - 1) / KL;
+  const double scale = 1.0 / (double)NR;
+  for (int64_t rep = 1; rep <= reps; ++rep) {
+    const int64_t m1 = (rep - 1) * KL + 1;
+    const int64_t m2 = std::min(rep * KL, NC);
+    const int64_t rep_size = m2 - m1 + 1;
+    for (int64_t i = 3; i <= rep_size; ++i)  // chebyshev_iter(H, L, rep_size)
+      KPM_TRY(spmm(ctx, H, &L[i - 2], &L[i - 3], &L[i - 1], 2.0, -1.0));
+    const int64_t nmax = p.mn_sym ? m2 : NC;
+    for (int64_t n1 = 1; n1 <= nmax; n1 += KR) {
+      const int64_t n2 = std::min(n1 + KR - 1, nmax);
+      const int64_t n_count = n2 - n1 + 1;
+      for (int64_t j = 1; j <= n_count; ++j) {
+        const int64_t n = n1 + j - 1;
+        const int ri = (int)((n - 1) % 3), rip = (int)((n + 1) % 3), ripp = (int)(n % 3);
+        if (n == 1) {
+          KPM_TRY(kpm_block_copy(&rr[0], &jpsi0r));
+        } else if (n == 2) {
+          KPM_TRY(spmm(ctx, H, &jpsi0r, nullptr, &rr[1], 1.0, 0.0));
+        } else {
+          KPM_TRY(spmm(ctx, H, &rr[rip], &rr[ripp], &rr[ri], 2.0, -1.0));
+        }
+        KPM_TRY(spmm(ctx, Jb, &rr[ri], nullptr, &R[j - 1], 1.0, 0.0));
+      }
+      KPM_TRY(launch_contract(ctx, ctx->slots_l.ptr, (int)rep_size, ctx->slots_r.ptr, (int)n_count,
+                              Kp, prec, p.gemm_mode, scale, mu_dev, NC, n1 - 1, m1 - 1, p.parity));
+    }
+    if (rep < reps) {  // chebyshev_iter_wrap(H, L, K_left)
+      KPM_TRY(spmm(ctx, H, &L[KL - 1], &L[KL - 2], &L[0], 2.0, -1.0));
+      KPM_TRY(spmm(ctx, H, &L[0], &L[KL - 1], &L[1], 2.0, -1.0));
+    }
+  }
+  return KPM_OK;
+}
+
+static void apply_mn_sym(std::vector<double>& mu, int64_t NC) {  // :1078-1086
+  for (int64_t m = 0; m < NC; ++m)
+    for (int64_t n = m + 1; n < NC; ++n) {
+      const int64_t a = m + NC * n, b = n + NC * m;
+      mu[2 * a + 1] = 0.0;
+      mu[2 * b] = mu[2 * a];
+      mu[2 * b + 1] = 0.0;
+    }
+}
+
+static int validate_2d(const kpm_csr* H, const kpm_csr* Ja, const kpm_csr* Jb, const Plan2d& p) {
+  if (!H || !Ja || !Jb) return fail(KPM_EINVAL, "null operator");
+  if (p.parity < 0 || p.parity > 2) return fail(KPM_EINVAL, "moment_parity not understood.");
+  if (p.NH != H->n || Ja->n != p.NH || Jb->n != p.NH)
+    return fail(KPM_EINVAL, "operator dimensions do not match NH");
+  if (p.NR <= 0 || p.NC < 1) return fail(KPM_EINVAL, "NR and NC must be positive");
+  if (p.K_left < 2) return fail(KPM_EINVAL, "ψall_l must have at least two slots");
+  if (p.K_right < 1) return fail(KPM_EINVAL, "JTnHJψr must have at least one slot");
+  if (p.K_left > INT32_MAX || p.K_right > INT32_MAX) return fail(KPM_EINVAL, "block too large");
+  if (p.prec != KPM_VEC_F64 && p.prec != KPM_VEC_F32) return fail(KPM_EINVAL, "bad vec_prec");
+  if (p.gemm_mode != KPM_GEMM_F64 && p.gemm_mode != KPM_GEMM_F32) return fail(KPM_EINVAL, "bad gemm_mode");
+  return KPM_OK;
+}
+
+static int64_t fld(int64_t a, int64_t b) {
+  int64_t q = a / b;
+  if ((a % b != 0) && ((a < 0) != (b < 0))) --q;
+  return q;
+}
+
+}  // namespace kpm
+
+using namespace kpm;
+
+extern "C" {
+
+int kpm_2d_blocking(kpm_ctx* ctx, int64_t NH, int64_t NR, int64_t NC, int64_t budget,
+                    int64_t arr_size, int64_t right_block, int64_t* K_left, int64_t* K_right,
+                    int64_t* bytes) {
+  if (!K_left || !K_right || !bytes) return fail(KPM_EINVAL, "null output");
+  if (NH <= 0) return fail(KPM_EINVAL, "NH must be positive (got " + std::to_string(NH) + ")");
+  if (NR <= 0) return fail(KPM_EINVAL, "NR must be positive (got " + std::to_string(NR) + ")");
+  if (NC < 2) return fail(KPM_EINVAL, "NC must be at least 2 (got " + std::to_string(NC) + ")");
+  const int64_t block_bytes = NH * NR * 16;
+  if (budget < 0) {  // :auto -> 50% of free device memory (:741)
+    if (!ctx) return fail(KPM_EINVAL, "auto budget needs a context");
+    size_t f = 0, t = 0;
+    KPM_HIP_TRY(hipSetDevice(ctx->device));
+    KPM_HIP_TRY(hipMemGetInfo(&f, &t));
+    budget = (int64_t)(f / 2);
+  }
+  const int64_t KR = right_block <= 0 ? std::min<int64_t>(NC, 16) : right_block;
+  const int64_t base = (KR + 7) * block_bytes;
+  const int64_t per_left = block_bytes + KR * 16;
+  int64_t KL;
+  if (arr_size <= 0) {
+    const int64_t kmax = fld(budget - base, per_left);
+    if (kmax < 2)
+      return fail(KPM_EINVAL, "workspace_bytes=" + std::to_string(budget) +
+                                  " cannot fit the minimum kpm_2d workspace (" +
+                                  std::to_string(base + 2 * per_left) + " bytes at right_block=" +
+                                  std::to_string(KR) + ")");
+    KL = std::min(kmax, NC);
+  } else {
+    if (arr_size < 2) return fail(KPM_EINVAL, "arr_size must be :auto or an integer at least 2");
+    KL = arr_size;
+  }
+  *K_left = KL;
+  *K_right = KR;
+  *bytes = base + KL * per_left;
+  return KPM_OK;
+}
+
+int kpm_2d(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, const kpm_csr* Jb, int64_t NC,
+           int64_t NR, int64_t NH, const double* psi_l, const double* psi_r, int64_t K_left,
+           int64_t K_right, int parity, int mn_sym, int vec_prec, int gemm_mode, double* mu) {
+  if (!ctx || !psi_l || !psi_r || !mu) return fail(KPM_EINVAL, "null argument");
+  Plan2d p{NC, NR, NH, K_left, K_right, parity, mn_sym, vec_prec, gemm_mode};
+  KPM_TRY(validate_2d(H, Ja, Jb, p));
+  KPM_HIP_TRY(hipSetDevice(ctx->device));
+  kpm_block l, r;
+  KPM_TRY(work_block(ctx, ctx->lwork, NH, NR, vec_prec, &l));
+  KPM_TRY(work_block(ctx, ctx->work[0], NH, NR, vec_prec, &r));
+  KPM_TRY(ensure(ctx, ctx->staging, (size_t)(NH * NR) * 16));
+  KPM_HIP_TRY(hipMemcpyAsync(ctx->staging.ptr, psi_l, (size_t)(NH * NR) * 16, hipMemcpyHostToDevice, ctx->stream));
+  KPM_TRY(launch_upload(ctx, ctx->staging.ptr, &l));
+  KPM_HIP_TRY(hipMemcpyAsync(ctx->staging.ptr, psi_r, (size_t)(NH * NR) * 16, hipMemcpyHostToDevice, ctx->stream));
+  KPM_TRY(launch_upload(ctx, ctx->staging.ptr, &r));
+  KPM_TRY(ensure(ctx, ctx->mu, (size_t)(NC * NC) * sizeof(double2)));
+  KPM_TRY(run_2d(ctx, H, Ja, Jb, p, &l, &r, ctx->mu.ptr));
+  KPM_HIP_TRY(hipMemcpyAsync(mu, ctx->mu.ptr, (size_t)(NC * NC) * sizeof(double2), hipMemcpyDeviceToHost, ctx->stream));
+  KPM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (mn_sym) {
+    std::vector<double> m(mu, mu + 2 * NC * NC);
+    apply_mn_sym(m, NC);
+    std::copy(m.begin(), m.end(), mu);
+  }
+  return KPM_OK;
+}
+
+int kpm_3d(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, const kpm_csr* Jb, const kpm_csr* Jg,
+           int64_t NC, int64_t NR, int64_t NH, const double* psi_l, const double* psi_r,
+           int64_t K_left, int64_t K_right, int vec_prec, int gemm_mode, double* mu) {
+  if (!ctx || !psi_l || !psi_r || !mu || !Jg) return fail(KPM_EINVAL, "null argument");
+  Plan2d p{NC, NR, NH, K_left, K_right, KPM_PARITY_NONE, 0, vec_prec, gemm_mode};
+  KPM_TRY(validate_2d(H, Ja, Jb, p));
+  if (Jg->n != NH) return fail(KPM_EINVAL, "operator dimensions do not match NH");
+  KPM_HIP_TRY(hipSetDevice(ctx->device));
+  kpm_block l, r, outer[3];
+  KPM_TRY(work_block(ctx, ctx->lwork, NH, NR, vec_prec, &l));
+  KPM_TRY(work_block(ctx, ctx->work[0], NH, NR, vec_prec, &r));
+  for (int i = 0; i < 3; ++i) KPM_TRY(work_block(ctx, ctx->work[1 + i], NH, NR, vec_prec, &outer[i]));
+  KPM_TRY(ensure(ctx, ctx->staging, (size_t)(NH * NR) * 16));
+  KPM_HIP_TRY(hipMemcpyAsync(ctx->staging.ptr, psi_l, (size_t)(NH * NR) * 16, hipMemcpyHostToDevice, ctx->stream));
+  KPM_TRY(launch_upload(ctx, ctx->staging.ptr, &l));
+  KPM_HIP_TRY(hipMemcpyAsync(ctx->staging.ptr, psi_r, (size_t)(NH * NR) * 16, hipMemcpyHostToDevice, ctx->stream));
+  KPM_TRY(launch_upload(ctx, ctx->staging.ptr, &r));
+  KPM_TRY(ensure(ctx, ctx->mu, (size_t)(NC * NC) * sizeof(double2)));
+  const size_t slab = (size_t)(NC * NC);
+  for (int64_t n1 = 1; n1 <= NC; ++n1) {
+    const int ri = (int)((n1 - 1) % 3), rip = (int)((n1 + 1) % 3), ripp = (int)(n1 % 3);
+    if (n1 == 1) {
+      KPM_TRY(spmm(ctx, Ja, &r, nullptr, &outer[0], 1.0, 0.0));
+    } else if (n1 == 2) {
+      KPM_TRY(spmm(ctx, H, &outer[rip], nullptr, &outer[1], 1.0, 0.0));
+    } else {
+      KPM_TRY(spmm(ctx, H, &outer[rip], &outer[ripp], &outer[ri], 2.0, -1.0));
+    }
+    // kpm_2d!(H, Jg, Jb, ..., psi_l, T_{n1-1} Ja psi_r) into mu[:,:,n1]
+    KPM_TRY(run_2d(ctx, H, Jg, Jb, p, &l, &outer[ri], ctx->mu.ptr));
+    KPM_HIP_TRY(hipMemcpyAsync(mu + 2 * slab * (size_t)(n1 - 1), ctx->mu.ptr, slab * sizeof(double2),
+                               hipMemcpyDeviceToHost, ctx->stream));
+  }
+  KPM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return KPM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,197 @@
+// gemm.hip -- the kpm_2d moment-table contraction on the MFMA cores.
+//
+// Replaces the ZGEMM mu_b = R^H L / NR of _kpm_2d_blocked!
+// (src/moment.jl:1055-1061) and the host accumulation mu[n,m] += conj(mu_b)
+// with the parity mask (src/moment.jl:1062-1072):
+//
+//   mu[n0+n, m0+m] += scale * sum_k conj(L_m[k]) R_n[k]      (k over NH*NRp)
+//
+// Slots are contiguous recurrence blocks (row-interleaved NH x NRp complex),
+// so a slot is a dense vector of K complex = 2K reals; the padded probe
+// columns are zero in every slot and add nothing.  The complex product runs
+// as two real GEMMs over the interleaved reals:
+//   Re = sum_k' R'[n,k'] L'[m,k']            L' = (Lr, Li) interleaved
+//   Im = sum_k' R'[n,k'] L''[m,k']           L'' = (-Li, Lr)
+// L'' is formed in registers from the pair partner, so each operand is read
+// once.  Each lane reads 8 consecutive reals of its row per 32-real step and
+// feeds them to 8 MFMAs (the k order inside a step is a fixed permutation,
+// identical for both operands).
+//
+// Instructions: v_mfma_f64_16x16x4f64 (FP64 mode) and v_mfma_f32_16x16x4f32
+// (FP32 mode, FP32 accumulation inside one K split).  Split-K partials are
+// stored in FP64 and reduced in a fixed order, then scaled, masked and added.
+//
+// Operand layout of a 16x16x4 MFMA (wave64): lane l supplies A[l%16][l/16]
+// and B[l/16][l%16]; it receives D[4*(l/16)+v][l%16] in accumulator element v.
+#include "kpm_internal.h"
+
+#include <algorithm>
+
+namespace kpm {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename MT>
+struct Mfma;
+
+template <>
+struct Mfma<double> {
+  using acc_t = d4;
+  __device__ static inline acc_t zero() { return acc_t{0.0, 0.0, 0.0, 0.0}; }
+  __device__ static inline acc_t mac(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+
+template <>
+struct Mfma<float> {
+  using acc_t = f4;
+  __device__ static inline acc_t zero() { return acc_t{0.f, 0.f, 0.f, 0.f}; }
+  __device__ static inline acc_t mac(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int kStep = 32;  // reals of K per iteration (8 per lane group of 16)
+
+// Workgroup tile: 64 (n) x 64 (m), four waves of 32 x 32 (2 x 2 MFMA tiles).
+// VT: storage real type of the slots; MT: MFMA arithmetic type.
+template <typename VT, typename MT>
+__global__ void __launch_bounds__(256)
+    k_contract(const VT* __restrict__ L, int nL, const VT* __restrict__ R, int nR, int64_t K2,
+               int64_t kchunk, double2* __restrict__ part, int ldn, int ldm) {
+  using MF = Mfma<MT>;
+  using acc_t = typename MF::acc_t;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int g = lane >> 4;
+  const int i = lane & 15;
+  const int n0 = blockIdx.y * 64 + (wave >> 1) * 32;
+  const int m0 = blockIdx.x * 64 + (wave & 1) * 32;
+  const int64_t kbeg = (int64_t)blockIdx.z * kchunk;
+  const int64_t kend = min(kbeg + kchunk, K2);
+
+  int na[2], mb[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    na[t] = min(n0 + 16 * t + i, nR - 1);
+    mb[t] = min(m0 + 16 * t + i, nL - 1);
+  }
+  acc_t are[2][2], aim[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) are[a][b] = aim[a][b] = MF::zero();
+
+  for (int64_t k = kbeg; k < kend; k += kStep) {
+    MT av[2][8], bv[2][8];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const VT* pa = R + (size_t)na[t] * K2 + k + 8 * g;
+      const VT* pb = L + (size_t)mb[t] * K2 + k + 8 * g;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        av[t][j] = (MT)pa[j];
+        bv[t][j] = (MT)pb[j];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+#pragma unroll
+      for (int tm = 0; tm < 2; ++tm) {
+        const MT bre = bv[tm][j];
+        const MT bim = (j & 1) ? bv[tm][j - 1] : -bv[tm][j + 1];
+#pragma unroll
+        for (int tn = 0; tn < 2; ++tn) {
+          are[tn][tm] = MF::mac(av[tn][j], bre, are[tn][tm]);
+          aim[tn][tm] = MF::mac(av[tn][j], bim, aim[tn][tm]);
+        }
+      }
+    }
+  }
+  double2* out = part + (size_t)blockIdx.z * ldn * ldm;
+#pragma unroll
+  for (int tn = 0; tn < 2; ++tn)
+#pragma unroll
+    for (int tm = 0; tm < 2; ++tm)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int n = n0 + 16 * tn + 4 * g + v;
+        const int m = m0 + 16 * tm + i;
+        if (n < nR && m < nL)
+          out[(size_t)n * ldm + m] = make_double2((double)are[tn][tm][v], (double)aim[tn][tm][v]);
+      }
+}
+
+__global__ void k_contract_reduce(const double2* __restrict__ part, int splits, int nR, int nL,
+                                  int ldn, int ldm, double scale, double2* __restrict__ mu,
+                                  int64_t NC, int64_t n0, int64_t m0, int parity) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)nR * nL) return;
+  const int n = (int)(idx / nL);
+  const int m = (int)(idx % nL);
+  // reference indices are 1-based: n_ref + m_ref = (n0+n+1) + (m0+m+1)
+  const int64_t s = n0 + n + m0 + m + 2;
+  const bool keep = parity == KPM_PARITY_NONE || (parity == KPM_PARITY_ODD && (s & 1)) ||
+                    (parity == KPM_PARITY_EVEN && !(s & 1));
+  if (!keep) return;
+  double sr = 0.0, si = 0.0;
+  for (int z = 0; z < splits; ++z) {
+    const double2 v = part[((size_t)z * ldn + n) * ldm + m];
+    sr += v.x;
+    si += v.y;
+  }
+  double2* dst = mu + (n0 + n) + NC * (m0 + m);
+  double2 cur = *dst;
+  cur.x += scale * sr;
+  cur.y += scale * si;
+  *dst = cur;
+}
+
+int launch_contract(kpm_ctx* ctx, const void* Lslots, int nL, const void* Rslots, int nR,
+                    int64_t K, int prec, int gemm_mode, double scale, void* mu_dev, int64_t NC,
+                    int64_t n0, int64_t m0, int parity) {
+  const int64_t K2 = 2 * K;  // reals per slot (slot stride, a multiple of kStep)
+  if (K2 % kStep != 0) return fail(KPM_EINVAL, "contraction slot length must be a multiple of 16");
+  const int tn = (nR + 63) / 64;
+  const int tm = (nL + 63) / 64;
+  const int64_t steps = K2 / kStep;
+  int64_t splits = std::max<int64_t>(1, (int64_t)(4 * std::max(ctx->num_cus, 1)) / (tn * tm));
+  splits = std::min<int64_t>(splits, std::max<int64_t>(1, steps / 16));
+  splits = std::min<int64_t>(splits, 1024);
+  const int64_t steps_per = (steps + splits - 1) / splits;
+  const int64_t kchunk = steps_per * kStep;
+  splits = (K2 + kchunk - 1) / kchunk;
+  const int ldn = nR, ldm = nL;
+  KPM_TRY(ensure(ctx, ctx->gemm_part, (size_t)splits * ldn * ldm * sizeof(double2)));
+  dim3 grid(tm, tn, (unsigned)splits);
+  if (prec == KPM_VEC_F64) {
+    if (gemm_mode == KPM_GEMM_F32)
+      hipLaunchKernelGGL((k_contract<double, float>), grid, dim3(256), 0, ctx->stream,
+                         (const double*)Lslots, nL, (const double*)Rslots, nR, K2, kchunk,
+                         (double2*)ctx->gemm_part.ptr, ldn, ldm);
+    else
+      hipLaunchKernelGGL((k_contract<double, double>), grid, dim3(256), 0, ctx->stream,
+                         (const double*)Lslots, nL, (const double*)Rslots, nR, K2, kchunk,
+                         (double2*)ctx->gemm_part.ptr, ldn, ldm);
+  } else {
+    if (gemm_mode == KPM_GEMM_F32)
+      hipLaunchKernelGGL((k_contract<float, float>), grid, dim3(256), 0, ctx->stream,
+                         (const float*)Lslots, nL, (const float*)Rslots, nR, K2, kchunk,
+                         (double2*)ctx->gemm_part.ptr, ldn, ldm);
+    else
+      hipLaunchKernelGGL((k_contract<float, double>), grid, dim3(256), 0, ctx->stream,
+                         (const float*)Lslots, nL, (const float*)Rslots, nR, K2, kchunk,
+                         (double2*)ctx->gemm_part.ptr, ldn, ldm);
+  }
+  KPM_HIP_TRY(hipGetLastError());
+  const int64_t total = (int64_t)nR * nL;
+  hipLaunchKernelGGL(k_contract_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     ctx->stream, (const double2*)ctx->gemm_part.ptr, (int)splits, nR, nL, ldn,
+                     ldm, scale, (double2*)mu_dev, NC, n0, m0, parity);
+  KPM_HIP_TRY(hipGetLastError());
+  return KPM_OK;
+}
+
+}  // namespace kpm

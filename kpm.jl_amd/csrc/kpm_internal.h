@@ -1,0 +1,157 @@
+// kpm_internal.h -- shared declarations of libkpm_hip.so (gfx950 only).
+//
+// Device data layout (HBM)
+//   recurrence block  NH rows x NRp columns, ROW-interleaved: element (i, r)
+//                     at i*NRp + r.  One row of a block is the NRp probe
+//                     values of site i, so a CSR gather of neighbour j reads
+//                     one contiguous NRp*elem-byte segment.  FP64 blocks hold
+//                     double2 (re, im), FP32 blocks float2.  NRp >= NR is NR
+//                     rounded up so a row is a power-of-two number (<= 64)
+//                     of 16-byte words, or a multiple of 64 words; padded
+//                     columns hold zeros and stay zero under the recurrence.
+//   operator          CSR, int32 row pointers / column indices, values
+//                     complex FP64 (double2) or real FP64 when Im == 0.
+//   moments           NR x NC column-major double2 on the device, copied
+//                     to the host once per engine call.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/kpm_hip.h"
+
+namespace kpm {
+
+// ---- error plumbing -------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define KPM_HIP_TRY(expr)                                                          \
+  do {                                                                             \
+    hipError_t _e = (expr);                                                        \
+    if (_e != hipSuccess)                                                          \
+      return ::kpm::fail(KPM_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+#define KPM_TRY(expr)        \
+  do {                       \
+    int _rc = (expr);        \
+    if (_rc != KPM_OK) return _rc; \
+  } while (0)
+
+// ---- row geometry ----------------------------------------------------------
+// A 16-byte word holds one complex double (FP64) or two complex floats (FP32).
+inline int cplx_per_word(int prec) { return prec == KPM_VEC_F32 ? 2 : 1; }
+inline int elem_bytes(int prec) { return prec == KPM_VEC_F32 ? 8 : 16; }
+// words per row for NR probe columns
+int row_words(int64_t NR, int prec);
+inline int64_t padded_nr(int64_t NR, int prec) { return (int64_t)row_words(NR, prec) * cplx_per_word(prec); }
+
+// ---- workspace cache ---------------------------------------------------------
+struct Buffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace kpm
+
+struct kpm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t events[16] = {};
+  int num_cus = 0;
+  // grow-only device scratch, keyed by role
+  kpm::Buffer partials;   // per-block dot partial sums
+  kpm::Buffer mu;         // device moment accumulator
+  kpm::Buffer staging;    // host-layout block for upload/download
+  kpm::Buffer gemm_part;  // split-K partials of the kpm_2d contraction
+  kpm::Buffer work[4];    // engine-owned recurrence blocks
+  kpm::Buffer lwork;      // bra block of the bra/ket engine
+  kpm::Buffer slots_l;    // kpm_2d left slot array
+  kpm::Buffer slots_r;    // kpm_2d right slot array
+  kpm::Buffer ring[5];    // kpm_2d / kpm_3d ring and starter blocks
+};
+
+struct kpm_csr {
+  kpm_ctx* ctx = nullptr;
+  int64_t n = 0;
+  int64_t nnz = 0;
+  int is_real = 0;
+  int max_row_nnz = 0;
+  int* rowptr = nullptr;  // n+1
+  int* colind = nullptr;  // nnz
+  void* vals = nullptr;   // nnz x (double2 | double)
+};
+
+struct kpm_block {
+  kpm_ctx* ctx = nullptr;
+  int64_t NH = 0;
+  int64_t NR = 0;
+  int64_t NRp = 0;
+  int prec = KPM_VEC_F64;
+  void* d = nullptr;
+  bool owned = true;
+};
+
+namespace kpm {
+
+int ensure(kpm_ctx* ctx, Buffer& b, size_t bytes);
+// A non-owning block view over a cached workspace buffer.
+int work_block(kpm_ctx* ctx, Buffer& buf, int64_t NH, int64_t NR, int prec, kpm_block* out);
+
+// ---- kernels (spmm.hip) -------------------------------------------------------
+enum StepMode { MODE_NONE = 0, MODE_DBL = 1, MODE_LR = 2 };
+
+// One fused pass out <- alpha*A*x + beta*z with optional dot partials:
+//   MODE_DBL: d0 = <x|x>, d1 = <x|out>;  MODE_LR: d0 = <l|out>.
+// z may alias out (in place) and may be null when beta == 0.
+struct StepLaunch {
+  const kpm_csr* A;
+  const kpm_block* x;
+  const kpm_block* z;
+  kpm_block* out;
+  const kpm_block* l;
+  double alpha, beta;
+  int mode;
+};
+// Launch geometry picked by the tuner for (LPR, rows per lane, grid size).
+struct StepConfig {
+  int unroll = 2;     // rows per lane per tile
+  int grid = 0;       // workgroups (multiple of 8); 0 = auto
+};
+int launch_step(kpm_ctx* ctx, const StepLaunch& s, const StepConfig& cfg, int* grid_used);
+// dot-only pass: d0 = <a|b> per column, partials into ctx->partials
+int launch_dot(kpm_ctx* ctx, const kpm_block* a, const kpm_block* b, int* grid_used);
+// Reduce partials [grid][nd][NRp] in a fixed order and write moment columns:
+//   mu[:, tcol[d]] = alpha[d]*sum_d + bscale[d]*mu[:, bcol[d]]  (bcol < 0: none)
+struct Combine {
+  int nd;
+  int tcol[2];
+  double alpha[2];
+  int bcol[2];
+  double bscale[2];
+};
+int launch_combine(kpm_ctx* ctx, int grid, int64_t NR, int64_t NRp, const Combine& c,
+                   void* mu_dev, int64_t NCmu);
+// layout conversions between the host column-major ComplexF64 block (in the
+// staging buffer, already on the device) and a device recurrence block
+int launch_upload(kpm_ctx* ctx, const void* staging, kpm_block* b);
+int launch_download(kpm_ctx* ctx, const kpm_block* b, void* staging);
+
+// ---- contraction (gemm.hip) -----------------------------------------------------
+// mu_blk[n][m] (n < nR, m < nL) = scale * sum_k conj(L_m[k]) * R_n[k], over
+// K = NH*NRp complex entries per slot, accumulated into the host-layout
+// table mu (NC x NC column-major device double2) at (n0 + n, m0 + m) with the
+// parity mask (0 none, 1 keep odd n+m, 2 keep even; 1-based n, m as in the
+// reference).
+int launch_contract(kpm_ctx* ctx, const void* Lslots, int nL, const void* Rslots, int nR,
+                    int64_t K, int prec, int gemm_mode, double scale, void* mu_dev, int64_t NC,
+                    int64_t n0, int64_t m0, int parity);
+
+// tuner hook (tune.cpp): best StepConfig for a shape, or defaults
+StepConfig tuned_config(kpm_ctx* ctx, const kpm_csr* A, int64_t NR, int prec, int mode);
+
+}  // namespace kpm

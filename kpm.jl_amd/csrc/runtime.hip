@@ -1,0 +1,396 @@
+// runtime.hip -- contexts, operators, blocks, errors, timing.
+//
+// This is the device layer of the replacement for ext/KPMCUDAExt.jl:
+//   CUDADevice + __init__ activation          ext/KPMCUDAExt.jl:13-20
+//   to_device(::SparseMatrixCSC) -> CSR         ext/KPMCUDAExt.jl:26-27
+//   to_device / maybe_to_host of dense blocks   ext/KPMCUDAExt.jl:28,36-39
+//   device_zeros / device_free_memory           ext/KPMCUDAExt.jl:43-45
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kpm_internal.h"
+
+namespace kpm {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int ensure(kpm_ctx* ctx, Buffer& b, size_t bytes) {
+  if (b.bytes >= bytes && b.ptr) return KPM_OK;
+  if (b.ptr) {
+    KPM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    KPM_HIP_TRY(hipFree(b.ptr));
+    b.ptr = nullptr;
+    b.bytes = 0;
+  }
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(&b.ptr, bytes);
+  if (e != hipSuccess) {
+    b.ptr = nullptr;
+    return fail(KPM_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
+                                hipGetErrorString(e));
+  }
+  b.bytes = bytes;
+  return KPM_OK;
+}
+
+int work_block(kpm_ctx* ctx, Buffer& buf, int64_t NH, int64_t NR, int prec, kpm_block* out) {
+  const int64_t NRp = padded_nr(NR, prec);
+  KPM_TRY(ensure(ctx, buf, (size_t)(NH * NRp) * elem_bytes(prec)));
+  out->ctx = ctx;
+  out->NH = NH;
+  out->NR = NR;
+  out->NRp = NRp;
+  out->prec = prec;
+  out->d = buf.ptr;
+  out->owned = false;
+  return KPM_OK;
+}
+
+static void free_buffer(Buffer& b) {
+  if (b.ptr) (void)hipFree(b.ptr);
+  b.ptr = nullptr;
+  b.bytes = 0;
+}
+
+// Build device CSR arrays from host (rowptr, colind, vals) already in CSR
+// form with 0-based int64 indices.
+static int upload_csr(kpm_ctx* ctx, int64_t n, const std::vector<int64_t>& rp,
+                      const std::vector<int64_t>& ci, const std::vector<double>& v_cplx,
+                      kpm_csr** out) {
+  const int64_t nnz = rp[n];
+  if (n >= INT32_MAX || nnz >= INT32_MAX)
+    return fail(KPM_EINVAL, "operator too large for int32 device indices (n or nnz >= 2^31)");
+  bool is_real = true;
+  for (int64_t k = 0; k < nnz; ++k)
+    if (v_cplx[2 * k + 1] != 0.0) {
+      is_real = false;
+      break;
+    }
+  std::vector<int> rp32(n + 1), ci32(std::max<int64_t>(nnz, 1));
+  int maxrow = 0;
+  for (int64_t i = 0; i <= n; ++i) rp32[i] = (int)rp[i];
+  for (int64_t i = 0; i < n; ++i) maxrow = std::max(maxrow, (int)(rp[i + 1] - rp[i]));
+  for (int64_t k = 0; k < nnz; ++k) ci32[k] = (int)ci[k];
+  kpm_csr* A = new kpm_csr();
+  A->ctx = ctx;
+  A->n = n;
+  A->nnz = nnz;
+  A->is_real = is_real ? 1 : 0;
+  A->max_row_nnz = maxrow;
+  const size_t vbytes = (size_t)std::max<int64_t>(nnz, 1) * (is_real ? sizeof(double) : 2 * sizeof(double));
+  hipError_t e = hipMalloc(&A->rowptr, sizeof(int) * (n + 1));
+  if (e == hipSuccess) e = hipMalloc(&A->colind, sizeof(int) * std::max<int64_t>(nnz, 1));
+  if (e == hipSuccess) e = hipMalloc(&A->vals, vbytes);
+  if (e != hipSuccess) {
+    if (A->rowptr) (void)hipFree(A->rowptr);
+    if (A->colind) (void)hipFree(A->colind);
+    if (A->vals) (void)hipFree(A->vals);
+    delete A;
+    return fail(KPM_ENOMEM, std::string("hipMalloc for CSR failed: ") + hipGetErrorString(e));
+  }
+  std::vector<double> vr;
+  const void* vsrc = v_cplx.data();
+  if (is_real) {
+    vr.resize(std::max<int64_t>(nnz, 1));
+    for (int64_t k = 0; k < nnz; ++k) vr[k] = v_cplx[2 * k];
+    vsrc = vr.data();
+  }
+  e = hipMemcpy(A->rowptr, rp32.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess && nnz > 0)
+    e = hipMemcpy(A->colind, ci32.data(), sizeof(int) * nnz, hipMemcpyHostToDevice);
+  if (e == hipSuccess && nnz > 0) e = hipMemcpy(A->vals, vsrc, nnz * (is_real ? 8 : 16), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(A->rowptr);
+    (void)hipFree(A->colind);
+    (void)hipFree(A->vals);
+    delete A;
+    return fail(KPM_EHIP, std::string("hipMemcpy for CSR failed: ") + hipGetErrorString(e));
+  }
+  *out = A;
+  return KPM_OK;
+}
+
+static int check_csc(int64_t n, const int64_t* colptr, const int64_t* rowval, int base) {
+  if (n <= 0) return fail(KPM_EINVAL, "operator dimension must be positive");
+  if (base != 0 && base != 1) return fail(KPM_EINVAL, "index_base must be 0 or 1");
+  if (!colptr) return fail(KPM_EINVAL, "null colptr");
+  if (colptr[0] - base != 0) return fail(KPM_EINVAL, "colptr must start at index_base");
+  const int64_t nnz = colptr[n] - base;
+  if (nnz < 0) return fail(KPM_EINVAL, "negative nnz");
+  for (int64_t j = 0; j < n; ++j)
+    if (colptr[j + 1] < colptr[j]) return fail(KPM_EINVAL, "colptr must be nondecreasing");
+  for (int64_t k = 0; k < nnz; ++k) {
+    const int64_t r = rowval[k] - base;
+    if (r < 0 || r >= n) return fail(KPM_EINVAL, "row index out of range");
+  }
+  return KPM_OK;
+}
+
+}  // namespace kpm
+
+using namespace kpm;
+
+extern "C" {
+
+const char* kpm_last_error(void) { return g_last_error.c_str(); }
+
+int kpm_abi_version(void) { return 1; }
+
+int kpm_device_count(int* count) {
+  if (!count) return fail(KPM_EINVAL, "null count");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) n = 0;
+  *count = n;
+  return KPM_OK;
+}
+
+int kpm_ctx_create(int device, kpm_ctx** out) {
+  if (!out) return fail(KPM_EINVAL, "null output");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+    return fail(KPM_EHIP, "no HIP device is available");
+  if (device < 0 || device >= n) return fail(KPM_EINVAL, "device index out of range");
+  KPM_HIP_TRY(hipSetDevice(device));
+  kpm_ctx* ctx = new kpm_ctx();
+  ctx->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+  if (ctx->num_cus <= 0) ctx->num_cus = 256;
+  hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  for (int i = 0; e == hipSuccess && i < 16; ++i) e = hipEventCreate(&ctx->events[i]);
+  if (e != hipSuccess) {
+    delete ctx;
+    return fail(KPM_EHIP, std::string("context creation failed: ") + hipGetErrorString(e));
+  }
+  *out = ctx;
+  return KPM_OK;
+}
+
+int kpm_ctx_destroy(kpm_ctx* ctx) {
+  if (!ctx) return KPM_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  free_buffer(ctx->partials);
+  free_buffer(ctx->mu);
+  free_buffer(ctx->staging);
+  free_buffer(ctx->gemm_part);
+  for (auto& b : ctx->work) free_buffer(b);
+  free_buffer(ctx->lwork);
+  free_buffer(ctx->slots_l);
+  free_buffer(ctx->slots_r);
+  for (auto& b : ctx->ring) free_buffer(b);
+  for (auto& ev : ctx->events)
+    if (ev) (void)hipEventDestroy(ev);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return KPM_OK;
+}
+
+int kpm_ctx_synchronize(kpm_ctx* ctx) {
+  if (!ctx) return fail(KPM_EINVAL, "null context");
+  KPM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return KPM_OK;
+}
+
+int kpm_device_free_memory(kpm_ctx* ctx, size_t* free_bytes, size_t* total_bytes) {
+  if (!ctx) return fail(KPM_EINVAL, "null context");
+  size_t f = 0, t = 0;
+  KPM_HIP_TRY(hipSetDevice(ctx->device));
+  KPM_HIP_TRY(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return KPM_OK;
+}
+
+int kpm_device_name(kpm_ctx* ctx, char* buf, size_t len) {
+  if (!ctx || !buf || len == 0) return fail(KPM_EINVAL, "bad arguments");
+  hipDeviceProp_t prop;
+  KPM_HIP_TRY(hipGetDeviceProperties(&prop, ctx->device));
+  std::snprintf(buf, len, "%s (%s)", prop.name, prop.gcnArchName);
+  return KPM_OK;
+}
+
+int kpm_csr_from_csc(kpm_ctx* ctx, int64_t n, const int64_t* colptr, const int64_t* rowval,
+                     const double* nzval, int index_base, kpm_csr** out) {
+  if (!ctx || !out) return fail(KPM_EINVAL, "null argument");
+  KPM_TRY(check_csc(n, colptr, rowval, index_base));
+  const int64_t nnz = colptr[n] - index_base;
+  // CSR(A) = transpose of the CSC arrays: count entries per row, then scatter
+  // columns in increasing order so each row's column indices are sorted.
+  std::vector<int64_t> rp(n + 1, 0), ci(std::max<int64_t>(nnz, 1));
+  std::vector<double> v(2 * std::max<int64_t>(nnz, 1));
+  for (int64_t k = 0; k < nnz; ++k) rp[rowval[k] - index_base + 1]++;
+  for (int64_t i = 0; i < n; ++i) rp[i + 1] += rp[i];
+  std::vector<int64_t> next(rp.begin(), rp.end() - 1);
+  for (int64_t j = 0; j < n; ++j)
+    for (int64_t k = colptr[j] - index_base; k < colptr[j + 1] - index_base; ++k) {
+      const int64_t r = rowval[k] - index_base;
+      const int64_t dst = next[r]++;
+      ci[dst] = j;
+      v[2 * dst] = nzval[2 * k];
+      v[2 * dst + 1] = nzval[2 * k + 1];
+    }
+  KPM_HIP_TRY(hipSetDevice(ctx->device));
+  return upload_csr(ctx, n, rp, ci, v, out);
+}
+
+int kpm_csr_adjoint_from_csc(kpm_ctx* ctx, int64_t n, const int64_t* colptr, const int64_t* rowval,
+                             const double* nzval, int index_base, kpm_csr** out) {
+  if (!ctx || !out) return fail(KPM_EINVAL, "null argument");
+  KPM_TRY(check_csc(n, colptr, rowval, index_base));
+  const int64_t nnz = colptr[n] - index_base;
+  // row j of adjoint(A) is column j of A, conjugated
+  std::vector<int64_t> rp(n + 1), ci(std::max<int64_t>(nnz, 1));
+  std::vector<double> v(2 * std::max<int64_t>(nnz, 1));
+  for (int64_t j = 0; j <= n; ++j) rp[j] = colptr[j] - index_base;
+  for (int64_t k = 0; k < nnz; ++k) {
+    ci[k] = rowval[k] - index_base;
+    v[2 * k] = nzval[2 * k];
+    v[2 * k + 1] = -nzval[2 * k + 1];
+  }
+  KPM_HIP_TRY(hipSetDevice(ctx->device));
+  return upload_csr(ctx, n, rp, ci, v, out);
+}
+
+int kpm_csr_from_csr(kpm_ctx* ctx, int64_t n, const int64_t* rowptr, const int64_t* colind,
+                     const double* vals, int index_base, kpm_csr** out) {
+  if (!ctx || !out) return fail(KPM_EINVAL, "null argument");
+  // the CSR arrays of A are the CSC arrays of A^T: validate with the same rules
+  KPM_TRY(check_csc(n, rowptr, colind, index_base));
+  const int64_t nnz = rowptr[n] - index_base;
+  std::vector<int64_t> rp(n + 1), ci(std::max<int64_t>(nnz, 1));
+  std::vector<double> v(2 * std::max<int64_t>(nnz, 1));
+  for (int64_t i = 0; i <= n; ++i) rp[i] = rowptr[i] - index_base;
+  for (int64_t k = 0; k < nnz; ++k) {
+    ci[k] = colind[k] - index_base;
+    v[2 * k] = vals[2 * k];
+    v[2 * k + 1] = vals[2 * k + 1];
+  }
+  KPM_HIP_TRY(hipSetDevice(ctx->device));
+  return upload_csr(ctx, n, rp, ci, v, out);
+}
+
+int kpm_csr_destroy(kpm_csr* A) {
+  if (!A) return KPM_OK;
+  (void)hipSetDevice(A->ctx->device);
+  (void)hipStreamSynchronize(A->ctx->stream);
+  (void)hipFree(A->rowptr);
+  (void)hipFree(A->colind);
+  (void)hipFree(A->vals);
+  delete A;
+  return KPM_OK;
+}
+
+int kpm_csr_info(const kpm_csr* A, int64_t* n, int64_t* nnz, int* is_real) {
+  if (!A) return fail(KPM_EINVAL, "null operator");
+  if (n) *n = A->n;
+  if (nnz) *nnz = A->nnz;
+  if (is_real) *is_real = A->is_real;
+  return KPM_OK;
+}
+
+int kpm_block_create(kpm_ctx* ctx, int64_t NH, int64_t NR, int vec_prec, kpm_block** out) {
+  if (!ctx || !out) return fail(KPM_EINVAL, "null argument");
+  if (NH <= 0 || NR <= 0) return fail(KPM_EINVAL, "block dimensions must be positive");
+  if (vec_prec != KPM_VEC_F64 && vec_prec != KPM_VEC_F32) return fail(KPM_EINVAL, "bad vec_prec");
+  KPM_HIP_TRY(hipSetDevice(ctx->device));
+  kpm_block* b = new kpm_block();
+  b->ctx = ctx;
+  b->NH = NH;
+  b->NR = NR;
+  b->NRp = padded_nr(NR, vec_prec);
+  b->prec = vec_prec;
+  const size_t bytes = (size_t)(NH * b->NRp) * elem_bytes(vec_prec);
+  hipError_t e = hipMalloc(&b->d, bytes);
+  if (e != hipSuccess) {
+    delete b;
+    return fail(KPM_ENOMEM, "hipMalloc of block failed: " + std::string(hipGetErrorString(e)));
+  }
+  e = hipMemsetAsync(b->d, 0, bytes, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(b->d);
+    delete b;
+    return fail(KPM_EHIP, std::string("block clear failed: ") + hipGetErrorString(e));
+  }
+  *out = b;
+  return KPM_OK;
+}
+
+int kpm_block_destroy(kpm_block* b) {
+  if (!b) return KPM_OK;
+  if (b->owned && b->d) {
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    (void)hipFree(b->d);
+  }
+  delete b;
+  return KPM_OK;
+}
+
+int kpm_block_upload(kpm_block* b, const double* host) {
+  if (!b || !host) return fail(KPM_EINVAL, "null argument");
+  kpm_ctx* ctx = b->ctx;
+  KPM_HIP_TRY(hipSetDevice(ctx->device));
+  const size_t hbytes = (size_t)(b->NH * b->NR) * 16;
+  KPM_TRY(ensure(ctx, ctx->staging, hbytes));
+  KPM_HIP_TRY(hipMemcpyAsync(ctx->staging.ptr, host, hbytes, hipMemcpyHostToDevice, ctx->stream));
+  KPM_TRY(launch_upload(ctx, ctx->staging.ptr, b));
+  KPM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return KPM_OK;
+}
+
+int kpm_block_download(const kpm_block* b, double* host) {
+  if (!b || !host) return fail(KPM_EINVAL, "null argument");
+  kpm_ctx* ctx = b->ctx;
+  KPM_HIP_TRY(hipSetDevice(ctx->device));
+  const size_t hbytes = (size_t)(b->NH * b->NR) * 16;
+  KPM_TRY(ensure(ctx, ctx->staging, hbytes));
+  KPM_TRY(launch_download(ctx, b, ctx->staging.ptr));
+  KPM_HIP_TRY(hipMemcpyAsync(host, ctx->staging.ptr, hbytes, hipMemcpyDeviceToHost, ctx->stream));
+  KPM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return KPM_OK;
+}
+
+int kpm_block_copy(kpm_block* dst, const kpm_block* src) {
+  if (!dst || !src) return fail(KPM_EINVAL, "null argument");
+  if (dst->NH != src->NH || dst->NR != src->NR || dst->prec != src->prec)
+    return fail(KPM_EINVAL, "block shapes or precisions differ");
+  if (dst == src || dst->d == src->d) return KPM_OK;
+  kpm_ctx* ctx = dst->ctx;
+  KPM_HIP_TRY(hipMemcpyAsync(dst->d, src->d, (size_t)(dst->NH * dst->NRp) * elem_bytes(dst->prec),
+                             hipMemcpyDeviceToDevice, ctx->stream));
+  return KPM_OK;
+}
+
+int kpm_block_zero(kpm_block* b) {
+  if (!b) return fail(KPM_EINVAL, "null argument");
+  KPM_HIP_TRY(hipMemsetAsync(b->d, 0, (size_t)(b->NH * b->NRp) * elem_bytes(b->prec), b->ctx->stream));
+  return KPM_OK;
+}
+
+int kpm_event_record(kpm_ctx* ctx, int slot) {
+  if (!ctx || slot < 0 || slot >= 16) return fail(KPM_EINVAL, "bad event slot");
+  KPM_HIP_TRY(hipEventRecord(ctx->events[slot], ctx->stream));
+  return KPM_OK;
+}
+
+int kpm_event_elapsed(kpm_ctx* ctx, int slot_a, int slot_b, float* ms) {
+  if (!ctx || !ms || slot_a < 0 || slot_a >= 16 || slot_b < 0 || slot_b >= 16)
+    return fail(KPM_EINVAL, "bad event slot");
+  KPM_HIP_TRY(hipEventSynchronize(ctx->events[slot_b]));
+  KPM_HIP_TRY(hipEventElapsedTime(ms, ctx->events[slot_a], ctx->events[slot_b]));
+  return KPM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,34 @@
+"""kpmhip -- MI355X (gfx950) Chebyshev moment engine behind KPM.jl's moment API.
+
+Compute runs in libkpm_hip.so (hand-written HIP kernels, C ABI in
+include/kpm_hip.h); this package is the host-side mirror of the reference's
+interface: the moment engine (src/moment.jl), its primitives
+(src/utils/chebyshev_iteration.jl, src/utils/vectors.jl), the device layer
+(src/device.jl), the rescaling (src/utils/external.jl), damping kernels
+(src/kernels), the DOS / Kubo–Bastin reconstruction heads and the typed front
+end (src/frontend.jl).
+"""
+from ._lib import ArgumentError, KPMHipError, KPMHipMissing, load_library
+from .applications import (chebyshev_lin_trans, chebyshevT, chebyshevT_0, chebyshevT_xn,
+                           fermiFunction, fermiFunctions, gausschebyshevt)
+from .applications import dos as dos_raw
+from .applications import dos0 as dos0_raw
+from .applications import kubo_bastin_cond as kubo_bastin_cond_raw
+from .device import (ACTIVE_DEVICE, CPUDevice, DeviceBlock, DeviceCSR, HIPDevice, activate,
+                     device_free_memory, get_context, maybe_on_device_zeros, maybe_to_host,
+                     to_device, whichcore)
+from .frontend import (ConductivityMoments, CurrentMoments, DosMoments, GreenMoments,
+                       RescaledHamiltonian, cond_moments, current_moments, dos, dos0, dos_moments,
+                       green_moments, kubo_bastin_cond, ldos_moments, nc, npairs,
+                       random_phase_vectors, rescale)
+from .kernels import (JacksonKernel, LorentzKernels, hn, mu2D_apply_kernel_and_h,
+                      mu3D_apply_kernel_and_h, muND_apply_kernel_and_h)
+from .moment import (engine_blocking, kpm_1d, kpm_1d_, kpm_1d_current, kpm_1d_current_,
+                     kpm_2d, kpm_2d_, kpm_2d_blocking, kpm_3d, kpm_3d_)
+from .primitives import (chebyshev_iter, chebyshev_iter_single, chebyshev_iter_wrap,
+                         columnwise_dot, moment_accumulator, normalize_by_col, r2_i, r2_ip,
+                         r2_ipp, r_i, r_ip, r_ipp, spmm)
+from .rescale import normalizeH
+from .rng import Xoshiro256pp, default_rng
+
+__all__ = [n for n in dir() if not n.startswith("_")]

@@ -1,0 +1,251 @@
+"""Host FP64 reconstruction heads that consume the moments: the DOS
+(src/applications/dos.jl:76-199 with src/utils/chebyshev.jl:7-39 and
+src/utils/chebyshev_lintrans.jl:6-27) and the Kubo–Bastin conductivity
+(src/applications/conductivity.jl:101-157), plus the Fermi functions
+(src/utils/physics.jl:15-35).
+"""
+from __future__ import annotations
+
+import math
+import warnings
+
+import numpy as np
+
+from ._lib import ArgumentError
+from .kernels import JacksonKernel, hn, mu2D_apply_kernel_and_h
+
+
+def chebyshevT(n, x):
+    """T_n(x) = cos(n acos x) (src/utils/chebyshev.jl:7)."""
+    return np.cos(np.multiply(n, np.arccos(x)))
+
+
+def chebyshevT_0(n):
+    """T_n(0): 0 odd, +1 n ≡ 0 (mod 4), -1 n ≡ 2 (mod 4) (src/utils/chebyshev.jl:10)."""
+    n = np.asarray(n)
+    return ((n + 1) % 2) * (2 - (n + 1) % 4)
+
+
+def chebyshevT_xn(x_grid, n_grid):
+    """T[x, n] matrix (src/utils/chebyshev.jl:33-39)."""
+    x = np.atleast_1d(np.asarray(x_grid, dtype=np.float64))
+    n = np.asarray(n_grid)
+    return np.cos(n[None, :] * np.arccos(x)[:, None])
+
+
+def chebyshev_lin_trans(x_grid, n_grid, mu_tilde):
+    """Σ_n μ̃_n T_n(x) on a point or grid (src/utils/chebyshev_lintrans.jl:6-27)."""
+    mu_tilde = np.asarray(mu_tilde)
+    if np.ndim(x_grid) == 0:
+        return np.sum(chebyshevT_xn(x_grid, n_grid) @ mu_tilde)
+    x = np.asarray(x_grid, dtype=np.float64)
+    Nx, Nn = len(x), len(n_grid)
+    if Nx * Nn / 65536 < 1000:
+        return chebyshevT_xn(x, n_grid) @ mu_tilde
+    y = np.zeros(Nx, dtype=np.result_type(mu_tilde, np.complex128))
+    for i in range(Nx):
+        y[i] = np.dot(chebyshevT(np.asarray(n_grid), x[i]), mu_tilde)
+    return y
+
+
+# ---- truncated Taylor arithmetic for dE_order derivatives ---------------------
+# dos.jl:134-145 takes nested ForwardDiff derivatives of _dos_single; the same
+# derivatives come exactly from Taylor coefficients in E (x = (E-b)/a is
+# linear in E), with T_n built by the three-term recurrence on series.
+
+
+def _tmul(a, b):
+    k = len(a)
+    out = np.zeros(k, dtype=np.result_type(a, b))
+    for i in range(k):
+        out[i] = np.dot(a[: i + 1], b[i::-1])
+    return out
+
+
+def _tsqrt(a):
+    k = len(a)
+    out = np.zeros(k, dtype=np.result_type(a, np.float64))
+    out[0] = math.sqrt(a[0])
+    for i in range(1, k):
+        s = a[i] - np.dot(out[1:i], out[i - 1:0:-1])
+        out[i] = s / (2 * out[0])
+    return out
+
+
+def _tdiv(a, b):
+    k = len(a)
+    out = np.zeros(k, dtype=np.result_type(a, b))
+    for i in range(k):
+        s = a[i] - np.dot(out[:i], b[i:0:-1])
+        out[i] = s / b[0]
+    return out
+
+
+def _dos_taylor(mu_tilde, a, b, E, order):
+    k = order + 1
+    x = np.zeros(k)
+    x[0] = (E - b) / a
+    if k > 1:
+        x[1] = 1.0 / a
+    t_prev = np.zeros(k)
+    t_prev[0] = 1.0
+    acc = mu_tilde[0] * t_prev.astype(np.result_type(mu_tilde, np.float64))
+    if len(mu_tilde) > 1:
+        t_cur = x.copy()
+        acc = acc + mu_tilde[1] * t_cur
+        for n in range(2, len(mu_tilde)):
+            t_next = 2.0 * _tmul(x, t_cur) - t_prev
+            acc = acc + mu_tilde[n] * t_next
+            t_prev, t_cur = t_cur, t_next
+    one_minus = -_tmul(x, x)
+    one_minus[0] += 1.0
+    denom = a * math.pi * _tsqrt(one_minus)
+    series = _tdiv(acc, denom)
+    return series[order] * math.factorial(order)
+
+
+def _dos_single(mu_tilde, a, b, E, NC):
+    """ρ at one energy, no derivative (src/applications/dos.jl:151-161)."""
+    x = (E - b) / a
+    return chebyshev_lin_trans(x, np.arange(NC), mu_tilde) / (a * math.pi * math.sqrt(1 - x * x))
+
+
+def dos(mu, H_rescale_factor, *, b=0.0, E_grid=None, N_tilde: int = 0, E_range=None, NC: int = 0,
+        kernel=JacksonKernel, dE_order: int = 0):
+    """ρ(E) = Σ_n h_n g_n μ_n T_n(x) / (aπ√(1-x²)), x = (E-b)/a; zero outside
+    the band (src/applications/dos.jl:76-149).  Returns (E_grid, rho)."""
+    mu = np.asarray(mu)
+    if mu.ndim != 1:
+        raise AssertionError("The input need to be 1D array")
+    if not H_rescale_factor > 0:
+        raise AssertionError("H_rescale_factor > 0")
+    a = float(H_rescale_factor)
+    if NC == 0:
+        NC = len(mu)
+    else:
+        if NC > len(mu):
+            warnings.warn(f"NC={NC} exceeds the number of moments; decreased to {len(mu)}.")
+        NC = min(NC, len(mu))
+    if E_grid is None:
+        if N_tilde == 0:
+            N_tilde = NC * 2
+        if E_range is None:
+            E_range = [b - 0.99 * a, b + 0.99 * a]
+        E_grid = (np.arange(N_tilde + 1) * (E_range[1] - E_range[0])) / N_tilde + E_range[0]
+    else:
+        if E_range is not None:
+            raise AssertionError("Should not set `E_grid` and `E_range` simultaneously.")
+        E_grid = np.asarray(E_grid, dtype=np.float64)
+        if N_tilde != 0 and len(E_grid) != N_tilde:
+            raise AssertionError("`N_tilde` does not match with `E_grid`.")
+    E_grid = np.asarray(E_grid, dtype=np.float64)
+    rho_full = np.zeros(len(E_grid), dtype=np.float64)
+    idx = np.abs(E_grid - b) < a
+    x_in = (E_grid[idx] - b) / a
+    ns = np.arange(NC)
+    hgn = np.asarray(kernel(ns, NC), dtype=np.float64) * hn(ns)
+    mu_tilde = mu[:NC] * hgn
+    if dE_order == 0:
+        rho = chebyshev_lin_trans(x_in, ns, mu_tilde)
+        rho = rho / (a * np.pi * np.sqrt(1 - x_in ** 2))
+    else:
+        rho = np.array([_dos_taylor(mu_tilde, a, b, E, dE_order) for E in E_grid[idx]])
+    rho_full[idx] = np.real(rho)
+    return E_grid, rho_full
+
+
+def dos0(mu, H_rescale_factor, *, NC: int = 0, kernel=JacksonKernel, dE_order: int = 0):
+    """DOS (or its second energy derivative) at x = 0 from the closed-form
+    T_n(0) (src/applications/dos.jl:163-199)."""
+    if not H_rescale_factor > 0:
+        raise AssertionError("H_rescale_factor > 0")
+    a = float(H_rescale_factor)
+    mu = np.asarray(mu)
+    if NC == 0:
+        NC = len(mu)
+    else:
+        if NC > len(mu):
+            warnings.warn(f"NC={NC} exceeds the number of moments; decreased to {len(mu)}.")
+        NC = min(NC, len(mu))
+    n0 = np.arange(0, NC, 4)
+    n2 = np.arange(2, NC, 4)
+    m0, m2 = mu[n0], mu[n2]
+    k0 = np.asarray(kernel(n0, NC)) * hn(n0)
+    k2 = np.asarray(kernel(n2, NC)) * hn(n2)
+    if dE_order == 0:
+        return (np.sum(m0 * k0) - np.sum(m2 * k2)) / (a * np.pi)
+    if dE_order == 2:
+        res = np.sum(m0 * k0 * (1 - n0 ** 2)) + np.sum(m2 * k2 * (n2 ** 2 - 1))
+        return res / (a ** 3 * np.pi)
+    raise NotImplementedError(f"unimplemented for derivative of order {dE_order}.")
+
+
+# ---- Fermi functions (src/utils/physics.jl:15-35) ------------------------------
+
+
+def fermiFunction(E, E_f, beta):
+    return 1.0 / (np.exp((E - E_f) * beta) + 1.0)
+
+
+def fermiFunctions(E_f: float, beta: float):
+    E_f = float(E_f)
+    beta = float(beta)
+    if math.isinf(beta):
+        return lambda x: ((np.asarray(x) < E_f).astype(np.float64) +
+                          (np.asarray(x) <= E_f).astype(np.float64)) / 2.0
+    return lambda x: fermiFunction(np.asarray(x, dtype=np.float64), E_f, beta)
+
+
+def gausschebyshevt(n: int):
+    """Gauss–Chebyshev (first kind) nodes and weights: x_k = cos((2k-1)π/(2n)),
+    w_k = π/n (the rule FastGaussQuadrature.gausschebyshevt returns; nodes via
+    sin for accuracy near ±1)."""
+    k = np.arange(1, n + 1)
+    x = np.sin(np.pi * (n - 2 * k + 1) / (2.0 * n))
+    return x[::-1].copy(), np.full(n, np.pi / n)
+
+
+def kubo_bastin_cond(mu2D, a: float, Ef: float, *, b: float = 0.0, NH: int, area: float,
+                     beta: float = math.inf, kernel=JacksonKernel, NC: int | None = None,
+                     quad_N: int | None = None, edge_cutoff: float = 1e-3) -> float:
+    """σ_αβ(Ef) in e²/h from the NC x NC moments
+    (src/applications/conductivity.jl:101-157)."""
+    if not a > 0:
+        raise ArgumentError("kubo_bastin_cond: a must be positive.")
+    if not area > 0:
+        raise ArgumentError("kubo_bastin_cond: area must be positive.")
+    if not NH > 0:
+        raise ArgumentError("kubo_bastin_cond: NH must be positive.")
+    mu2D = np.asarray(mu2D)
+    NC = mu2D.shape[0] if NC is None else NC
+    quad_N = 8 * NC if quad_N is None else quad_N
+    if not quad_N > 0:
+        raise ArgumentError("kubo_bastin_cond: quad_N must be positive.")
+    if not (0 <= edge_cutoff < 1):
+        raise ArgumentError("kubo_bastin_cond: edge_cutoff must be in [0, 1).")
+    NC = min(NC, mu2D.shape[0], mu2D.shape[1])
+    mt = mu2D_apply_kernel_and_h(mu2D[:NC, :NC], NC, kernel)
+    ff = fermiFunctions(Ef, beta)
+    nodes, weights = gausschebyshevt(quad_N)
+    fv = ff(a * nodes + b)
+    keep = (np.abs(nodes) < 1 - edge_cutoff) & (fv != 0)
+    if not np.any(keep):
+        return 0.0
+    x = nodes[keep]
+    phi = weights[keep] * fv[keep] / (1 - x ** 2) ** 1.5
+    ns = np.arange(NC)[:, None]
+    acc = 0.0 + 0.0j
+    chunk = max(1, min(len(x), -(-1_000_000 // NC)))
+    for lo in range(0, len(x), chunk):
+        xs = x[lo:lo + chunk][None, :]
+        ps = phi[lo:lo + chunk][None, :]
+        th = np.arccos(xs)
+        s = np.sqrt(1 - xs ** 2)
+        Cm = np.cos(ns * th)
+        Em = np.exp(1j * ns * th)
+        WM = (xs - 1j * ns * s) * Em
+        WN = (xs + 1j * ns * s) * np.conj(Em)
+        U = mt.T @ Cm
+        V = mt @ Cm
+        acc += np.sum(ps * (U * WM)) + np.sum(ps * (V * WN))
+    return float(-2 * NH / (area * a ** 2) * acc.real)

@@ -1,0 +1,179 @@
+"""Lattice model generators for the benchmark and test workloads (vectorised,
+CSC ComplexF64, 0-based).  Each restates a model builder of the reference:
+
+- ``square_lattice``   benchmarks/core_bench.jl:35-64 (periodic square, hopping -1,
+                       minimum-image currents)
+- ``square_anderson``  the square lattice with on-site disorder U[-W/2, W/2]
+- ``graphene``         examples/GrapheneModel.jl:33-71 (honeycomb, cell index
+                       mod(y) + Ny*mod(x), A = 2c, B = 2c+1), plus on-site disorder
+- ``haldane_model``    test/ed_reference.jl:69-147 (with Jx, Jy, area)
+- ``cubic_model``      test/ed_reference.jl:396-442 (with Jx, Jy, Jz, volume)
+- ``chain``            the periodic 1-D chain of test/dos_test.jl:214-228
+
+Disorder comes from the build's xoshiro256++ generator (kpmhip.rng), not from
+Julia's Xoshiro stream.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import scipy.sparse as sp
+
+from .rng import Xoshiro256pp
+
+
+def _csc(rows, cols, vals, n):
+    return sp.coo_matrix((np.asarray(vals, dtype=np.complex128), (np.asarray(rows), np.asarray(cols))),
+                         shape=(n, n)).tocsc()
+
+
+def square_lattice(Lx: int, Ly: int, *, currents: bool = False, t: float = 1.0):
+    """Periodic square lattice, every directed NN entry -t; currents
+    Jα[i,j] = H[i,j] (r_i - r_j)_α with minimum-image displacements."""
+    NH = Lx * Ly
+    x, y = np.meshgrid(np.arange(Lx), np.arange(Ly), indexing="ij")
+    x = x.T.ravel()  # y outer, x inner (site = x + Lx*y)
+    y = y.T.ravel()
+    site = x + Lx * y
+    rows, cols, hv, jx, jy = [], [], [], [], []
+    mi = lambda d, L: (d + L // 2) % L - L // 2  # noqa: E731
+    for xn, yn in (((x + 1) % Lx, y), ((x - 1) % Lx, y), (x, (y + 1) % Ly), (x, (y - 1) % Ly)):
+        j = xn + Lx * yn
+        rows.append(site)
+        cols.append(j)
+        hv.append(np.full(NH, -t, dtype=np.complex128))
+        jx.append(-t * mi(x - xn, Lx).astype(np.complex128))
+        jy.append(-t * mi(y - yn, Ly).astype(np.complex128))
+    rows, cols = np.concatenate(rows), np.concatenate(cols)
+    H = _csc(rows, cols, np.concatenate(hv), NH)
+    if not currents:
+        return H
+    return H, _csc(rows, cols, np.concatenate(jx), NH), _csc(rows, cols, np.concatenate(jy), NH)
+
+
+def square_anderson(L: int, W: float = 2.0, *, t: float = 1.0, seed: int = 42, rng=None):
+    """L x L periodic square lattice, hopping -t, on-site ε_i ~ U[-W/2, W/2]."""
+    H = square_lattice(L, L, t=t)
+    rng = rng if rng is not None else Xoshiro256pp(seed)
+    eps = W * (rng.rand(L * L) - 0.5)
+    return (H + sp.diags(eps.astype(np.complex128), format="csc")).tocsc()
+
+
+def graphene(Nx: int, Ny: int, *, t: float = 1.0, disorder: float = 0.0, delta: float = 0.0,
+             seed: int = 42, rng=None):
+    """Honeycomb Nx x Ny cells x 2 orbitals, NN hopping -t, staggered Δ/2, and
+    on-site disorder U[-disorder, disorder] (disorder = half-width)."""
+    N = 2 * Nx * Ny
+    xs, ys = np.meshgrid(np.arange(Nx), np.arange(Ny), indexing="ij")
+    xs = xs.ravel()
+    ys = ys.ravel()
+    cell = lambda x, y: (y % Ny) + Ny * (x % Nx)  # noqa: E731  0-based cell index
+    cur = cell(xs, ys)
+    pbx = cell(xs + 1, ys)
+    pby = cell(xs, ys + 1)
+    A = 2 * cur
+    B = 2 * cur + 1
+    rows = np.concatenate([A, A, B, B, B, 2 * pbx, B, 2 * pby])
+    cols = np.concatenate([A, B, A, B, 2 * pbx, B, 2 * pby, B])
+    nc = len(cur)
+    vals = np.concatenate([np.full(nc, delta / 2), np.full(nc, -t), np.full(nc, -t),
+                           np.full(nc, -delta / 2), np.full(nc, -t), np.full(nc, -t),
+                           np.full(nc, -t), np.full(nc, -t)]).astype(np.complex128)
+    if disorder != 0.0:
+        rng = rng if rng is not None else Xoshiro256pp(seed)
+        eps = disorder * (2.0 * rng.rand(N) - 1.0)
+        rows = np.concatenate([rows, np.arange(N)])
+        cols = np.concatenate([cols, np.arange(N)])
+        vals = np.concatenate([vals, eps.astype(np.complex128)])
+    return _csc(rows, cols, vals, N)
+
+
+def haldane_model(Lx: int, Ly: int, *, t: float = 1.0, t2: float = 0.2, phi: float = math.pi / 2,
+                  m: float = 0.0):
+    """Haldane model on an Lx x Ly honeycomb torus: returns (H, Jx, Jy, area)."""
+    a1 = np.array([math.sqrt(3), 0.0])
+    a2 = np.array([math.sqrt(3) / 2, 1.5])
+    dAB = np.array([math.sqrt(3) / 2, 0.5])
+    N = 2 * Lx * Ly
+    X, Y = np.meshgrid(np.arange(1, Lx + 1), np.arange(1, Ly + 1), indexing="ij")
+    X = X.ravel()
+    Y = Y.ravel()
+    cell = lambda x, y: ((y - 1) % Ly) + Ly * ((x - 1) % Lx)  # noqa: E731  0-based
+    sA = lambda x, y: 2 * cell(x, y)  # noqa: E731
+    sB = lambda x, y: 2 * cell(x, y) + 1  # noqa: E731
+    I, J, V, VX, VY = [], [], [], [], []
+
+    def addhop(i, j, amp, d):
+        amp = np.broadcast_to(np.asarray(amp, dtype=np.complex128), i.shape)
+        I.extend([i, j])
+        J.extend([j, i])
+        V.extend([amp, np.conj(amp)])
+        VX.extend([amp * d[0], -np.conj(amp) * d[0]])
+        VY.extend([amp * d[1], -np.conj(amp) * d[1]])
+
+    addhop(sB(X, Y), sA(X, Y), -t + 0j, dAB)
+    addhop(sB(X, Y), sA(X + 1, Y), -t + 0j, dAB - a1)
+    addhop(sB(X, Y), sA(X, Y + 1), -t + 0j, dAB - a2)
+    for d, (xo, yo) in ((a1, (1, 0)), (a2 - a1, (-1, 1)), (-a2, (0, -1))):
+        addhop(sA(X + xo, Y + yo), sA(X, Y), t2 * np.exp(1j * phi), d)
+        addhop(sB(X + xo, Y + yo), sB(X, Y), t2 * np.exp(-1j * phi), d)
+    z = np.zeros(X.shape, dtype=np.complex128)
+    I.extend([sA(X, Y), sB(X, Y)])
+    J.extend([sA(X, Y), sB(X, Y)])
+    V.extend([np.full(X.shape, m + 0j), np.full(X.shape, -m + 0j)])
+    VX.extend([z, z])
+    VY.extend([z, z])
+    I, J = np.concatenate(I), np.concatenate(J)
+    H = _csc(I, J, np.concatenate(V), N)
+    Jx = _csc(I, J, np.concatenate(VX), N)
+    Jy = _csc(I, J, np.concatenate(VY), N)
+    area = Lx * Ly * abs(a1[0] * a2[1] - a1[1] * a2[0])
+    return H, Jx, Jy, area
+
+
+def cubic_model(L: int, *, t: float = 1.0, W: float = 0.0, seed: int = 1, rng=None):
+    """Periodic L^3 simple-cubic model, hopping -t, on-site W*(U - 1/2) drawn in
+    site order (x fastest); returns (H, Jx, Jy, Jz, volume)."""
+    if L < 3:
+        from ._lib import ArgumentError
+        raise ArgumentError("cubic_model: L must be at least 3")
+    if W < 0:
+        from ._lib import ArgumentError
+        raise ArgumentError("cubic_model: W must be nonnegative")
+    N = L ** 3
+    z, y, x = np.meshgrid(np.arange(L), np.arange(L), np.arange(L), indexing="ij")
+    x, y, z = x.ravel(), y.ravel(), z.ravel()
+    site = lambda a, b, c: (a % L) + L * (b % L) + L * L * (c % L)  # noqa: E731
+    j = site(x, y, z)
+    I, J, V, VX, VY, VZ = [], [], [], [], [], []
+    for (dx, dy, dz) in ((1, 0, 0), (0, 1, 0), (0, 0, 1)):
+        i = site(x + dx, y + dy, z + dz)
+        amp = np.full(N, -t + 0j)
+        I.extend([i, j])
+        J.extend([j, i])
+        V.extend([amp, np.conj(amp)])
+        VX.extend([amp * dx, -np.conj(amp) * dx])
+        VY.extend([amp * dy, -np.conj(amp) * dy])
+        VZ.extend([amp * dz, -np.conj(amp) * dz])
+    rng = rng if rng is not None else Xoshiro256pp(seed)
+    onsite = W * (rng.rand(N) - 0.5) if W != 0 else np.zeros(N)
+    zz = np.zeros(N, dtype=np.complex128)
+    I.append(j)
+    J.append(j)
+    V.append(onsite.astype(np.complex128))
+    VX.append(zz)
+    VY.append(zz)
+    VZ.append(zz)
+    I, J = np.concatenate(I), np.concatenate(J)
+    mk = lambda vals: _csc(I, J, np.concatenate(vals), N)  # noqa: E731
+    return mk(V), mk(VX), mk(VY), mk(VZ), float(N)
+
+
+def chain(N: int, *, t: float = 1.0, shift: float = 0.0):
+    """Periodic 1-D chain with on-site `shift` and hopping -t."""
+    i = np.arange(N)
+    rows = np.concatenate([i, i, (i + 1) % N])
+    cols = np.concatenate([i, (i + 1) % N, i])
+    vals = np.concatenate([np.full(N, shift), np.full(N, -t), np.full(N, -t)]).astype(np.complex128)
+    return _csc(rows, cols, vals, N)

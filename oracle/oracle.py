@@ -1,0 +1,273 @@
+"""The oracle -- TEST INFRASTRUCTURE ONLY.
+
+Python access to the C restatement of KPM.jl's CPU moment engine
+(oracle/kpm_oracle.c -> oracle/liboracle.so) plus independent numpy dense
+references used to pin it.  Only tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg may import this module; the product path
+(kpmhip + libkpm_hip.so) never does.
+
+Dense references (numpy, tiny sizes) restate the reference's own test
+references: _chebyshev_matrices / _kpm2d_dense_reference /
+_kpm3d_dense_reference (test/kpm2d_blocking_test.jl:25-58) and the ED moment
+identity mu_n = sum_i |<v_i|psi>|^2 T_n(lambda_i) (test/dos_test.jl:39-43).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import scipy.sparse as sp
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_pd = C.POINTER(C.c_double)
+_pi = C.POINTER(C.c_int64)
+_i = C.c_int64
+
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        sigs = {
+            "kpm_oracle_set_threads": (None, [C.c_int]),
+            "kpm_oracle_get_threads": (C.c_int, []),
+            "kpm_oracle_1d": (C.c_int, [_i, _pi, _pi, _pd, _i, _i, _pd, _pd]),
+            "kpm_oracle_1d_lr": (C.c_int, [_i, _pi, _pi, _pd, _i, _i, _pd, _pd, _pd]),
+            "kpm_oracle_1d_current": (C.c_int, [_i, _pi, _pi, _pd, _pi, _pi, _pd, _i, _i, _pd, _pd]),
+            "kpm_oracle_2d_blocking": (C.c_int, [_i, _i, _i, _i, _i, _i, _pi, _pi, _pi]),
+            "kpm_oracle_2d": (C.c_int, [_i, _pi, _pi, _pd, _pi, _pi, _pd, _pi, _pi, _pd, _i, _i,
+                                        _pd, _pd, _i, _i, C.c_int, C.c_int, _pd]),
+            "kpm_oracle_3d": (C.c_int, [_i, _pi, _pi, _pd, _pi, _pi, _pd, _pi, _pi, _pd, _pi, _pi,
+                                        _pd, _i, _i, _pd, _pd, _i, _i, _pd]),
+            "kpm_oracle_cheb_iter": (C.c_int, [_i, _pi, _pi, _pd, _i, _i, _pd]),
+            "kpm_oracle_1d_steps": (C.c_int, [_i, _pi, _pi, _pd, _i, _i, _pd, _pd, _pd]),
+        }
+        for name, (res, args) in sigs.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def set_threads(n: int):
+    lib().kpm_oracle_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(lib().kpm_oracle_get_threads())
+
+
+class CSC:
+    """Julia SparseMatrixCSC arrays (0-based) of a square matrix."""
+
+    def __init__(self, A):
+        A = sp.csc_matrix(A, dtype=np.complex128)
+        A.sort_indices()
+        self.n = A.shape[0]
+        self.colptr = np.ascontiguousarray(A.indptr, dtype=np.int64)
+        self.rowval = np.ascontiguousarray(A.indices, dtype=np.int64)
+        self.nz = np.ascontiguousarray(A.data, dtype=np.complex128)
+
+    def args(self):
+        return (self.n, self.colptr.ctypes.data_as(_pi), self.rowval.ctypes.data_as(_pi),
+                self.nz.ctypes.data_as(_pd))
+
+    def args3(self):
+        return self.args()[1:]
+
+
+def _dp(a):
+    assert a.flags.f_contiguous or a.flags.c_contiguous
+    return a.ctypes.data_as(_pd)
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise ValueError(f"oracle {what} rejected its arguments (status {rc})")
+
+
+def _blk(psi):
+    a = np.asarray(psi, dtype=np.complex128)
+    if a.ndim == 1:
+        a = a.reshape(-1, 1)
+    return np.asfortranarray(a)
+
+
+def kpm_1d(H, NC: int, psi) -> np.ndarray:
+    """kpm_1d! doubling (src/moment.jl:420-492): NR x NC complex."""
+    A = CSC(H)
+    p = _blk(psi)
+    NR = p.shape[1]
+    mu = np.zeros((NR, NC), dtype=np.complex128, order="F")
+    _check(lib().kpm_oracle_1d(*A.args(), NC, NR, _dp(p), _dp(mu)), "kpm_1d")
+    return mu
+
+
+def kpm_1d_lr(H, NC: int, psi_l, psi_r) -> np.ndarray:
+    A = CSC(H)
+    pl, pr = _blk(psi_l), _blk(psi_r)
+    NR = pl.shape[1]
+    mu = np.zeros((NR, NC), dtype=np.complex128, order="F")
+    _check(lib().kpm_oracle_1d_lr(*A.args(), NC, NR, _dp(pl), _dp(pr), _dp(mu)), "kpm_1d_lr")
+    return mu
+
+
+def kpm_1d_current(H, J, NC: int, psi) -> np.ndarray:
+    A = CSC(H)
+    Jc = CSC(J)
+    p = _blk(psi)
+    NR = p.shape[1]
+    mu = np.zeros((NR, NC), dtype=np.complex128, order="F")
+    _check(lib().kpm_oracle_1d_current(*A.args(), *Jc.args3(), NC, NR, _dp(p), _dp(mu)),
+           "kpm_1d_current")
+    return mu
+
+
+def kpm_2d_blocking(NH, NR, NC, budget, arr_size=0, right_block=0):
+    KL, KR, by = C.c_int64(), C.c_int64(), C.c_int64()
+    rc = lib().kpm_oracle_2d_blocking(NH, NR, NC, budget, arr_size, right_block, C.byref(KL),
+                                      C.byref(KR), C.byref(by))
+    if rc != 0:
+        raise ValueError("ArgumentError")
+    return {"arr_size": KL.value, "right_block": KR.value, "bytes": by.value}
+
+
+def kpm_2d(H, Ja, Jb, NC: int, psi_l, psi_r, K_left: int, K_right: int, parity: int = 0,
+           mn_sym: bool = False) -> np.ndarray:
+    A, Ac, Bc = CSC(H), CSC(Ja), CSC(Jb)
+    pl, pr = _blk(psi_l), _blk(psi_r)
+    NR = pl.shape[1]
+    mu = np.zeros((NC, NC), dtype=np.complex128, order="F")
+    _check(lib().kpm_oracle_2d(*A.args(), *Ac.args3(), *Bc.args3(), NC, NR, _dp(pl), _dp(pr),
+                               K_left, K_right, parity, 1 if mn_sym else 0, _dp(mu)), "kpm_2d")
+    return mu
+
+
+def kpm_3d(H, Ja, Jb, Jg, NC: int, psi_l, psi_r, K_left: int = 3, K_right: int | None = None):
+    K_right = min(NC, 16) if K_right is None else K_right
+    A, Ac, Bc, Gc = CSC(H), CSC(Ja), CSC(Jb), CSC(Jg)
+    pl, pr = _blk(psi_l), _blk(psi_r)
+    NR = pl.shape[1]
+    mu = np.zeros((NC, NC, NC), dtype=np.complex128, order="F")
+    _check(lib().kpm_oracle_3d(*A.args(), *Ac.args3(), *Bc.args3(), *Gc.args3(), NC, NR, _dp(pl),
+                               _dp(pr), K_left, K_right, _dp(mu)), "kpm_3d")
+    return mu
+
+
+def cheb_iter(H, slots: np.ndarray) -> np.ndarray:
+    """chebyshev_iter over an NH x NR x n workspace (slots 1, 2 given)."""
+    A = CSC(H)
+    s = np.asfortranarray(np.array(slots, dtype=np.complex128))
+    NH, NR, n = s.shape
+    _check(lib().kpm_oracle_cheb_iter(*A.args(), NR, n, _dp(s)), "cheb_iter")
+    return s
+
+
+class StepRunner:
+    """Bounded run of the doubling loop for CPU-baseline timing."""
+
+    def __init__(self, H, psi):
+        self.A = CSC(H)
+        self.v0 = _blk(psi).copy(order="F")
+        self.v1 = self.v0.copy(order="F")
+        self.NR = self.v0.shape[1]
+
+    def run(self, steps: int) -> np.ndarray:
+        mu = np.zeros((self.NR, 2 * steps), dtype=np.complex128, order="F")
+        _check(lib().kpm_oracle_1d_steps(*self.A.args(), self.NR, steps, _dp(self.v0), _dp(self.v1),
+                                         _dp(mu)), "1d_steps")
+        return mu
+
+
+# ---- independent dense references -----------------------------------------------
+
+
+def chebyshev_matrices(H, NC: int):
+    """T_0..T_{NC-1}(H) as dense matrices (test/kpm2d_blocking_test.jl:25-35)."""
+    Hd = H.toarray() if sp.issparse(H) else np.asarray(H, dtype=np.complex128)
+    N = Hd.shape[0]
+    T = [np.eye(N, dtype=np.complex128)]
+    if NC == 1:
+        return T
+    T.append(Hd.astype(np.complex128))
+    for _ in range(2, NC):
+        T.append(2 * Hd @ T[-1] - T[-2])
+    return T
+
+
+def kpm1d_dense(H, psi, NC):
+    """[<psi_i|T_n psi_i>] NR x NC (test/kpm2d_blocking_test.jl:249)."""
+    T = chebyshev_matrices(H, NC)
+    p = _blk(psi)
+    return np.array([[np.vdot(p[:, i], T[n] @ p[:, i]) for n in range(NC)]
+                     for i in range(p.shape[1])])
+
+
+def kpm1d_lr_dense(H, psi_l, psi_r, NC):
+    T = chebyshev_matrices(H, NC)
+    pl, pr = _blk(psi_l), _blk(psi_r)
+    return np.array([[np.vdot(pl[:, i], T[n] @ pr[:, i]) for n in range(NC)]
+                     for i in range(pl.shape[1])])
+
+
+def current_dense(H, J, psi, NC):
+    """[<psi_i|J T_n psi_i>] (test/kpm2d_blocking_test.jl:266)."""
+    T = chebyshev_matrices(H, NC)
+    Jd = J.toarray() if sp.issparse(J) else np.asarray(J)
+    p = _blk(psi)
+    return np.array([[np.vdot(p[:, i], Jd @ (T[n] @ p[:, i])) for n in range(NC)]
+                     for i in range(p.shape[1])])
+
+
+def kpm2d_dense(H, Ja, Jb, psi_l, psi_r, NC):
+    """mu[n,m] = sum_i <T_m psi_l_i | Jb T_n Ja psi_r_i>/NR
+    (test/kpm2d_blocking_test.jl:37-45)."""
+    T = chebyshev_matrices(H, NC)
+    Ja = Ja.toarray() if sp.issparse(Ja) else np.asarray(Ja)
+    Jb = Jb.toarray() if sp.issparse(Jb) else np.asarray(Jb)
+    pl, pr = _blk(psi_l), _blk(psi_r)
+    NR = pl.shape[1]
+    Lm = np.stack([T[m] @ pl for m in range(NC)])            # m, N, NR
+    Rn = np.stack([Jb @ (T[n] @ (Ja @ pr)) for n in range(NC)])  # n, N, NR
+    return np.einsum("mir,nir->nm", Lm.conj(), Rn) / NR
+
+
+def kpm3d_dense(H, Ja, Jb, Jg, psi_l, psi_r, NC):
+    """mu[n2,n3,n1] (test/kpm2d_blocking_test.jl:47-58)."""
+    T = chebyshev_matrices(H, NC)
+    dense = lambda X: X.toarray() if sp.issparse(X) else np.asarray(X)  # noqa: E731
+    Ja, Jb, Jg = dense(Ja), dense(Jb), dense(Jg)
+    pl, pr = _blk(psi_l), _blk(psi_r)
+    NR = pl.shape[1]
+    mu = np.zeros((NC, NC, NC), dtype=np.complex128)
+    Lm = np.stack([T[m] @ pl for m in range(NC)])
+    for n1 in range(NC):
+        v = T[n1] @ (Ja @ pr)
+        Rn = np.stack([Jb @ (T[n2] @ (Jg @ v)) for n2 in range(NC)])
+        mu[:, :, n1] = np.einsum("mir,nir->nm", Lm.conj(), Rn) / NR
+    return mu
+
+
+def ed_moments(H, psi, NC):
+    """mu_n = sum_i |<v_i|psi>|^2 T_n(lambda_i) (test/dos_test.jl:39-43), per column."""
+    Hd = H.toarray() if sp.issparse(H) else np.asarray(H)
+    lam, V = np.linalg.eigh(Hd)
+    p = _blk(psi)
+    w = np.abs(V.conj().T @ p) ** 2
+    th = np.arccos(np.clip(lam, -1, 1))
+    Tn = np.cos(np.arange(NC)[:, None] * th[None, :])
+    return (Tn @ w).T
