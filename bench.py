@@ -72,6 +72,8 @@ def main() -> int:
     Hd = K.to_device(H)
 
     def run_leg(precision: str):
+        tuned = K.autotune(Hd, NR, precision)
+        print(f"[bench] autotune {precision}: {tuned}", file=sys.stderr, flush=True)
         blk = K.DeviceBlock.from_host(psi, precision)
         mu = np.zeros((NR, NC), dtype=np.complex128, order="F")
         lib = ctx.lib
@@ -94,15 +96,23 @@ def main() -> int:
         ms = ctypes.c_float(0.0)
         K._lib.check(lib.kpm_bench_step(ctx.handle, Hd.handle, x.handle, z.handle,
                                         args.kernel_launches, ctypes.byref(ms)), "bench_step")
+        cms = ctypes.c_float(0.0)
+        K._lib.check(lib.kpm_bench_copy(ctx.handle, z.handle, x.handle, 20, ctypes.byref(cms)),
+                     "bench_copy")
+        copy_gbs = 2 * NH * K.moment._padded_nr(NR, x.prec) * (8 if x.prec else 16) / (cms.value / 20 * 1e-3) / 1e9
+        print(f"[bench] {precision}: step kernel {ms.value / args.kernel_launches:.4f} ms, "
+              f"copy kernel {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
         del x, z, blk
-        return mu.copy(), wall / args.steps, ev_ms / 1e3 / args.steps, ms.value / args.kernel_launches
+        tuned["copy_kernel_gbs"] = round(copy_gbs, 1)
+        return (mu.copy(), wall / args.steps, ev_ms / 1e3 / args.steps,
+                ms.value / args.kernel_launches, tuned)
 
     print("[bench] FP32-recurrence leg ...", file=sys.stderr, flush=True)
-    mu32, wall32, ev32, kms32 = run_leg("f32")
+    mu32, wall32, ev32, kms32, tune32 = run_leg("f32")
     mu64 = None
     if not args.skip_fp64:
         print("[bench] FP64 leg ...", file=sys.stderr, flush=True)
-        mu64, wall64, ev64, kms64 = run_leg("f64")
+        mu64, wall64, ev64, kms64, tune64 = run_leg("f64")
 
     units = NC * NR * nnz
     value = units / wall32
@@ -113,7 +123,7 @@ def main() -> int:
     roofline = {"bound": "hbm", "achieved": round(achieved32, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved32 / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "k_step<FP32,LPR=32,MODE_DBL>", "bytes_per_launch": b32,
-                "avg_launch_ms": round(kms32, 5)}
+                "avg_launch_ms": round(kms32, 5), "tuned": tune32}
     tfile = os.path.join(ROOT, "profiles", "traffic_graphene2M_fp32.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
@@ -131,6 +141,7 @@ def main() -> int:
             "fp32_speedup_vs_fp64": round(wall64 / wall32, 3),
             "fp64_kernel_gbs": round(b64 / (kms64 * 1e-3) / 1e9, 1),
             "fp64_kernel_frac": round(b64 / (kms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "fp64_tuned": tune64,
             "max_abs_dmu_fp32_vs_fp64": float(np.max(np.abs(avg32 - avg64))),
             "max_abs_dmu_per_probe": float(np.max(np.abs(mu32 - mu64))),
         })

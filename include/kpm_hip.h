@@ -165,6 +165,10 @@ int kpm_3d(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, const kpm_csr* Jb,
  * The output is not meaningful (timing only). */
 int kpm_bench_step(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z, int64_t nsteps,
                    float* ms);
+/* Streaming copy src -> dst (same shape) `reps` times on the context
+ * stream, bracketed by HIP events: the plain copy kernel of known size that
+ * calibrates the HBM byte counters and measures the bandwidth ceiling. */
+int kpm_bench_copy(kpm_ctx* ctx, kpm_block* dst, const kpm_block* src, int64_t reps, float* ms);
 /* HIP-event timer on the context stream: record into slot 0..15, read the
  * elapsed milliseconds between two recorded slots. */
 int kpm_event_record(kpm_ctx* ctx, int slot);
@@ -177,7 +181,7 @@ int kpm_event_elapsed(kpm_ctx* ctx, int slot_a, int slot_b, float* ms);
  * records the fastest for this shape and returns it. */
 int kpm_autotune_step(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z, int reps,
                       int* best_unroll, int* best_grid, float* best_ms_per_step);
-/* Set the table entry for a shape directly (unroll 1, 2 or 4; grid 0 = auto). */
+/* Set the table entry for a shape directly (unroll 1 or 2; grid 0 = auto). */
 int kpm_tune_set(kpm_ctx* ctx, const kpm_csr* H, int64_t NR, int vec_prec, int unroll, int grid);
 
 #ifdef __cplusplus

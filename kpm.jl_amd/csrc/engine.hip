@@ -250,4 +250,18 @@ int kpm_bench_step(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z, i
   return KPM_OK;
 }
 
+int kpm_bench_copy(kpm_ctx* ctx, kpm_block* dst, const kpm_block* src, int64_t reps, float* ms) {
+  if (!ctx || !dst || !src || !ms || reps < 1) return fail(KPM_EINVAL, "bad arguments");
+  if (dst->NH != src->NH || dst->NRp != src->NRp || dst->prec != src->prec)
+    return fail(KPM_EINVAL, "block shapes differ");
+  KPM_HIP_TRY(hipSetDevice(ctx->device));
+  const int64_t nwords = dst->NH * dst->NRp * elem_bytes(dst->prec) / 16;
+  KPM_HIP_TRY(hipEventRecord(ctx->events[14], ctx->stream));
+  for (int64_t r = 0; r < reps; ++r) KPM_TRY(launch_copy16(ctx, src->d, dst->d, nwords));
+  KPM_HIP_TRY(hipEventRecord(ctx->events[15], ctx->stream));
+  KPM_HIP_TRY(hipEventSynchronize(ctx->events[15]));
+  KPM_HIP_TRY(hipEventElapsedTime(ms, ctx->events[14], ctx->events[15]));
+  return KPM_OK;
+}
+
 }  // extern "C"

@@ -22,7 +22,9 @@
 // stored in FP64 and reduced in a fixed order, then scaled, masked and added.
 //
 // Operand layout of a 16x16x4 MFMA (wave64): lane l supplies A[l%16][l/16]
-// and B[l/16][l%16]; it receives D[4*(l/16)+v][l%16] in accumulator element v.
+// and B[l/16][l%16].  It receives, in accumulator element v, D[4*(l/16)+v][l%16]
+// for the f32 instruction and D[l/16+4*v][l%16] for the f64 instruction
+// (measured on gfx950 with an in-bounds probe).
 #include "kpm_internal.h"
 
 #include <algorithm>
@@ -38,6 +40,8 @@ struct Mfma;
 template <>
 struct Mfma<double> {
   using acc_t = d4;
+  // output row of accumulator element v for lane group g = l/16
+  __device__ static inline int row(int g, int v) { return g + 4 * v; }
   __device__ static inline acc_t zero() { return acc_t{0.0, 0.0, 0.0, 0.0}; }
   __device__ static inline acc_t mac(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -47,6 +51,7 @@ struct Mfma<double> {
 template <>
 struct Mfma<float> {
   using acc_t = f4;
+  __device__ static inline int row(int g, int v) { return 4 * g + v; }
   __device__ static inline acc_t zero() { return acc_t{0.f, 0.f, 0.f, 0.f}; }
   __device__ static inline acc_t mac(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -117,7 +122,7 @@ __global__ void __launch_bounds__(256)
     for (int tm = 0; tm < 2; ++tm)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int n = n0 + 16 * tn + 4 * g + v;
+        const int n = n0 + 16 * tn + MF::row(g, v);
         const int m = m0 + 16 * tm + i;
         if (n < nR && m < nL)
           out[(size_t)n * ldm + m] = make_double2((double)are[tn][tm][v], (double)aim[tn][tm][v]);

@@ -141,6 +141,9 @@ int launch_combine(kpm_ctx* ctx, int grid, int64_t NR, int64_t NRp, const Combin
 int launch_upload(kpm_ctx* ctx, const void* staging, kpm_block* b);
 int launch_download(kpm_ctx* ctx, const kpm_block* b, void* staging);
 
+// streaming copy of nwords 16-byte words (calibration / bandwidth ceiling)
+int launch_copy16(kpm_ctx* ctx, const void* src, void* dst, int64_t nwords);
+
 // ---- contraction (gemm.hip) -----------------------------------------------------
 // mu_blk[n][m] (n < nR, m < nL) = scale * sum_k conj(L_m[k]) * R_n[k], over
 // K = NH*NRp complex entries per slot, accumulated into the host-layout

@@ -66,9 +66,24 @@ struct Word<KPM_VEC_F32> {
 };
 
 constexpr int kBlock = 256;  // 4 waves
-constexpr int kBatch = 4;    // gathers per row per batch
 
-template <int PREC, int LPR, int UNR, bool REALH, int MODE>
+// Per-lane metadata of one row tile: the lane's rows, their first nonzero and
+// nonzero count, and the column indices of the first KB nonzeros.
+template <int UNR>
+struct RowMeta {
+  int row[UNR], s[UNR], cnt[UNR];
+};
+template <int UNR, int KB>
+struct ColMeta {
+  int c[UNR][KB];
+};
+
+// Software pipeline over the tiles of a workgroup: while the gathers of
+// tile t are in flight the lane already loads the column indices of tile t+1
+// and the row pointers of tile t+2, so each tile costs one memory latency
+// instead of three dependent ones (row pointer -> column index -> gather).
+// Nonzeros beyond the first KB of a row are handled by a plain tail loop.
+template <int PREC, int LPR, int UNR, int KB, bool REALH, int MODE>
 __global__ void __launch_bounds__(kBlock)
     k_step(const int* __restrict__ rowptr, const int* __restrict__ colind,
            const void* __restrict__ vals_, const typename Word<PREC>::T* __restrict__ x,
@@ -89,6 +104,7 @@ __global__ void __launch_bounds__(kBlock)
   const int sub = lane % LPR;
   const int rslot = lane / LPR;
   const int wcol = blockIdx.y * LPR + sub;
+  const int rbase = wave * (RPW * UNR) + rslot;
 
   const double2* cvals = reinterpret_cast<const double2*>(vals_);
   const double* rvals = reinterpret_cast<const double*>(vals_);
@@ -107,81 +123,121 @@ __global__ void __launch_bounds__(kBlock)
   const int t0 = (int)(((int64_t)ntiles * xcd) >> 3);
   const int t1 = (int)(((int64_t)ntiles * (xcd + 1)) >> 3);
 
-  for (int t = t0 + q; t < t1; t += Bx) {
-    int rows[UNR], s[UNR], cnt[UNR];
-    int maxc = 0;
+  auto load_rows = [&](int t, RowMeta<UNR>& R) {
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
-      const int row = t * TR + wave * (RPW * UNR) + u * RPW + rslot;
-      rows[u] = row;
-      if (row < NH) {
-        s[u] = rowptr[row];
-        cnt[u] = rowptr[row + 1] - s[u];
+      const int row = t * TR + rbase + u * RPW;
+      R.row[u] = row;
+      if (t < t1 && row < NH) {
+        const int s0 = rowptr[row];
+        R.s[u] = s0;
+        R.cnt[u] = rowptr[row + 1] - s0;
       } else {
-        s[u] = 0;
-        cnt[u] = 0;
+        R.s[u] = 0;
+        R.cnt[u] = 0;
       }
-      maxc = max(maxc, cnt[u]);
     }
-    double yr[UNR][CPL], yi[UNR][CPL];
+  };
+  auto load_cols = [&](const RowMeta<UNR>& R, ColMeta<UNR, KB>& Cm) {
 #pragma unroll
     for (int u = 0; u < UNR; ++u)
 #pragma unroll
-      for (int c = 0; c < CPL; ++c) yr[u][c] = yi[u][c] = 0.0;
-
-    for (int j = 0; j < maxc; j += kBatch) {
-      T xv[UNR][kBatch];
-      double hr[UNR][kBatch], hi[UNR][kBatch];
+      for (int jj = 0; jj < KB; ++jj) Cm.c[u][jj] = jj < R.cnt[u] ? colind[R.s[u] + jj] : 0;
+  };
+  auto mac = [&](double hr, double hi, const T& xw, double* yr, double* yi) {
+    double xr[CPL], xi[CPL];
+    WT::unpack(xw, xr, xi);
 #pragma unroll
-      for (int u = 0; u < UNR; ++u)
-#pragma unroll
-        for (int jj = 0; jj < kBatch; ++jj) {
-          const int kk = j + jj;
-          if (kk < cnt[u]) {
-            const int k = s[u] + kk;
-            const int c = colind[k];
-            if constexpr (REALH) {
-              hr[u][jj] = rvals[k];
-              hi[u][jj] = 0.0;
-            } else {
-              const double2 h = cvals[k];
-              hr[u][jj] = h.x;
-              hi[u][jj] = h.y;
-            }
-            xv[u][jj] = x[(size_t)c * RW + wcol];
-          } else {
-            hr[u][jj] = 0.0;
-            hi[u][jj] = 0.0;
-            xv[u][jj] = WT::zero();
-          }
-        }
-#pragma unroll
-      for (int u = 0; u < UNR; ++u)
-#pragma unroll
-        for (int jj = 0; jj < kBatch; ++jj) {
-          double xr[CPL], xi[CPL];
-          WT::unpack(xv[u][jj], xr, xi);
-#pragma unroll
-          for (int c = 0; c < CPL; ++c) {
-            if constexpr (REALH) {
-              yr[u][c] = fma(hr[u][jj], xr[c], yr[u][c]);
-              yi[u][c] = fma(hr[u][jj], xi[c], yi[u][c]);
-            } else {
-              yr[u][c] = fma(hr[u][jj], xr[c], fma(-hi[u][jj], xi[c], yr[u][c]));
-              yi[u][c] = fma(hr[u][jj], xi[c], fma(hi[u][jj], xr[c], yi[u][c]));
-            }
-          }
-        }
+    for (int c = 0; c < CPL; ++c) {
+      if constexpr (REALH) {
+        yr[c] = fma(hr, xr[c], yr[c]);
+        yi[c] = fma(hr, xi[c], yi[c]);
+      } else {
+        yr[c] = fma(hr, xr[c], fma(-hi, xi[c], yr[c]));
+        yi[c] = fma(hr, xi[c], fma(hi, xr[c], yi[c]));
+      }
     }
+  };
 
+  int t = t0 + q;
+  RowMeta<UNR> R0, R1;
+  ColMeta<UNR, KB> C0;
+  load_rows(t, R0);
+  load_cols(R0, C0);
+  load_rows(t + Bx, R1);
+
+  for (; t < t1; t += Bx) {
+    // 1. this tile's gathers, values and row-local operands
+    T xv[UNR][KB];
+    double hr[UNR][KB], hi[UNR][KB];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u)
+#pragma unroll
+      for (int jj = 0; jj < KB; ++jj) {
+        if (jj < R0.cnt[u]) {
+          const int k = R0.s[u] + jj;
+          if constexpr (REALH) {
+            hr[u][jj] = rvals[k];
+            hi[u][jj] = 0.0;
+          } else {
+            const double2 h = cvals[k];
+            hr[u][jj] = h.x;
+            hi[u][jj] = h.y;
+          }
+          xv[u][jj] = x[(size_t)C0.c[u][jj] * RW + wcol];
+        } else {
+          hr[u][jj] = 0.0;
+          hi[u][jj] = 0.0;
+          xv[u][jj] = WT::zero();
+        }
+      }
+    T zv[UNR], sv[UNR];
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
-      if (rows[u] >= NH) continue;
-      const size_t off = (size_t)rows[u] * RW + wcol;
+      const bool live = R0.row[u] < NH && t < t1;
+      const size_t off = (size_t)(live ? R0.row[u] : 0) * RW + wcol;
+      zv[u] = (beta != 0.0 && live) ? z[off] : WT::zero();
+      if constexpr (MODE == MODE_DBL) sv[u] = live ? x[off] : WT::zero();
+      if constexpr (MODE == MODE_LR) sv[u] = live ? l[off] : WT::zero();
+    }
+    // 2. prefetch: column indices of the next tile, row pointers of the one after
+    ColMeta<UNR, KB> C1;
+    load_cols(R1, C1);
+    RowMeta<UNR> R2;
+    load_rows(t + 2 * Bx, R2);
+
+    // 3. accumulate
+    double yr[UNR][CPL], yi[UNR][CPL];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) yr[u][c] = yi[u][c] = 0.0;
+#pragma unroll
+      for (int jj = 0; jj < KB; ++jj) mac(hr[u][jj], hi[u][jj], xv[u][jj], yr[u], yi[u]);
+      for (int j = KB; j < R0.cnt[u]; ++j) {  // rows longer than KB
+        const int k = R0.s[u] + j;
+        double h_r, h_i;
+        if constexpr (REALH) {
+          h_r = rvals[k];
+          h_i = 0.0;
+        } else {
+          const double2 h = cvals[k];
+          h_r = h.x;
+          h_i = h.y;
+        }
+        mac(h_r, h_i, x[(size_t)colind[k] * RW + wcol], yr[u], yi[u]);
+      }
+    }
+
+    // 4. combine with z, store, fused dots
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      if (R0.row[u] >= NH) continue;
+      const size_t off = (size_t)R0.row[u] * RW + wcol;
       double orr[CPL], oii[CPL];
       if (beta != 0.0) {
         double zr[CPL], zi[CPL];
-        WT::unpack(z[off], zr, zi);
+        WT::unpack(zv[u], zr, zi);
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
           orr[c] = fma(alpha, yr[u][c], beta * zr[c]);
@@ -197,28 +253,28 @@ __global__ void __launch_bounds__(kBlock)
       const T ow = WT::pack(orr, oii);
       out[off] = ow;
       if constexpr (MODE != MODE_NONE) {
-        double sr[CPL], si[CPL];
+        double sr[CPL], si[CPL], ar[CPL], ai[CPL];
         WT::unpack(ow, sr, si);
+        WT::unpack(sv[u], ar, ai);
         if constexpr (MODE == MODE_DBL) {
-          double xr[CPL], xi[CPL];
-          WT::unpack(x[off], xr, xi);
 #pragma unroll
           for (int c = 0; c < CPL; ++c) {
-            accr[0][c] = fma(xr[c], xr[c], fma(xi[c], xi[c], accr[0][c]));
-            accr[1][c] = fma(xr[c], sr[c], fma(xi[c], si[c], accr[1][c]));
-            acci[1][c] = fma(xr[c], si[c], fma(-xi[c], sr[c], acci[1][c]));
+            accr[0][c] = fma(ar[c], ar[c], fma(ai[c], ai[c], accr[0][c]));
+            accr[1][c] = fma(ar[c], sr[c], fma(ai[c], si[c], accr[1][c]));
+            acci[1][c] = fma(ar[c], si[c], fma(-ai[c], sr[c], acci[1][c]));
           }
         } else {
-          double lr[CPL], li[CPL];
-          WT::unpack(l[off], lr, li);
 #pragma unroll
           for (int c = 0; c < CPL; ++c) {
-            accr[0][c] = fma(lr[c], sr[c], fma(li[c], si[c], accr[0][c]));
-            acci[0][c] = fma(lr[c], si[c], fma(-li[c], sr[c], acci[0][c]));
+            accr[0][c] = fma(ar[c], sr[c], fma(ai[c], si[c], accr[0][c]));
+            acci[0][c] = fma(ar[c], si[c], fma(-ai[c], sr[c], acci[0][c]));
           }
         }
       }
     }
+    R0 = R1;
+    C0 = C1;
+    R1 = R2;
   }
 
   if constexpr (ND > 0) {
@@ -375,18 +431,36 @@ __global__ void k_download(const void* __restrict__ src_, double2* __restrict__ 
   }
 }
 
+// plain streaming copy of n 16-byte words: the calibration kernel for the
+// HBM byte counters and the practical bandwidth ceiling of the device
+__global__ void __launch_bounds__(kBlock) k_copy16(const float4* __restrict__ src,
+                                                   float4* __restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    dst[i] = src[i];
+}
+
+int launch_copy16(kpm_ctx* ctx, const void* src, void* dst, int64_t nwords) {
+  const int64_t g = std::min<int64_t>((nwords + kBlock - 1) / kBlock,
+                                      (int64_t)std::max(ctx->num_cus, 8) * 16);
+  hipLaunchKernelGGL(k_copy16, dim3((unsigned)std::max<int64_t>(g, 1)), dim3(kBlock), 0,
+                     ctx->stream, (const float4*)src, (float4*)dst, nwords);
+  KPM_HIP_TRY(hipGetLastError());
+  return KPM_OK;
+}
+
 // ---------------------------------------------------------------------------
 // dispatch tables
 
 using StepFn = void (*)(const int*, const int*, const void*, const void*, const void*, void*,
                         const void*, double2*, int, int, int, int, double, double);
 
-template <int PREC, int LPR, int UNR, bool REALH, int MODE>
+template <int PREC, int LPR, int UNR, int KB, bool REALH, int MODE>
 static void launch_step_t(dim3 grid, hipStream_t st, const int* rp, const int* ci, const void* v,
                           const void* x, const void* z, void* out, const void* l, double2* part,
                           int NH, int RW, int NRp, int ntiles, double alpha, double beta) {
   using T = typename Word<PREC>::T;
-  hipLaunchKernelGGL((k_step<PREC, LPR, UNR, REALH, MODE>), grid, dim3(kBlock), 0, st, rp, ci, v,
+  hipLaunchKernelGGL((k_step<PREC, LPR, UNR, KB, REALH, MODE>), grid, dim3(kBlock), 0, st, rp, ci, v,
                      (const T*)x, (const T*)z, (T*)out, (const T*)l, part, NH, RW, NRp, ntiles,
                      alpha, beta);
 }
@@ -395,39 +469,37 @@ using StepLauncher = void (*)(dim3, hipStream_t, const int*, const int*, const v
                               const void*, void*, const void*, double2*, int, int, int, int,
                               double, double);
 
-template <int PREC, int LPR, int UNR, bool REALH>
+template <int PREC, int LPR, int UNR, int KB, bool REALH>
 static StepLauncher pick_mode(int mode) {
   switch (mode) {
-    case MODE_DBL: return &launch_step_t<PREC, LPR, UNR, REALH, MODE_DBL>;
-    case MODE_LR: return &launch_step_t<PREC, LPR, UNR, REALH, MODE_LR>;
-    default: return &launch_step_t<PREC, LPR, UNR, REALH, MODE_NONE>;
+    case MODE_DBL: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_DBL>;
+    case MODE_LR: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_LR>;
+    default: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_NONE>;
   }
 }
 
 template <int PREC, int LPR, int UNR>
-static StepLauncher pick_real(bool realh, int mode) {
-  return realh ? pick_mode<PREC, LPR, UNR, true>(mode) : pick_mode<PREC, LPR, UNR, false>(mode);
+static StepLauncher pick_real(int kb, bool realh, int mode) {
+  if (kb <= 4)
+    return realh ? pick_mode<PREC, LPR, UNR, 4, true>(mode) : pick_mode<PREC, LPR, UNR, 4, false>(mode);
+  return realh ? pick_mode<PREC, LPR, UNR, 8, true>(mode) : pick_mode<PREC, LPR, UNR, 8, false>(mode);
 }
 
 template <int PREC, int LPR>
-static StepLauncher pick_unr(int unr, bool realh, int mode) {
-  switch (unr) {
-    case 1: return pick_real<PREC, LPR, 1>(realh, mode);
-    case 4: return pick_real<PREC, LPR, 4>(realh, mode);
-    default: return pick_real<PREC, LPR, 2>(realh, mode);
-  }
+static StepLauncher pick_unr(int unr, int kb, bool realh, int mode) {
+  return unr == 1 ? pick_real<PREC, LPR, 1>(kb, realh, mode) : pick_real<PREC, LPR, 2>(kb, realh, mode);
 }
 
 template <int PREC>
-static StepLauncher pick_lpr(int lpr, int unr, bool realh, int mode) {
+static StepLauncher pick_lpr(int lpr, int unr, int kb, bool realh, int mode) {
   switch (lpr) {
-    case 1: return pick_unr<PREC, 1>(unr, realh, mode);
-    case 2: return pick_unr<PREC, 2>(unr, realh, mode);
-    case 4: return pick_unr<PREC, 4>(unr, realh, mode);
-    case 8: return pick_unr<PREC, 8>(unr, realh, mode);
-    case 16: return pick_unr<PREC, 16>(unr, realh, mode);
-    case 32: return pick_unr<PREC, 32>(unr, realh, mode);
-    default: return pick_unr<PREC, 64>(unr, realh, mode);
+    case 1: return pick_unr<PREC, 1>(unr, kb, realh, mode);
+    case 2: return pick_unr<PREC, 2>(unr, kb, realh, mode);
+    case 4: return pick_unr<PREC, 4>(unr, kb, realh, mode);
+    case 8: return pick_unr<PREC, 8>(unr, kb, realh, mode);
+    case 16: return pick_unr<PREC, 16>(unr, kb, realh, mode);
+    case 32: return pick_unr<PREC, 32>(unr, kb, realh, mode);
+    default: return pick_unr<PREC, 64>(unr, kb, realh, mode);
   }
 }
 
@@ -455,7 +527,8 @@ int launch_step(kpm_ctx* ctx, const StepLaunch& s, const StepConfig& cfg, int* g
   const int RW = row_words(x->NR, prec);
   const int lpr = std::min(RW, 64);
   const int chunks = RW / lpr;
-  const int unr = cfg.unroll == 1 || cfg.unroll == 4 ? cfg.unroll : 2;
+  const int unr = cfg.unroll == 1 ? 1 : 2;
+  const int kb = s.A->max_row_nnz <= 4 ? 4 : 8;
   const int rpw = 64 / lpr;
   const int64_t TR = 4LL * rpw * unr;
   const int64_t ntiles = (x->NH + TR - 1) / TR;
@@ -463,8 +536,8 @@ int launch_step(kpm_ctx* ctx, const StepLaunch& s, const StepConfig& cfg, int* g
   int G = cfg.grid > 0 ? (cfg.grid + 7) / 8 * 8 : default_grid(ctx, ntiles);
   const int nd = s.mode == MODE_DBL ? 2 : (s.mode == MODE_LR ? 1 : 0);
   if (nd > 0) KPM_TRY(ensure(ctx, ctx->partials, (size_t)G * nd * x->NRp * sizeof(double2)));
-  StepLauncher fn = prec == KPM_VEC_F32 ? pick_lpr<KPM_VEC_F32>(lpr, unr, s.A->is_real != 0, s.mode)
-                                        : pick_lpr<KPM_VEC_F64>(lpr, unr, s.A->is_real != 0, s.mode);
+  StepLauncher fn = prec == KPM_VEC_F32 ? pick_lpr<KPM_VEC_F32>(lpr, unr, kb, s.A->is_real != 0, s.mode)
+                                        : pick_lpr<KPM_VEC_F64>(lpr, unr, kb, s.A->is_real != 0, s.mode);
   fn(dim3(G, chunks), ctx->stream, s.A->rowptr, s.A->colind, s.A->vals, x->d, s.z ? s.z->d : nullptr,
      s.out->d, s.l ? s.l->d : nullptr, (double2*)ctx->partials.ptr, (int)x->NH, RW, (int)x->NRp,
      (int)ntiles, s.alpha, s.beta);

@@ -43,15 +43,15 @@ extern "C" {
 // Time every candidate geometry of the MODE_DBL pass (the kpm_1d hot loop)
 // over `reps` launches each on blocks X, Z (Z is overwritten), record the
 // fastest for this shape (it then applies to all modes of the shape), and
-// return it.  Candidates: rows in flight per lane {1, 2, 4} x workgroups per
-// CU {2, 4, 8, 16}.
+// return it.  Candidates: rows in flight per lane {1, 2} x workgroups per CU
+// {2, 4, 8, 16}.
 int kpm_autotune_step(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z, int reps,
                       int* best_unroll, int* best_grid, float* best_ms_per_step) {
   if (!ctx || !H || !X || !Z || reps < 1) return fail(KPM_EINVAL, "bad arguments");
   if (X->NH != H->n || Z->NH != H->n || X->NR != Z->NR || X->prec != Z->prec)
     return fail(KPM_EINVAL, "block shapes do not match");
   KPM_HIP_TRY(hipSetDevice(ctx->device));
-  const int unrolls[3] = {1, 2, 4};
+  const int unrolls[2] = {1, 2};
   const int per_cu[4] = {2, 4, 8, 16};
   float best = 1e30f;
   StepConfig bestc;
@@ -88,7 +88,7 @@ int kpm_autotune_step(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z
 // Set a table entry directly (e.g. from a saved tuning file).
 int kpm_tune_set(kpm_ctx* ctx, const kpm_csr* H, int64_t NR, int vec_prec, int unroll, int grid) {
   if (!ctx || !H || NR <= 0) return fail(KPM_EINVAL, "bad arguments");
-  if (unroll != 1 && unroll != 2 && unroll != 4) return fail(KPM_EINVAL, "unroll must be 1, 2 or 4");
+  if (unroll != 1 && unroll != 2) return fail(KPM_EINVAL, "unroll must be 1 or 2");
   if (grid < 0) return fail(KPM_EINVAL, "grid must be >= 0");
   StepConfig c;
   c.unroll = unroll;
