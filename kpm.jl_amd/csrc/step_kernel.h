@@ -78,6 +78,47 @@ struct Word<KPM_VEC_F32> {
 
 constexpr int kBlock = 256;  // 4 waves
 
+// Streamed operands (z = v_{n-1}, read once; out = v_{n+1}, not reread until
+// the next pass) use non-temporal loads/stores so they do not evict the
+// neighbour rows of x that the gathers reuse.  KPM_STREAM_NT=0 builds plain
+// accesses instead.
+#ifndef KPM_STREAM_NT
+#define KPM_STREAM_NT 1
+#endif
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+
+__device__ inline double2 ld_stream(const double2* p) {
+#if KPM_STREAM_NT
+  const nt_d2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(p));
+  return make_double2(v.x, v.y);
+#else
+  return *p;
+#endif
+}
+__device__ inline float4 ld_stream(const float4* p) {
+#if KPM_STREAM_NT
+  const nt_f4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+#else
+  return *p;
+#endif
+}
+__device__ inline void st_stream(double2* p, const double2& v) {
+#if KPM_STREAM_NT
+  __builtin_nontemporal_store(nt_d2{v.x, v.y}, reinterpret_cast<nt_d2*>(p));
+#else
+  *p = v;
+#endif
+}
+__device__ inline void st_stream(float4* p, const float4& v) {
+#if KPM_STREAM_NT
+  __builtin_nontemporal_store(nt_f4{v.x, v.y, v.z, v.w}, reinterpret_cast<nt_f4*>(p));
+#else
+  *p = v;
+#endif
+}
+
 template <int UNR>
 struct RowMeta {
   int row[UNR], s[UNR], cnt[UNR];
@@ -202,7 +243,7 @@ __global__ void __launch_bounds__(kBlock)
     for (int u = 0; u < UNR; ++u) {
       const int rc = R0.row[u] < NH ? R0.row[u] : 0;
       const size_t off = (size_t)rc * RW + wcol;
-      zv[u] = z[off];  // z is always a valid block; unused when beta == 0
+      zv[u] = ld_stream(z + off);  // z is always a valid block; unused when beta == 0
       if constexpr (MODE == MODE_DBL) sv[u] = x[off];
       if constexpr (MODE == MODE_LR) sv[u] = l[off];
     }
@@ -251,7 +292,7 @@ __global__ void __launch_bounds__(kBlock)
         oii[c] = has_z ? fma(alpha, yi[u][c], beta * zi[c]) : alpha * yi[u][c];
       }
       const T ow = WT::pack(orr, oii);
-      if (live) out[(size_t)R0.row[u] * RW + wcol] = ow;
+      if (live) st_stream(out + (size_t)R0.row[u] * RW + wcol, ow);
       if constexpr (MODE != MODE_NONE) {
         double sr[CPL], si[CPL], ar[CPL], ai[CPL];
         WT::unpack(ow, sr, si);

@@ -12,7 +12,8 @@ import threading
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libkpm_hip.so")
+# KPM_HIP_LIB overrides the in-tree build (e.g. a packaged copy of the library)
+LIB_PATH = os.environ.get("KPM_HIP_LIB") or os.path.join(_HERE, "libkpm_hip.so")
 
 KPM_OK = 0
 KPM_EINVAL = -1
