@@ -158,6 +158,26 @@ int kpm_3d(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, const kpm_csr* Jb,
            const double* psi_r, int64_t K_left, int64_t K_right, int vec_prec, int gemm_mode,
            double* mu);
 
+/* ---- batched realisations ---------------------------------------------------
+ * No single-call counterpart in the reference, which runs one disorder
+ * realisation per process and calls kpm_1d! on each
+ * (scripts/anderson_seebeck_ensemble.jl:1-20, src/moment.jl:420-492).
+ * kpm_batch_create packs nreal independent operators, given as ONE
+ * block-diagonal Julia CSC matrix with realisation r on rows/columns
+ * [row_offsets[r], row_offsets[r+1]), into a padded device CSR. */
+typedef struct kpm_batch kpm_batch;
+int kpm_batch_create(kpm_ctx* ctx, int nreal, const int64_t* row_offsets, const int64_t* colptr,
+                     const int64_t* rowval, const double* nzval, int index_base, kpm_batch** out);
+int kpm_batch_destroy(kpm_batch* B);
+/* kpm_1d! equal-vector doubling moments of every realisation in one fused
+ * launch per step.  psi: the stacked probes (row_offsets[nreal] x NR,
+ * column-major ComplexF64); mu: nreal blocks of NR x NC column-major
+ * ComplexF64.  use_graph != 0 records the NC/2 steps once into a HIP graph
+ * and replays it on later calls of the same shape. */
+int kpm_batch_1d(kpm_ctx* ctx, kpm_batch* B, int64_t NC, int64_t NR, const double* psi, double* mu,
+                 int vec_prec, int use_graph);
+int kpm_batch_info(const kpm_batch* B, int64_t* padded_rows, int64_t* nnz, int* nseg);
+
 /* ---- measurement ---------------------------------------------------------
  * Launch the fused recurrence+doubling-dot pass `nsteps` times on the
  * context stream over blocks X (v_n) and Z (v_{n-1}, overwritten), bracketed

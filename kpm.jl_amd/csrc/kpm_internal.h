@@ -20,6 +20,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/kpm_hip.h"
 
@@ -99,11 +100,17 @@ struct kpm_block {
 namespace kpm {
 
 int ensure(kpm_ctx* ctx, Buffer& b, size_t bytes);
+// host CSR (0-based int64 rowptr/colind, interleaved complex values) -> device CSR
+int upload_csr(kpm_ctx* ctx, int64_t n, const std::vector<int64_t>& rp, const std::vector<int64_t>& ci,
+               const std::vector<double>& v_cplx, kpm_csr** out);
+// validate Julia CSC arrays (square n x n, index base 0 or 1)
+int check_csc(int64_t n, const int64_t* colptr, const int64_t* rowval, int base);
 // A non-owning block view over a cached workspace buffer.
 int work_block(kpm_ctx* ctx, Buffer& buf, int64_t NH, int64_t NR, int prec, kpm_block* out);
 
 // ---- kernels (spmm.hip) -------------------------------------------------------
-enum StepMode { MODE_NONE = 0, MODE_DBL = 1, MODE_LR = 2 };
+// MODE_DBL_SEG: MODE_DBL on the segmented schedule of the batched engine
+enum StepMode { MODE_NONE = 0, MODE_DBL = 1, MODE_LR = 2, MODE_DBL_SEG = 3 };
 
 // One fused pass out <- alpha*A*x + beta*z with optional dot partials:
 //   MODE_DBL: d0 = <x|x>, d1 = <x|out>;  MODE_LR: d0 = <l|out>.
@@ -123,6 +130,19 @@ struct StepConfig {
   int grid = 0;       // workgroups (multiple of 8); 0 = auto
 };
 int launch_step(kpm_ctx* ctx, const StepLaunch& s, const StepConfig& cfg, int* grid_used);
+// geometry of the fused pass for a block shape: lanes per row, row-chunks,
+// rows per tile, nonzeros per row in flight, tail flag
+struct StepGeom {
+  int RW, lpr, chunks, unr, kb, has_tail;
+  int64_t TR;
+};
+StepGeom step_geom(const kpm_csr* A, int64_t NR, int prec, const StepConfig& cfg);
+// persistent grid for ntiles row tiles (multiple of 8, up to 8 workgroups per CU)
+int default_grid(kpm_ctx* ctx, int64_t ntiles);
+// segmented (batched) MODE_DBL pass on G workgroups with per-workgroup tile
+// ranges tile_lo[G+1], tile realisations treal[ntiles], segment bases seg0[G]
+int launch_step_seg(kpm_ctx* ctx, const StepLaunch& s, const StepGeom& g, int G, int64_t ntiles,
+                    const int* tile_lo, const int* treal, const int* seg0, double2* partials);
 // dot-only pass: d0 = <a|b> per column, partials into ctx->partials
 int launch_dot(kpm_ctx* ctx, const kpm_block* a, const kpm_block* b, int* grid_used);
 // Reduce partials [grid][nd][NRp] in a fixed order and write moment columns:

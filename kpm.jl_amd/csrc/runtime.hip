@@ -63,7 +63,7 @@ static void free_buffer(Buffer& b) {
 
 // Build device CSR arrays from host (rowptr, colind, vals) already in CSR
 // form with 0-based int64 indices.
-static int upload_csr(kpm_ctx* ctx, int64_t n, const std::vector<int64_t>& rp,
+int upload_csr(kpm_ctx* ctx, int64_t n, const std::vector<int64_t>& rp,
                       const std::vector<int64_t>& ci, const std::vector<double>& v_cplx,
                       kpm_csr** out) {
   const int64_t nnz = rp[n];
@@ -119,7 +119,7 @@ static int upload_csr(kpm_ctx* ctx, int64_t n, const std::vector<int64_t>& rp,
   return KPM_OK;
 }
 
-static int check_csc(int64_t n, const int64_t* colptr, const int64_t* rowval, int base) {
+int check_csc(int64_t n, const int64_t* colptr, const int64_t* rowval, int base) {
   if (n <= 0) return fail(KPM_EINVAL, "operator dimension must be positive");
   if (base != 0 && base != 1) return fail(KPM_EINVAL, "index_base must be 0 or 1");
   if (!colptr) return fail(KPM_EINVAL, "null colptr");

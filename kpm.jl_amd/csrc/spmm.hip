@@ -168,36 +168,56 @@ int row_words(int64_t NR, int prec) {
   return (int)((w + 63) / 64 * 64);
 }
 
-static int default_grid(kpm_ctx* ctx, int64_t ntiles) {
+int default_grid(kpm_ctx* ctx, int64_t ntiles) {
   const int64_t cap = (int64_t)std::max(ctx->num_cus, 8) * 8;  // 8 workgroups of 4 waves per CU
   int64_t g = std::min<int64_t>(ntiles, cap);
   g = (g + 7) / 8 * 8;
   return (int)std::max<int64_t>(g, 8);
 }
 
+StepGeom step_geom(const kpm_csr* A, int64_t NR, int prec, const StepConfig& cfg) {
+  StepGeom g;
+  g.RW = row_words(NR, prec);
+  g.lpr = std::min(g.RW, 64);
+  g.chunks = g.RW / g.lpr;
+  g.kb = A->max_row_nnz <= 4 ? 4 : (A->max_row_nnz <= 8 ? 8 : 16);
+  g.unr = (cfg.unroll == 1 || g.kb == 16) ? 1 : 2;
+  g.has_tail = A->max_row_nnz > g.kb ? 1 : 0;
+  g.TR = 4LL * (64 / g.lpr) * g.unr;
+  return g;
+}
+
+static StepLauncher pick(int prec, const StepGeom& g, bool realh, int mode) {
+  return prec == KPM_VEC_F32 ? pick_step_f32(g.lpr, g.unr, g.kb, realh, mode)
+                             : pick_step_f64(g.lpr, g.unr, g.kb, realh, mode);
+}
+
 int launch_step(kpm_ctx* ctx, const StepLaunch& s, const StepConfig& cfg, int* grid_used) {
   const kpm_block* x = s.x;
   const int prec = x->prec;
-  const int RW = row_words(x->NR, prec);
-  const int lpr = std::min(RW, 64);
-  const int chunks = RW / lpr;
-  const int kb = s.A->max_row_nnz <= 4 ? 4 : (s.A->max_row_nnz <= 8 ? 8 : 16);
-  const int unr = (cfg.unroll == 1 || kb == 16) ? 1 : 2;
-  const int has_tail = s.A->max_row_nnz > kb ? 1 : 0;
-  const int rpw = 64 / lpr;
-  const int64_t TR = 4LL * rpw * unr;
-  const int64_t ntiles = (x->NH + TR - 1) / TR;
+  const StepGeom g = step_geom(s.A, x->NR, prec, cfg);
+  const int64_t ntiles = (x->NH + g.TR - 1) / g.TR;
   if (ntiles > INT32_MAX) return fail(KPM_EINVAL, "row count too large for the recurrence kernel");
   int G = cfg.grid > 0 ? (cfg.grid + 7) / 8 * 8 : default_grid(ctx, ntiles);
   const int nd = s.mode == MODE_DBL ? 2 : (s.mode == MODE_LR ? 1 : 0);
   if (nd > 0) KPM_TRY(ensure(ctx, ctx->partials, (size_t)G * nd * x->NRp * sizeof(double2)));
-  StepLauncher fn = prec == KPM_VEC_F32 ? pick_step_f32(lpr, unr, kb, s.A->is_real != 0, s.mode)
-                                        : pick_step_f64(lpr, unr, kb, s.A->is_real != 0, s.mode);
-  fn(dim3(G, chunks), ctx->stream, s.A->rowptr, s.A->colind, s.A->vals, x->d, s.z ? s.z->d : x->d,
-     s.out->d, s.l ? s.l->d : nullptr, (double2*)ctx->partials.ptr, (int)x->NH, RW, (int)x->NRp,
-     (int)ntiles, s.alpha, s.beta, has_tail);
+  StepLauncher fn = pick(prec, g, s.A->is_real != 0, s.mode);
+  fn(dim3(G, g.chunks), ctx->stream, s.A->rowptr, s.A->colind, s.A->vals, x->d, s.z ? s.z->d : x->d,
+     s.out->d, s.l ? s.l->d : nullptr, (double2*)ctx->partials.ptr, (int)x->NH, g.RW, (int)x->NRp,
+     (int)ntiles, s.alpha, s.beta, g.has_tail, nullptr, nullptr, nullptr);
   KPM_HIP_TRY(hipGetLastError());
   if (grid_used) *grid_used = G;
+  return KPM_OK;
+}
+
+int launch_step_seg(kpm_ctx* ctx, const StepLaunch& s, const StepGeom& g, int G, int64_t ntiles,
+                    const int* tile_lo, const int* treal, const int* seg0, double2* partials) {
+  const kpm_block* x = s.x;
+  StepLauncher fn = pick(x->prec, g, s.A->is_real != 0, MODE_DBL_SEG);
+  fn(dim3(G, g.chunks), ctx->stream, s.A->rowptr, s.A->colind, s.A->vals, x->d, s.z ? s.z->d : x->d,
+     s.out->d, nullptr, partials, (int)x->NH, g.RW, (int)x->NRp, (int)ntiles, s.alpha, s.beta,
+     g.has_tail, tile_lo, treal, seg0);
+  KPM_HIP_TRY(hipGetLastError());
   return KPM_OK;
 }
 

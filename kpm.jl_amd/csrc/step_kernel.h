@@ -128,13 +128,21 @@ struct ColMeta {
   int c[UNR][KB];
 };
 
-template <int PREC, int LPR, int UNR, int KB, bool REALH, int MODE>
+// Segmented schedule (SEG = true, batched realisations): workgroup b owns the
+// contiguous tiles [tile_lo[b], tile_lo[b+1]); treal[t] is the realisation of
+// tile t (tiles never straddle realisations), and the workgroup flushes its
+// dot partials to slot seg0[b] + k whenever the realisation changes.  The
+// change is uniform over the workgroup, so the barrier inside the flush is
+// reached by all of its threads.
+template <int PREC, int LPR, int UNR, int KB, bool REALH, int MODE, bool SEG>
 __global__ void __launch_bounds__(kBlock)
     k_step(const int* __restrict__ rowptr, const int* __restrict__ colind,
            const void* __restrict__ vals_, const typename Word<PREC>::T* __restrict__ x,
            const typename Word<PREC>::T* z, typename Word<PREC>::T* out,
            const typename Word<PREC>::T* __restrict__ l, double2* __restrict__ partials, int NH,
-           int RW, int NRp, int ntiles, double alpha, double beta, int has_tail) {
+           int RW, int NRp, int ntiles, double alpha, double beta, int has_tail,
+           const int* __restrict__ tile_lo, const int* __restrict__ treal,
+           const int* __restrict__ seg0) {
   using WT = Word<PREC>;
   using T = typename WT::T;
   constexpr int CPL = WT::CPL;
@@ -163,11 +171,17 @@ __global__ void __launch_bounds__(kBlock)
 
   const int G = gridDim.x;
   const int b = blockIdx.x;
-  const int xcd = b & 7;
-  const int q = b >> 3;
-  const int Bx = G >> 3;
-  const int t0 = (int)(((int64_t)ntiles * xcd) >> 3);
-  const int t1 = (int)(((int64_t)ntiles * (xcd + 1)) >> 3);
+  int tstart, t1, Bx;
+  if constexpr (SEG) {
+    tstart = tile_lo[b];
+    t1 = tile_lo[b + 1];
+    Bx = 1;
+  } else {
+    const int xcd = b & 7;
+    Bx = G >> 3;
+    tstart = (int)(((int64_t)ntiles * xcd) >> 3) + (b >> 3);
+    t1 = (int)(((int64_t)ntiles * (xcd + 1)) >> 3);
+  }
 
   // row pointers of tile t (rows beyond the matrix or the range read row 0
   // and get count 0)
@@ -210,7 +224,39 @@ __global__ void __launch_bounds__(kBlock)
     }
   };
 
-  int t = t0 + q;
+  // fixed-order workgroup reduction of the per-lane dot sums into dst
+  __shared__ double2 red[ND > 0 ? kBlock * NDA * CPL : 1];
+  auto flush = [&](double2* dst) {
+#pragma unroll
+    for (int d = 0; d < NDA; ++d)
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) red[(d * CPL + c) * kBlock + tid] = make_double2(accr[d][c], acci[d][c]);
+    __syncthreads();
+    constexpr int nout = LPR * CPL * ND;
+    for (int o = tid; o < nout; o += kBlock) {
+      const int d = o / (LPR * CPL);
+      const int rem = o % (LPR * CPL);
+      const int sb = rem / CPL;
+      const int c = rem % CPL;
+      double sr = 0.0, si = 0.0;
+      for (int w = 0; w < 4; ++w)
+        for (int rs = 0; rs < RPW; ++rs) {
+          const double2 v = red[(d * CPL + c) * kBlock + w * 64 + rs * LPR + sb];
+          sr += v.x;
+          si += v.y;
+        }
+      const int col = (blockIdx.y * LPR + sb) * CPL + c;
+      dst[(size_t)d * NRp + col] = make_double2(sr, si);
+    }
+    __syncthreads();
+  };
+  int seg = 0, cur = 0;
+  if constexpr (SEG) {
+    seg = seg0[b];
+    cur = tstart < t1 ? treal[tstart] : -1;
+  }
+
+  int t = tstart;
   RowMeta<UNR> R0, R1;
   ColMeta<UNR, KB> C0;
   load_rows(t, R0);
@@ -322,57 +368,47 @@ __global__ void __launch_bounds__(kBlock)
     R0 = R1;
     C0 = C1;
     R1 = R2;
-  }
-
-  if constexpr (ND > 0) {
-    __shared__ double2 red[kBlock * NDA * CPL];
+    if constexpr (SEG && ND > 0) {
+      const int nxt = t + 1 < t1 ? treal[t + 1] : -1;  // uniform over the workgroup
+      if (nxt != cur) {
+        flush(partials + (size_t)seg * ND * NRp);
 #pragma unroll
-    for (int d = 0; d < NDA; ++d)
+        for (int d = 0; d < NDA; ++d)
 #pragma unroll
-      for (int c = 0; c < CPL; ++c) red[(d * CPL + c) * kBlock + tid] = make_double2(accr[d][c], acci[d][c]);
-    __syncthreads();
-    constexpr int nout = LPR * CPL * ND;
-    for (int o = tid; o < nout; o += kBlock) {
-      const int d = o / (LPR * CPL);
-      const int rem = o % (LPR * CPL);
-      const int sb = rem / CPL;
-      const int c = rem % CPL;
-      double sr = 0.0, si = 0.0;
-      for (int w = 0; w < 4; ++w)
-        for (int rs = 0; rs < RPW; ++rs) {
-          const double2 v = red[(d * CPL + c) * kBlock + w * 64 + rs * LPR + sb];
-          sr += v.x;
-          si += v.y;
-        }
-      const int col = (blockIdx.y * LPR + sb) * CPL + c;
-      partials[((size_t)b * ND + d) * NRp + col] = make_double2(sr, si);
+          for (int c = 0; c < CPL; ++c) accr[d][c] = acci[d][c] = 0.0;
+        ++seg;
+        cur = nxt;
+      }
     }
   }
+
+  if constexpr (!SEG && ND > 0) flush(partials + (size_t)b * ND * NRp);
 }
 
 // ---- launcher tables -------------------------------------------------------------
 
 using StepLauncher = void (*)(dim3, hipStream_t, const int*, const int*, const void*, const void*,
                               const void*, void*, const void*, double2*, int, int, int, int,
-                              double, double, int);
+                              double, double, int, const int*, const int*, const int*);
 
-template <int PREC, int LPR, int UNR, int KB, bool REALH, int MODE>
+template <int PREC, int LPR, int UNR, int KB, bool REALH, int MODE, bool SEG>
 static void launch_step_t(dim3 grid, hipStream_t st, const int* rp, const int* ci, const void* v,
                           const void* x, const void* z, void* out, const void* l, double2* part,
                           int NH, int RW, int NRp, int ntiles, double alpha, double beta,
-                          int has_tail) {
+                          int has_tail, const int* tile_lo, const int* treal, const int* seg0) {
   using T = typename Word<PREC>::T;
-  hipLaunchKernelGGL((k_step<PREC, LPR, UNR, KB, REALH, MODE>), grid, dim3(kBlock), 0, st, rp, ci,
-                     v, (const T*)x, (const T*)z, (T*)out, (const T*)l, part, NH, RW, NRp, ntiles,
-                     alpha, beta, has_tail);
+  hipLaunchKernelGGL((k_step<PREC, LPR, UNR, KB, REALH, MODE, SEG>), grid, dim3(kBlock), 0, st, rp,
+                     ci, v, (const T*)x, (const T*)z, (T*)out, (const T*)l, part, NH, RW, NRp,
+                     ntiles, alpha, beta, has_tail, tile_lo, treal, seg0);
 }
 
 template <int PREC, int LPR, int UNR, int KB, bool REALH>
 static StepLauncher pick_mode(int mode) {
   switch (mode) {
-    case MODE_DBL: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_DBL>;
-    case MODE_LR: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_LR>;
-    default: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_NONE>;
+    case MODE_DBL: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_DBL, false>;
+    case MODE_LR: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_LR, false>;
+    case MODE_DBL_SEG: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_DBL, true>;
+    default: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_NONE, false>;
   }
 }
 

@@ -14,6 +14,7 @@ from .applications import (chebyshev_lin_trans, chebyshevT, chebyshevT_0, chebys
 from .applications import dos as dos_raw
 from .applications import dos0 as dos0_raw
 from .applications import kubo_bastin_cond as kubo_bastin_cond_raw
+from .batch import BatchOperator, kpm_batch_1d
 from .device import (ACTIVE_DEVICE, CPUDevice, DeviceBlock, DeviceCSR, HIPDevice, activate,
                      device_free_memory, get_context, maybe_on_device_zeros, maybe_to_host,
                      to_device, whichcore)

@@ -79,6 +79,10 @@ _SIGS = {
                          C.c_int, C.c_int, C.c_int, _pd]),
     "kpm_3d": (C.c_int, [_p, _p, _p, _p, _p, _i64, _i64, _i64, _pd, _pd, _i64, _i64, C.c_int,
                          C.c_int, _pd]),
+    "kpm_batch_create": (C.c_int, [_p, C.c_int, _pi64, _pi64, _pi64, _pd, C.c_int, C.POINTER(_p)]),
+    "kpm_batch_destroy": (C.c_int, [_p]),
+    "kpm_batch_1d": (C.c_int, [_p, _p, _i64, _i64, _pd, _pd, C.c_int, C.c_int]),
+    "kpm_batch_info": (C.c_int, [_p, _pi64, _pi64, C.POINTER(C.c_int)]),
     "kpm_bench_step": (C.c_int, [_p, _p, _p, _p, _i64, C.POINTER(C.c_float)]),
     "kpm_bench_copy": (C.c_int, [_p, _p, _p, _i64, C.POINTER(C.c_float)]),
     "kpm_event_record": (C.c_int, [_p, C.c_int]),
