@@ -158,6 +158,19 @@ int kpm_3d(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, const kpm_csr* Jb,
            const double* psi_r, int64_t K_left, int64_t K_right, int vec_prec, int gemm_mode,
            double* mu);
 
+/* ---- reverse-mode gradient ---------------------------------------------------
+ * No counterpart in the reference.  For the equal-vector moments
+ * mu_n = (1/NR) sum_r <psi_r|T_n(H)|psi_r> (n < NC; the moments of
+ * kpm_1d!, src/moment.jl:420-492) and g[n] = dL/dmu_n (NC reals), writes
+ * grad[k] = dL/dH_k for every stored entry k of H, in the order of the CSC
+ * arrays H was built from (interleaved complex; dL = sum_k grad[k] dH_k).
+ * Reverse Chebyshev (Clenshaw) recurrence with the forward vectors
+ * recomputed from checkpoints every ckpt_stride steps (0: ceil(sqrt(NC))).
+ * H must come from kpm_csr_from_csc. */
+int kpm_1d_adjoint(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t NR, int64_t NH,
+                   const double* psi, const double* g, double* grad, int vec_prec,
+                   int64_t ckpt_stride);
+
 /* ---- batched realisations ---------------------------------------------------
  * No single-call counterpart in the reference, which runs one disorder
  * realisation per process and calls kpm_1d! on each
@@ -176,6 +189,15 @@ int kpm_batch_destroy(kpm_batch* B);
  * and replays it on later calls of the same shape. */
 int kpm_batch_1d(kpm_ctx* ctx, kpm_batch* B, int64_t NC, int64_t NR, const double* psi, double* mu,
                  int vec_prec, int use_graph);
+/* The two halves of kpm_batch_1d: stage the stacked probes into a resident
+ * padded device block, then run the recurrence from it (the probes stay
+ * resident for repeated runs). */
+int kpm_batch_upload(kpm_ctx* ctx, kpm_batch* B, int64_t NR, const double* psi, int vec_prec);
+int kpm_batch_run(kpm_ctx* ctx, kpm_batch* B, int64_t NC, double* mu, int use_graph);
+/* Launch-shape tuner of the batched pass (no reference counterpart): times
+ * every candidate on the uploaded probes and records the fastest. */
+int kpm_batch_autotune(kpm_ctx* ctx, kpm_batch* B, int reps, int* best_unroll, int* best_grid,
+                       float* best_ms_per_step);
 int kpm_batch_info(const kpm_batch* B, int64_t* padded_rows, int64_t* nnz, int* nseg);
 
 /* ---- measurement ---------------------------------------------------------

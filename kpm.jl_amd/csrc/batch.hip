@@ -34,6 +34,9 @@ struct kpm_batch {
   kpm_csr* A = nullptr;       // padded block-diagonal operator
   kpm::BatchSched sched;
   kpm::Buffer partials;
+  kpm::Buffer probes;  // resident padded probe block (kpm_batch_upload)
+  int64_t probe_NR = 0;
+  int probe_prec = -1;
   // recorded step sequence
   hipGraphExec_t graph = nullptr;
   std::tuple<int64_t, int64_t, int, int64_t, int, const void*, const void*, const void*, const void*,
@@ -226,14 +229,40 @@ int kpm_batch_destroy(kpm_batch* B) {
   return KPM_OK;
 }
 
-int kpm_batch_1d(kpm_ctx* ctx, kpm_batch* B, int64_t NC, int64_t NR, const double* psi, double* mu,
-                 int vec_prec, int use_graph) {
-  if (!ctx || !B || !psi || !mu) return fail(KPM_EINVAL, "null argument");
-  if (NC % 2 != 0 || NC < 2) return fail(KPM_EINVAL, "Invalid NC: NC should be even.");
+int kpm_batch_upload(kpm_ctx* ctx, kpm_batch* B, int64_t NR, const double* psi, int vec_prec) {
+  if (!ctx || !B || !psi) return fail(KPM_EINVAL, "null argument");
   if (NR <= 0) return fail(KPM_EINVAL, "NR must be positive");
   if (vec_prec != KPM_VEC_F64 && vec_prec != KPM_VEC_F32) return fail(KPM_EINVAL, "bad vec_prec");
   KPM_HIP_TRY(hipSetDevice(ctx->device));
   const int64_t P = B->poff[B->nreal], n = B->off[B->nreal];
+  kpm_block pb;
+  KPM_TRY(work_block(ctx, B->probes, P, NR, vec_prec, &pb));
+  // stacked (n x NR, column-major) -> padded rows, then the device layout
+  std::vector<double> padded((size_t)P * NR * 2, 0.0);
+  for (int64_t c = 0; c < NR; ++c)
+    for (int r = 0; r < B->nreal; ++r) {
+      const int64_t len = B->off[r + 1] - B->off[r];
+      std::copy(psi + 2 * (c * n + B->off[r]), psi + 2 * (c * n + B->off[r] + len),
+                padded.data() + 2 * (c * P + B->poff[r]));
+    }
+  KPM_TRY(ensure(ctx, ctx->staging, padded.size() * sizeof(double)));
+  KPM_HIP_TRY(hipMemcpyAsync(ctx->staging.ptr, padded.data(), padded.size() * sizeof(double),
+                             hipMemcpyHostToDevice, ctx->stream));
+  KPM_TRY(launch_upload(ctx, ctx->staging.ptr, &pb));
+  KPM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  B->probe_NR = NR;
+  B->probe_prec = vec_prec;
+  return KPM_OK;
+}
+
+int kpm_batch_run(kpm_ctx* ctx, kpm_batch* B, int64_t NC, double* mu, int use_graph) {
+  if (!ctx || !B || !mu) return fail(KPM_EINVAL, "null argument");
+  if (B->probe_prec < 0) return fail(KPM_EINVAL, "no probes uploaded (kpm_batch_upload)");
+  if (NC % 2 != 0 || NC < 2) return fail(KPM_EINVAL, "Invalid NC: NC should be even.");
+  KPM_HIP_TRY(hipSetDevice(ctx->device));
+  const int64_t NR = B->probe_NR;
+  const int vec_prec = B->probe_prec;
+  const int64_t P = B->poff[B->nreal];
   const StepConfig cfg = tuned_config(ctx, B->A, NR, vec_prec, MODE_DBL);
   const StepGeom g = step_geom(B->A, NR, vec_prec, cfg);
   const int64_t ntiles = P / g.TR;
@@ -241,25 +270,16 @@ int kpm_batch_1d(kpm_ctx* ctx, kpm_batch* B, int64_t NC, int64_t NR, const doubl
   KPM_TRY(build_sched(B, g.TR, G));
   const int64_t NRp = padded_nr(NR, vec_prec);
   KPM_TRY(ensure(ctx, B->partials, (size_t)std::max(B->sched.nseg, 1) * 2 * NRp * sizeof(double2)));
-  kpm_block w0, w1;
+  kpm_block w0, w1, pb;
   KPM_TRY(work_block(ctx, ctx->work[0], P, NR, vec_prec, &w0));
   KPM_TRY(work_block(ctx, ctx->work[1], P, NR, vec_prec, &w1));
+  KPM_TRY(work_block(ctx, B->probes, P, NR, vec_prec, &pb));
   KPM_TRY(ensure(ctx, ctx->mu, (size_t)B->nreal * NR * NC * sizeof(double2)));
-  // probes: stacked (n x NR, column-major) -> padded rows, then device layout
-  {
-    std::vector<double> padded((size_t)P * NR * 2, 0.0);
-    for (int64_t c = 0; c < NR; ++c)
-      for (int r = 0; r < B->nreal; ++r) {
-        const int64_t len = B->off[r + 1] - B->off[r];
-        std::copy(psi + 2 * (c * n + B->off[r]), psi + 2 * (c * n + B->off[r] + len),
-                  padded.data() + 2 * (c * P + B->poff[r]));
-      }
-    KPM_TRY(ensure(ctx, ctx->staging, padded.size() * sizeof(double)));
-    KPM_HIP_TRY(hipMemcpyAsync(ctx->staging.ptr, padded.data(), padded.size() * sizeof(double),
-                               hipMemcpyHostToDevice, ctx->stream));
-    KPM_TRY(launch_upload(ctx, ctx->staging.ptr, &w0));
-    KPM_HIP_TRY(hipStreamSynchronize(ctx->stream));
-  }
+  auto run = [&]() -> int {
+    KPM_HIP_TRY(hipMemcpyAsync(w0.d, pb.d, (size_t)(P * NRp) * elem_bytes(vec_prec),
+                               hipMemcpyDeviceToDevice, ctx->stream));
+    return batch_steps(ctx, B, NC, g, &w0, &w1, ctx->mu.ptr);
+  };
   if (use_graph) {
     auto key = std::make_tuple(NC, NR, vec_prec, g.TR, G, (const void*)w0.d, (const void*)w1.d,
                                (const void*)ctx->mu.ptr, (const void*)B->partials.ptr,
@@ -271,7 +291,7 @@ int kpm_batch_1d(kpm_ctx* ctx, kpm_batch* B, int64_t NC, int64_t NR, const doubl
       }
       hipGraph_t graph;
       KPM_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      int rc = batch_steps(ctx, B, NC, g, &w0, &w1, ctx->mu.ptr);
+      int rc = run();
       hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
       if (rc != KPM_OK) return rc;
       if (e != hipSuccess) return fail(KPM_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
@@ -282,11 +302,74 @@ int kpm_batch_1d(kpm_ctx* ctx, kpm_batch* B, int64_t NC, int64_t NR, const doubl
     }
     KPM_HIP_TRY(hipGraphLaunch(B->graph, ctx->stream));
   } else {
-    KPM_TRY(batch_steps(ctx, B, NC, g, &w0, &w1, ctx->mu.ptr));
+    KPM_TRY(run());
   }
   KPM_HIP_TRY(hipMemcpyAsync(mu, ctx->mu.ptr, (size_t)B->nreal * NR * NC * sizeof(double2),
                              hipMemcpyDeviceToHost, ctx->stream));
   KPM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return KPM_OK;
+}
+
+int kpm_batch_1d(kpm_ctx* ctx, kpm_batch* B, int64_t NC, int64_t NR, const double* psi, double* mu,
+                 int vec_prec, int use_graph) {
+  if (NC % 2 != 0 || NC < 2) return fail(KPM_EINVAL, "Invalid NC: NC should be even.");
+  KPM_TRY(kpm_batch_upload(ctx, B, NR, psi, vec_prec));
+  return kpm_batch_run(ctx, B, NC, mu, use_graph);
+}
+
+// Time every candidate geometry of the segmented pass on the uploaded
+// probes (work blocks are overwritten) and record the fastest for this
+// shape; returns it.
+int kpm_batch_autotune(kpm_ctx* ctx, kpm_batch* B, int reps, int* best_unroll, int* best_grid,
+                       float* best_ms_per_step) {
+  if (!ctx || !B || reps < 1) return fail(KPM_EINVAL, "bad arguments");
+  if (B->probe_prec < 0) return fail(KPM_EINVAL, "no probes uploaded (kpm_batch_upload)");
+  KPM_HIP_TRY(hipSetDevice(ctx->device));
+  const int64_t NR = B->probe_NR;
+  const int prec = B->probe_prec;
+  const int64_t P = B->poff[B->nreal];
+  const int64_t NRp = padded_nr(NR, prec);
+  kpm_block w0, w1, pb;
+  KPM_TRY(work_block(ctx, ctx->work[0], P, NR, prec, &w0));
+  KPM_TRY(work_block(ctx, ctx->work[1], P, NR, prec, &w1));
+  KPM_TRY(work_block(ctx, B->probes, P, NR, prec, &pb));
+  KPM_HIP_TRY(hipMemcpyAsync(w0.d, pb.d, (size_t)(P * NRp) * elem_bytes(prec), hipMemcpyDeviceToDevice,
+                             ctx->stream));
+  KPM_HIP_TRY(hipMemcpyAsync(w1.d, pb.d, (size_t)(P * NRp) * elem_bytes(prec), hipMemcpyDeviceToDevice,
+                             ctx->stream));
+  float best = 1e30f;
+  StepConfig bestc;
+  for (int u : {1, 2})
+    for (int pc : {2, 4, 8, 16}) {
+      StepConfig c;
+      c.unroll = u;
+      c.grid = std::max(ctx->num_cus, 8) * pc;
+      const StepGeom g = step_geom(B->A, NR, prec, c);
+      KPM_TRY(build_sched(B, g.TR, c.grid));
+      KPM_TRY(ensure(ctx, B->partials, (size_t)std::max(B->sched.nseg, 1) * 2 * NRp * sizeof(double2)));
+      StepLaunch s{B->A, &w0, &w1, &w1, nullptr, 2.0, -1.0, MODE_DBL_SEG};
+      auto launch = [&]() {
+        return launch_step_seg(ctx, s, g, B->sched.G, B->sched.ntiles, (const int*)B->sched.tile_lo.ptr,
+                               (const int*)B->sched.treal.ptr, (const int*)B->sched.seg0.ptr,
+                               (double2*)B->partials.ptr);
+      };
+      KPM_TRY(launch());
+      KPM_HIP_TRY(hipEventRecord(ctx->events[12], ctx->stream));
+      for (int r = 0; r < reps; ++r) KPM_TRY(launch());
+      KPM_HIP_TRY(hipEventRecord(ctx->events[13], ctx->stream));
+      KPM_HIP_TRY(hipEventSynchronize(ctx->events[13]));
+      float ms = 0.f;
+      KPM_HIP_TRY(hipEventElapsedTime(&ms, ctx->events[12], ctx->events[13]));
+      ms /= (float)reps;
+      if (ms < best) {
+        best = ms;
+        bestc = c;
+      }
+    }
+  tune_record(ctx, B->A, NR, prec, bestc);
+  if (best_unroll) *best_unroll = bestc.unroll;
+  if (best_grid) *best_grid = bestc.grid;
+  if (best_ms_per_step) *best_ms_per_step = best;
   return KPM_OK;
 }
 

@@ -78,6 +78,9 @@ struct kpm_ctx {
 
 struct kpm_csr {
   kpm_ctx* ctx = nullptr;
+  // position in this CSR of the k-th stored entry of the CSC arrays it was
+  // built from (the adjoint returns gradients in the caller's CSC order)
+  std::vector<int64_t> csc_to_csr;
   int64_t n = 0;
   int64_t nnz = 0;
   int is_real = 0;
@@ -110,7 +113,8 @@ int work_block(kpm_ctx* ctx, Buffer& buf, int64_t NH, int64_t NR, int prec, kpm_
 
 // ---- kernels (spmm.hip) -------------------------------------------------------
 // MODE_DBL_SEG: MODE_DBL on the segmented schedule of the batched engine
-enum StepMode { MODE_NONE = 0, MODE_DBL = 1, MODE_LR = 2, MODE_DBL_SEG = 3 };
+// MODE_SRC: out += gamma * l (no dots), the reverse Clenshaw step of the adjoint
+enum StepMode { MODE_NONE = 0, MODE_DBL = 1, MODE_LR = 2, MODE_DBL_SEG = 3, MODE_SRC = 4 };
 
 // One fused pass out <- alpha*A*x + beta*z with optional dot partials:
 //   MODE_DBL: d0 = <x|x>, d1 = <x|out>;  MODE_LR: d0 = <l|out>.
@@ -123,6 +127,7 @@ struct StepLaunch {
   const kpm_block* l;
   double alpha, beta;
   int mode;
+  double gamma = 0.0;  // MODE_SRC only
 };
 // Launch geometry picked by the tuner for (LPR, rows per lane, grid size).
 struct StepConfig {
@@ -174,7 +179,8 @@ int launch_contract(kpm_ctx* ctx, const void* Lslots, int nL, const void* Rslots
                     int64_t K, int prec, int gemm_mode, double scale, void* mu_dev, int64_t NC,
                     int64_t n0, int64_t m0, int parity);
 
-// tuner hook (tune.cpp): best StepConfig for a shape, or defaults
+// tuner table (tune.hip): best StepConfig for a shape, or defaults; record one
 StepConfig tuned_config(kpm_ctx* ctx, const kpm_csr* A, int64_t NR, int prec, int mode);
+void tune_record(kpm_ctx* ctx, const kpm_csr* A, int64_t NR, int prec, const StepConfig& c);
 
 }  // namespace kpm

@@ -232,6 +232,7 @@ int kpm_csr_from_csc(kpm_ctx* ctx, int64_t n, const int64_t* colptr, const int64
   for (int64_t k = 0; k < nnz; ++k) rp[rowval[k] - index_base + 1]++;
   for (int64_t i = 0; i < n; ++i) rp[i + 1] += rp[i];
   std::vector<int64_t> next(rp.begin(), rp.end() - 1);
+  std::vector<int64_t> perm(nnz);
   for (int64_t j = 0; j < n; ++j)
     for (int64_t k = colptr[j] - index_base; k < colptr[j + 1] - index_base; ++k) {
       const int64_t r = rowval[k] - index_base;
@@ -239,9 +240,12 @@ int kpm_csr_from_csc(kpm_ctx* ctx, int64_t n, const int64_t* colptr, const int64
       ci[dst] = j;
       v[2 * dst] = nzval[2 * k];
       v[2 * dst + 1] = nzval[2 * k + 1];
+      perm[k] = dst;
     }
   KPM_HIP_TRY(hipSetDevice(ctx->device));
-  return upload_csr(ctx, n, rp, ci, v, out);
+  KPM_TRY(upload_csr(ctx, n, rp, ci, v, out));
+  (*out)->csc_to_csr = std::move(perm);
+  return KPM_OK;
 }
 
 int kpm_csr_adjoint_from_csc(kpm_ctx* ctx, int64_t n, const int64_t* colptr, const int64_t* rowval,

@@ -201,10 +201,11 @@ int launch_step(kpm_ctx* ctx, const StepLaunch& s, const StepConfig& cfg, int* g
   int G = cfg.grid > 0 ? (cfg.grid + 7) / 8 * 8 : default_grid(ctx, ntiles);
   const int nd = s.mode == MODE_DBL ? 2 : (s.mode == MODE_LR ? 1 : 0);
   if (nd > 0) KPM_TRY(ensure(ctx, ctx->partials, (size_t)G * nd * x->NRp * sizeof(double2)));
+  if ((s.mode == MODE_LR || s.mode == MODE_SRC) && !s.l) return fail(KPM_EINVAL, "missing operand block");
   StepLauncher fn = pick(prec, g, s.A->is_real != 0, s.mode);
   fn(dim3(G, g.chunks), ctx->stream, s.A->rowptr, s.A->colind, s.A->vals, x->d, s.z ? s.z->d : x->d,
      s.out->d, s.l ? s.l->d : nullptr, (double2*)ctx->partials.ptr, (int)x->NH, g.RW, (int)x->NRp,
-     (int)ntiles, s.alpha, s.beta, g.has_tail, nullptr, nullptr, nullptr);
+     (int)ntiles, s.alpha, s.beta, g.has_tail, nullptr, nullptr, nullptr, s.gamma);
   KPM_HIP_TRY(hipGetLastError());
   if (grid_used) *grid_used = G;
   return KPM_OK;
@@ -216,7 +217,7 @@ int launch_step_seg(kpm_ctx* ctx, const StepLaunch& s, const StepGeom& g, int G,
   StepLauncher fn = pick(x->prec, g, s.A->is_real != 0, MODE_DBL_SEG);
   fn(dim3(G, g.chunks), ctx->stream, s.A->rowptr, s.A->colind, s.A->vals, x->d, s.z ? s.z->d : x->d,
      s.out->d, nullptr, partials, (int)x->NH, g.RW, (int)x->NRp, (int)ntiles, s.alpha, s.beta,
-     g.has_tail, tile_lo, treal, seg0);
+     g.has_tail, tile_lo, treal, seg0, 0.0);
   KPM_HIP_TRY(hipGetLastError());
   return KPM_OK;
 }

@@ -10,6 +10,7 @@
 //   out[i,:] = alpha * sum_k A[i,k] x[k,:] + beta * z[i,:]
 //   MODE_DBL: d0[:] += conj(x[i,:]) x[i,:],  d1[:] += conj(x[i,:]) out[i,:]
 //   MODE_LR : d0[:] += conj(l[i,:]) out[i,:]
+//   MODE_SRC: out[i,:] += gamma * l[i,:], no dots (the adjoint's source term)
 // z may alias out (in place); when beta == 0 the launcher passes x as z and
 // the kernel selects (never multiplies) it away.
 //
@@ -142,7 +143,7 @@ __global__ void __launch_bounds__(kBlock)
            const typename Word<PREC>::T* __restrict__ l, double2* __restrict__ partials, int NH,
            int RW, int NRp, int ntiles, double alpha, double beta, int has_tail,
            const int* __restrict__ tile_lo, const int* __restrict__ treal,
-           const int* __restrict__ seg0) {
+           const int* __restrict__ seg0, double gamma) {
   using WT = Word<PREC>;
   using T = typename WT::T;
   constexpr int CPL = WT::CPL;
@@ -291,7 +292,7 @@ __global__ void __launch_bounds__(kBlock)
       const size_t off = (size_t)rc * RW + wcol;
       zv[u] = ld_stream(z + off);  // z is always a valid block; unused when beta == 0
       if constexpr (MODE == MODE_DBL) sv[u] = x[off];
-      if constexpr (MODE == MODE_LR) sv[u] = l[off];
+      if constexpr (MODE == MODE_LR || MODE == MODE_SRC) sv[u] = l[off];
     }
     // 2. prefetch: column indices of the next tile, row pointers of the one after
     ColMeta<UNR, KB> C1;
@@ -337,9 +338,18 @@ __global__ void __launch_bounds__(kBlock)
         orr[c] = has_z ? fma(alpha, yr[u][c], beta * zr[c]) : alpha * yr[u][c];
         oii[c] = has_z ? fma(alpha, yi[u][c], beta * zi[c]) : alpha * yi[u][c];
       }
+      if constexpr (MODE == MODE_SRC) {  // + gamma * s (reverse Clenshaw source)
+        double qr[CPL], qi[CPL];
+        WT::unpack(sv[u], qr, qi);
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+          orr[c] = fma(gamma, qr[c], orr[c]);
+          oii[c] = fma(gamma, qi[c], oii[c]);
+        }
+      }
       const T ow = WT::pack(orr, oii);
       if (live) st_stream(out + (size_t)R0.row[u] * RW + wcol, ow);
-      if constexpr (MODE != MODE_NONE) {
+      if constexpr (MODE == MODE_DBL || MODE == MODE_LR) {
         double sr[CPL], si[CPL], ar[CPL], ai[CPL];
         WT::unpack(ow, sr, si);
         WT::unpack(sv[u], ar, ai);
@@ -389,17 +399,18 @@ __global__ void __launch_bounds__(kBlock)
 
 using StepLauncher = void (*)(dim3, hipStream_t, const int*, const int*, const void*, const void*,
                               const void*, void*, const void*, double2*, int, int, int, int,
-                              double, double, int, const int*, const int*, const int*);
+                              double, double, int, const int*, const int*, const int*, double);
 
 template <int PREC, int LPR, int UNR, int KB, bool REALH, int MODE, bool SEG>
 static void launch_step_t(dim3 grid, hipStream_t st, const int* rp, const int* ci, const void* v,
                           const void* x, const void* z, void* out, const void* l, double2* part,
                           int NH, int RW, int NRp, int ntiles, double alpha, double beta,
-                          int has_tail, const int* tile_lo, const int* treal, const int* seg0) {
+                          int has_tail, const int* tile_lo, const int* treal, const int* seg0,
+                          double gamma) {
   using T = typename Word<PREC>::T;
   hipLaunchKernelGGL((k_step<PREC, LPR, UNR, KB, REALH, MODE, SEG>), grid, dim3(kBlock), 0, st, rp,
                      ci, v, (const T*)x, (const T*)z, (T*)out, (const T*)l, part, NH, RW, NRp,
-                     ntiles, alpha, beta, has_tail, tile_lo, treal, seg0);
+                     ntiles, alpha, beta, has_tail, tile_lo, treal, seg0, gamma);
 }
 
 template <int PREC, int LPR, int UNR, int KB, bool REALH>
@@ -408,6 +419,7 @@ static StepLauncher pick_mode(int mode) {
     case MODE_DBL: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_DBL, false>;
     case MODE_LR: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_LR, false>;
     case MODE_DBL_SEG: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_DBL, true>;
+    case MODE_SRC: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_SRC, false>;
     default: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_NONE, false>;
   }
 }

@@ -34,6 +34,11 @@ StepConfig tuned_config(kpm_ctx* ctx, const kpm_csr* A, int64_t NR, int prec, in
   return c;
 }
 
+void tune_record(kpm_ctx* ctx, const kpm_csr* A, int64_t NR, int prec, const StepConfig& c) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (int mode : {MODE_NONE, MODE_DBL, MODE_LR}) g_table[make_key(ctx, A, NR, prec, mode)] = c;
+}
+
 }  // namespace kpm
 
 using namespace kpm;
@@ -74,11 +79,7 @@ int kpm_autotune_step(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z
         bestc = c;
       }
     }
-  {
-    std::lock_guard<std::mutex> lk(g_mu);
-    for (int mode : {MODE_NONE, MODE_DBL, MODE_LR})
-      g_table[make_key(ctx, H, X->NR, X->prec, mode)] = bestc;
-  }
+  tune_record(ctx, H, X->NR, X->prec, bestc);
   if (best_unroll) *best_unroll = bestc.unroll;
   if (best_grid) *best_grid = bestc.grid;
   if (best_ms_per_step) *best_ms_per_step = best;
@@ -93,8 +94,7 @@ int kpm_tune_set(kpm_ctx* ctx, const kpm_csr* H, int64_t NR, int vec_prec, int u
   StepConfig c;
   c.unroll = unroll;
   c.grid = grid;
-  std::lock_guard<std::mutex> lk(g_mu);
-  for (int mode : {MODE_NONE, MODE_DBL, MODE_LR}) g_table[make_key(ctx, H, NR, vec_prec, mode)] = c;
+  tune_record(ctx, H, NR, vec_prec, c);
   return KPM_OK;
 }
 

@@ -9,6 +9,8 @@ interface: the moment engine (src/moment.jl), its primitives
 end (src/frontend.jl).
 """
 from ._lib import ArgumentError, KPMHipError, KPMHipMissing, load_library
+from .adjoint import (bond_gradient, dos_loss, kpm_1d_adjoint, moments_rrule,
+                      site_gradient)
 from .applications import (chebyshev_lin_trans, chebyshevT, chebyshevT_0, chebyshevT_xn,
                            fermiFunction, fermiFunctions, gausschebyshevt)
 from .applications import dos as dos_raw

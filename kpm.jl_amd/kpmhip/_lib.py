@@ -79,10 +79,15 @@ _SIGS = {
                          C.c_int, C.c_int, C.c_int, _pd]),
     "kpm_3d": (C.c_int, [_p, _p, _p, _p, _p, _i64, _i64, _i64, _pd, _pd, _i64, _i64, C.c_int,
                          C.c_int, _pd]),
+    "kpm_1d_adjoint": (C.c_int, [_p, _p, _i64, _i64, _i64, _pd, _pd, _pd, C.c_int, _i64]),
     "kpm_batch_create": (C.c_int, [_p, C.c_int, _pi64, _pi64, _pi64, _pd, C.c_int, C.POINTER(_p)]),
     "kpm_batch_destroy": (C.c_int, [_p]),
     "kpm_batch_1d": (C.c_int, [_p, _p, _i64, _i64, _pd, _pd, C.c_int, C.c_int]),
     "kpm_batch_info": (C.c_int, [_p, _pi64, _pi64, C.POINTER(C.c_int)]),
+    "kpm_batch_upload": (C.c_int, [_p, _p, _i64, _pd, C.c_int]),
+    "kpm_batch_run": (C.c_int, [_p, _p, _i64, _pd, C.c_int]),
+    "kpm_batch_autotune": (C.c_int, [_p, _p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_float)]),
     "kpm_bench_step": (C.c_int, [_p, _p, _p, _p, _i64, C.POINTER(C.c_float)]),
     "kpm_bench_copy": (C.c_int, [_p, _p, _p, _i64, C.POINTER(C.c_float)]),
     "kpm_event_record": (C.c_int, [_p, C.c_int]),

@@ -42,6 +42,37 @@ class BatchOperator:
         self.handle = h
         self.nreal = len(Hs)
 
+    def upload(self, psi, NR: int, precision="f64"):
+        """Stage the probes (per-realisation list or stacked block) on the device."""
+        NR = int(NR)
+        if isinstance(psi, (list, tuple)):
+            if len(psi) != self.nreal:
+                raise ArgumentError("one probe block per realisation is required")
+            psi = np.concatenate([np.asarray(p).reshape(p.shape[0], -1) for p in psi], axis=0)
+        psi = np.asfortranarray(psi, dtype=np.complex128)
+        if psi.shape != (int(self.offsets[-1]), NR):
+            raise AssertionError(
+                f"probe block has shape {psi.shape}; expected ({self.offsets[-1]}, {NR})")
+        check(self.ctx.lib.kpm_batch_upload(self.ctx.handle, self.handle, NR, dptr(psi),
+                                            prec_code(precision)), "kpm_batch_upload")
+        self.NR = NR
+
+    def run(self, NC: int, *, use_graph: bool = False) -> np.ndarray:
+        """Moments of every realisation from the resident probes: (nreal, NR, NC)."""
+        NC = int(NC)
+        if NC % 2 != 0:
+            raise AssertionError("Invalid NC: NC should be even.")
+        mu = np.zeros((self.nreal, NC, self.NR), dtype=np.complex128)
+        check(self.ctx.lib.kpm_batch_run(self.ctx.handle, self.handle, NC, dptr(mu),
+                                         1 if use_graph else 0), "kpm_batch_run")
+        return np.transpose(mu, (0, 2, 1)).copy()
+
+    def autotune(self, reps: int = 5) -> dict:
+        u, g, ms = C.c_int(), C.c_int(), C.c_float()
+        check(self.ctx.lib.kpm_batch_autotune(self.ctx.handle, self.handle, int(reps), C.byref(u),
+                                              C.byref(g), C.byref(ms)), "kpm_batch_autotune")
+        return {"unroll": u.value, "grid": g.value, "ms_per_pass": ms.value}
+
     def info(self) -> dict:
         pr, nnz, ns = C.c_int64(), C.c_int64(), C.c_int()
         check(self.ctx.lib.kpm_batch_info(self.handle, C.byref(pr), C.byref(nnz), C.byref(ns)),
@@ -70,18 +101,8 @@ def kpm_batch_1d(batch, NC: int, NR: int, psi, *, precision="f64", use_graph: bo
     NC, NR = int(NC), int(NR)
     if NC % 2 != 0:
         raise AssertionError("Invalid NC: NC should be even.")
-    if isinstance(psi, (list, tuple)):
-        if len(psi) != batch.nreal:
-            raise ArgumentError("one probe block per realisation is required")
-        psi = np.concatenate([np.asarray(p).reshape(p.shape[0], -1) for p in psi], axis=0)
-    psi = np.asfortranarray(psi, dtype=np.complex128)
-    if psi.shape != (int(batch.offsets[-1]), NR):
-        raise AssertionError(f"probe block has shape {psi.shape}; expected ({batch.offsets[-1]}, {NR})")
-    mu = np.zeros((batch.nreal, NC, NR), dtype=np.complex128)  # each realisation NR x NC col-major
-    ctx = batch.ctx
-    check(ctx.lib.kpm_batch_1d(ctx.handle, batch.handle, NC, NR, dptr(psi), dptr(mu),
-                               prec_code(precision), 1 if use_graph else 0), "kpm_batch_1d")
-    mu = np.transpose(mu, (0, 2, 1))  # (nreal, NR, NC)
+    batch.upload(psi, NR, precision)
+    mu = batch.run(NC, use_graph=use_graph)
     if avg_output:
         return mu.mean(axis=1).real.copy()
-    return mu.copy()
+    return mu

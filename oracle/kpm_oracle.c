@@ -414,3 +414,47 @@ int kpm_oracle_1d_steps(int64_t NH, const int64_t* colptr, const int64_t* rowval
     }
     return 0;
 }
+
+/* Forward-mode tangent of the direct recurrence (the derivative of kpm_1d's
+ * moments along a perturbation dH with H's sparsity pattern): alpha_0 = psi,
+ * alpha_1 = H psi, alpha_{k+1} = 2 H alpha_k - alpha_{k-1};
+ * dalpha_0 = 0, dalpha_1 = dH psi, dalpha_{k+1} = 2 dH alpha_k + 2 H dalpha_k - dalpha_{k-1};
+ * mu[:,n] = <psi|alpha_n>, dmu[:,n] = <psi|dalpha_n> (NR x NC each).
+ * dnzval holds dH's values in H's CSC order. */
+int kpm_oracle_1d_tangent(int64_t NH, const int64_t* colptr, const int64_t* rowval,
+                          const double* nzval, const double* dnzval, int64_t NC, int64_t NR,
+                          const double* psi, double* mu_all, double* dmu_all) {
+    if (NH <= 0 || NR <= 0 || NC < 2) return OR_EINVAL;
+    const cplx* nz = (const cplx*)nzval;
+    const cplx* dnz = (const cplx*)dnzval;
+    const cplx* p = (const cplx*)psi;
+    cplx* mu = (cplx*)mu_all;
+    cplx* dmu = (cplx*)dmu_all;
+    cplx* w = (cplx*)calloc((size_t)(5 * NH), sizeof(cplx));
+    if (!w) return OR_ENOMEM;
+    for (int64_t r = 0; r < NR; ++r) {
+        const cplx* v = p + r * NH;
+        cplx *a0 = w, *a1 = w + NH, *d0 = w + 2 * NH, *d1 = w + 3 * NH, *tmp = w + 4 * NH;
+        memcpy(a0, v, sizeof(cplx) * NH);
+        for (int64_t i = 0; i < NH; ++i) d0[i] = 0.0;
+        csc_mul5(NH, colptr, rowval, nz, a0, a1, 1.0, 0.0);
+        csc_mul5(NH, colptr, rowval, dnz, a0, d1, 1.0, 0.0);
+        mu[r] = vdot(NH, v, a0);
+        dmu[r] = 0.0;
+        mu[r + NR] = vdot(NH, v, a1);
+        dmu[r + NR] = vdot(NH, v, d1);
+        for (int64_t n = 2; n < NC; ++n) {
+            /* d_{n} = 2 dH a_{n-1} + 2 H d_{n-1} - d_{n-2}, into d0 (holds d_{n-2}) */
+            csc_mul5(NH, colptr, rowval, dnz, a1, tmp, 2.0, 0.0);
+            csc_mul5(NH, colptr, rowval, nz, d1, d0, 2.0, -1.0);
+            for (int64_t i = 0; i < NH; ++i) d0[i] += tmp[i];
+            csc_mul5(NH, colptr, rowval, nz, a1, a0, 2.0, -1.0);
+            mu[r + NR * n] = vdot(NH, v, a0);
+            dmu[r + NR * n] = vdot(NH, v, d0);
+            cplx* t = a0; a0 = a1; a1 = t;
+            t = d0; d0 = d1; d1 = t;
+        }
+    }
+    free(w);
+    return 0;
+}

@@ -54,6 +54,7 @@ def lib():
                                         _pd, _i, _i, _pd, _pd, _i, _i, _pd]),
             "kpm_oracle_cheb_iter": (C.c_int, [_i, _pi, _pi, _pd, _i, _i, _pd]),
             "kpm_oracle_1d_steps": (C.c_int, [_i, _pi, _pi, _pd, _i, _i, _pd, _pd, _pd]),
+            "kpm_oracle_1d_tangent": (C.c_int, [_i, _pi, _pi, _pd, _pd, _i, _i, _pd, _pd, _pd]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -135,6 +136,22 @@ def kpm_1d_current(H, J, NC: int, psi) -> np.ndarray:
     _check(lib().kpm_oracle_1d_current(*A.args(), *Jc.args3(), NC, NR, _dp(p), _dp(mu)),
            "kpm_1d_current")
     return mu
+
+
+def kpm_1d_tangent(H, dH_data, NC: int, psi):
+    """Moments of the direct recurrence and their derivative along dH, whose
+    values dH_data follow H's (sorted) CSC order: (mu, dmu), NR x NC each."""
+    A = CSC(H)
+    d = np.ascontiguousarray(dH_data, dtype=np.complex128)
+    if d.shape != A.nz.shape:
+        raise ValueError("dH_data must match H's stored entries")
+    p = _blk(psi)
+    NR = p.shape[1]
+    mu = np.zeros((NR, NC), dtype=np.complex128, order="F")
+    dmu = np.zeros((NR, NC), dtype=np.complex128, order="F")
+    _check(lib().kpm_oracle_1d_tangent(*A.args(), d.ctypes.data_as(_pd), NC, NR, _dp(p), _dp(mu),
+                                       _dp(dmu)), "1d_tangent")
+    return mu, dmu
 
 
 def kpm_2d_blocking(NH, NR, NC, budget, arr_size=0, right_block=0):

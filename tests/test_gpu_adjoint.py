@@ -1,0 +1,118 @@
+"""The reverse-mode gradient of the moments (GPU) against the forward-mode
+tangent of the oracle and central finite differences (h = 1e-6)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import kpmhip as K
+import oracle
+from kpmhip import models
+from kpmhip.rng import Xoshiro256pp
+
+pytestmark = pytest.mark.gpu
+
+
+def hermitian_direction(H, seed):
+    """A random Hermitian perturbation on H's (symmetric) sparsity pattern,
+    as values in H's sorted CSC order; returns (A, dH_data)."""
+    A = sp.csc_matrix(H, dtype=np.complex128)
+    A.sort_indices()
+    rng = Xoshiro256pp(seed)
+    rows = A.indices
+    cols = np.repeat(np.arange(A.shape[1]), np.diff(A.indptr))
+    z = (rng.rand(A.nnz) - 0.5) + 1j * (rng.rand(A.nnz) - 0.5)
+    val = {}
+    for k, (r, c) in enumerate(zip(rows, cols)):
+        if r < c:
+            val[(r, c)] = z[k]
+        elif r == c:
+            val[(r, c)] = z[k].real
+    out = np.zeros(A.nnz, dtype=np.complex128)
+    for k, (r, c) in enumerate(zip(rows, cols)):
+        out[k] = val[(r, c)] if r <= c else np.conj(val[(c, r)])
+    return A, out
+
+
+def model(L=8, seed=3):
+    H = models.square_anderson(L, 2.0, seed=seed)
+    Hc = sp.csc_matrix(H, dtype=np.complex128)
+    # complex Peierls-like phases on the bonds keep it Hermitian but not real
+    Hc = Hc.tolil()
+    rng = Xoshiro256pp(seed + 1)
+    n = Hc.shape[0]
+    for i in range(0, n, 3):
+        j = (i + 1) % n
+        if Hc[i, j] != 0:
+            ph = np.exp(1j * 2 * np.pi * rng.rand())
+            Hc[i, j] = Hc[i, j] * ph
+            Hc[j, i] = np.conj(Hc[i, j])
+    return (sp.csc_matrix(Hc) / 5.5).tocsc()
+
+
+@pytest.mark.parametrize("ckpt", [0, 1, 5, 64])
+def test_adjoint_vs_tangent(ckpt):
+    H = model()
+    NC, NR = 48, 3
+    psi = K.random_phase_vectors(Xoshiro256pp(7), H.shape[0], NR)
+    g = Xoshiro256pp(8).rand(NC) - 0.5
+    grad = K.kpm_1d_adjoint(H, NC, psi, g, ckpt_stride=ckpt)
+    A, dH = hermitian_direction(H, 11)
+    G = sp.csc_matrix(grad)
+    G.sort_indices()
+    pred = np.real(np.sum(G.data * dH))
+    mu, dmu = oracle.kpm_1d_tangent(A, dH, NC, psi)
+    tang = float(np.sum(g * np.real(dmu.mean(axis=0))))
+    print(f"ckpt={ckpt} adjoint {pred:.15e} tangent {tang:.15e}", flush=True)
+    assert abs(pred - tang) <= 1e-10 * max(1.0, abs(tang))
+
+
+def test_adjoint_vs_finite_differences_dos_loss():
+    H = model(L=10, seed=5)
+    NC, NR = 64, 4
+    psi = K.random_phase_vectors(Xoshiro256pp(21), H.shape[0], NR)
+    a, b = 1.0, 0.0
+    E = np.linspace(-0.8, 0.8, 32)
+    target = np.full(len(E), 0.3)
+    mu = oracle.kpm_1d(H, NC, psi).mean(axis=0).real
+    L0, g = K.dos_loss(mu, a, b, E, target)
+    grad = sp.csc_matrix(K.kpm_1d_adjoint(H, NC, psi, g))
+    grad.sort_indices()
+    A, dH = hermitian_direction(H, 12)
+    pred = np.real(np.sum(grad.data * dH))
+    h = 1e-6
+    def loss(Hm):
+        m = oracle.kpm_1d(Hm, NC, psi).mean(axis=0).real
+        return K.dos_loss(m, a, b, E, target)[0]
+    Hp = A.copy(); Hp.data = A.data + h * dH
+    Hm = A.copy(); Hm.data = A.data - h * dH
+    fd = (loss(Hp) - loss(Hm)) / (2 * h)
+    print(f"dos loss {L0:.6e}: adjoint {pred:.10e} fd {fd:.10e}", flush=True)
+    assert abs(pred - fd) <= 1e-5 * max(abs(fd), 1e-12)
+
+
+def test_site_and_bond_gradients_vs_fd():
+    """per-site potential and per-bond hopping gradients of the physical H
+    through the fixed rescaling H_norm = (H - b)/a."""
+    L, a, b = 6, 5.0, 0.1
+    Hphys = models.square_anderson(L, 2.0, seed=9)
+    Hn = ((Hphys - b * sp.identity(L * L)) / a).tocsc()
+    NC, NR = 40, 2
+    psi = K.random_phase_vectors(Xoshiro256pp(4), L * L, NR)
+    g = Xoshiro256pp(5).rand(NC) - 0.5
+    grad = K.kpm_1d_adjoint(Hn, NC, psi, g)
+    gs = K.site_gradient(grad, a)
+    gb = K.bond_gradient(Hn, grad, a).tocoo()
+    def L_of(Hp):
+        Hnp = ((Hp - b * sp.identity(L * L)) / a).tocsc()
+        return float(np.sum(g * oracle.kpm_1d(Hnp, NC, psi).mean(axis=0).real))
+    h = 1e-6
+    for i in (0, 7, 20):
+        E = sp.csc_matrix(([1.0], ([i], [i])), shape=Hphys.shape)
+        fd = (L_of(Hphys + h * E) - L_of(Hphys - h * E)) / (2 * h)
+        assert abs(gs[i] - fd) <= 1e-5 * max(abs(fd), 1e-9), (i, gs[i], fd)
+    for k in (0, 3, 11):
+        i, j = gb.row[k], gb.col[k]
+        E = sp.csc_matrix(([1.0, 1.0], ([i, j], [j, i])), shape=Hphys.shape)
+        u = np.sign(np.real(Hphys[i, j]))  # real hoppings: unit phase = sign
+        fd = (L_of(Hphys + h * u * E) - L_of(Hphys - h * u * E)) / (2 * h)
+        assert abs(gb.data[k] - fd) <= 1e-5 * max(abs(fd), 1e-9), (k, gb.data[k], fd)

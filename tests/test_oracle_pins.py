@@ -201,3 +201,29 @@ def test_chain_identity_probes_closed_form():
     lam = -2 * np.cos(2 * np.pi * np.arange(N) / N) / a
     exact = np.cos(np.arange(NC)[:, None] * np.arccos(lam)[None, :]).mean(axis=1)
     np.testing.assert_allclose(mu, exact, rtol=1e-10, atol=1e-12)
+
+
+def test_tangent_vs_finite_differences():
+    """The oracle's forward-mode tangent (used to pin the GPU adjoint) equals
+    central differences of the oracle's own moments."""
+    rng = Xoshiro256pp(31, lanes=4)
+    H = (models.square_anderson(5, 2.0, seed=2) / 5.5).tocsc()
+    H.sort_indices()
+    psi = phases(rng, 25, 2)
+    d = np.zeros(H.nnz, dtype=complex)
+    rows = H.indices
+    cols = np.repeat(np.arange(25), np.diff(H.indptr))
+    zz = {}
+    for k, (r, c) in enumerate(zip(rows, cols)):
+        key = (min(r, c), max(r, c))
+        if key not in zz:
+            zz[key] = (rng.rand() - 0.5) + (1j * (rng.rand() - 0.5) if r != c else 0)
+        d[k] = zz[key] if r <= c else np.conj(zz[key])
+    NC = 20
+    mu, dmu = oracle.kpm_1d_tangent(H, d, NC, psi)
+    np.testing.assert_allclose(mu, oracle.kpm_1d(H, NC, psi), atol=1e-12)
+    h = 1e-6
+    Hp = H.copy(); Hp.data = H.data + h * d
+    Hm = H.copy(); Hm.data = H.data - h * d
+    fd = (oracle.kpm_1d(Hp, NC, psi) - oracle.kpm_1d(Hm, NC, psi)) / (2 * h)
+    np.testing.assert_allclose(dmu, fd, atol=1e-7)
