@@ -47,7 +47,7 @@ def main() -> int:
     ap.add_argument("--skip-fp64", action="store_true", help="omit the FP64 comparison leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--kernel-launches", type=int, default=100)
-    ap.add_argument("--workload", choices=["graphene", "ensemble"], default="graphene",
+    ap.add_argument("--workload", choices=["graphene", "ensemble", "conductivity"], default="graphene",
                     help="graphene: the headline single-lattice DOS; ensemble: batched disorder "
                          "realisations (cubic Anderson, one fused launch per step)")
     ap.add_argument("--nreal", type=int, default=256)
@@ -60,6 +60,8 @@ def main() -> int:
 
     if args.workload == "ensemble":
         return bench_ensemble(args)
+    if args.workload == "conductivity":
+        return bench_conductivity(args)
 
     import numpy as np
 
@@ -192,6 +194,51 @@ def main() -> int:
         **extra,
     }
     print(json.dumps(out))
+    return 0
+
+
+def bench_conductivity(args) -> int:
+    """Kubo-Bastin sigma_xy of the Haldane model (honeycomb 256 x 256 cells,
+    NH = 131,072, t1 = 1, t2 = 0.1 e^{i pi/2}, M = 0, periodic): kpm_2d with
+    NC = 512, NR = 16 (all 512 left and right slots resident, one MFMA
+    contraction), FP64 vectors with the FP64 and FP32 contraction modes.
+    One step = one kpm_2d call; reports the contraction's share and the
+    quantised sigma_xy in the gap."""
+    import numpy as np
+
+    import kpmhip as K
+    from kpmhip import models
+    from kpmhip.rng import Xoshiro256pp
+
+    if not K.activate(0):
+        raise SystemExit("no HIP device visible")
+    Lc, NC, NR = args.cells if args.cells != 1024 else 256, 512, 16
+    H, Jx, Jy, area = models.haldane_model(Lc, Lc, t=1.0, t2=0.1)
+    NH = H.shape[0]
+    a, b = 3.8, 0.0  # fixed rescaling above the Gershgorin bound 3 t1 + 6 t2 = 3.6
+    Hn = (H / a).tocsc()
+    psi = K.random_phase_vectors(Xoshiro256pp(42), NH, NR)
+    out = {}
+    for gemm in ("f64", "f32"):
+        K.kpm_2d(Hn, Jx, Jy, NC, NR, psi_in=psi, gemm=gemm)  # warm-up
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            mu = K.kpm_2d(Hn, Jx, Jy, NC, NR, psi_in=psi, gemm=gemm)
+        wall = (time.perf_counter() - t0) / args.steps
+        sig = K.kubo_bastin_cond_raw(mu, a, 0.0, b=b, NH=NH, area=area)
+        out[gemm] = {"ms_per_step": round(wall * 1e3, 2), "sigma_xy": sig}
+        print(f"[bench] kpm_2d gemm={gemm}: {wall * 1e3:.1f} ms, sigma_xy={sig:.6f}",
+              file=sys.stderr, flush=True)
+    flops = 8.0 * NC * NC * NH * NR
+    res = {"metric": "kpm_2d moment-table time (Haldane sigma_xy)", "value": out["f64"]["ms_per_step"],
+           "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+           "ms_per_step": out["f64"]["ms_per_step"], "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": "c128", "data": "synthetic",
+           "config": {"workload": f"Haldane {Lc}x{Lc} kpm_2d", "NH": NH, "NC": NC, "NR": NR,
+                      "a": a, "contraction_flops": flops},
+           "fp64_gemm": out["f64"], "fp32_gemm": out["f32"],
+           "sigma_xy_fp64_minus_fp32": out["f64"]["sigma_xy"] - out["f32"]["sigma_xy"]}
+    print(json.dumps(res))
     return 0
 
 
