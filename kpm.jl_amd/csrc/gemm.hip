@@ -13,12 +13,13 @@
 //   Re = sum_k' R'[n,k'] L'[m,k']            L' = (Lr, Li) interleaved
 //   Im = sum_k' R'[n,k'] L''[m,k']           L'' = (-Li, Lr)
 // L'' is formed in registers from the pair partner, so each operand is read
-// once.  Each lane reads 8 consecutive reals of its row per 32-real step and
-// feeds them to 8 MFMAs (the k order inside a step is a fixed permutation,
-// identical for both operands).
+// once.  Each lane takes 8 consecutive reals of its row per 32-real stage from
+// LDS and feeds them to 8 MFMAs (the k order inside a stage is a fixed
+// permutation, identical for both operands).
 //
 // Instructions: v_mfma_f64_16x16x4f64 (FP64 mode) and v_mfma_f32_16x16x4f32
-// (FP32 mode, FP32 accumulation inside one K split).  Split-K partials are
+// (FP32 mode: FP32 accumulation over runs of 256 reals of K, promoted into
+// FP64 accumulators after each run).  Split-K partials are
 // stored in FP64 and reduced in a fixed order, then scaled, masked and added.
 //
 // Operand layout of a 16x16x4 MFMA (wave64): lane l supplies A[l%16][l/16]
@@ -58,9 +59,16 @@ struct Mfma<float> {
   }
 };
 
-constexpr int kStep = 32;  // reals of K per iteration (8 per lane group of 16)
+constexpr int kStep = 32;  // reals of K per stage (16 complex values per slot row)
 
 // Workgroup tile: 64 (n) x 64 (m), four waves of 32 x 32 (2 x 2 MFMA tiles).
+// Each stage stages the 64 R rows and 64 L rows of one 32-real K segment in
+// LDS with coalesced loads (consecutive lanes read consecutive 16-byte words
+// of one row), converted to the MFMA type; waves then read their MFMA
+// operands from LDS.  Rows are padded by 16 bytes so the 16 lanes of an
+// MFMA lane group, which read 16 different rows at the same offset, hit 16
+// different bank quads.  Two LDS stages: the global loads of stage s+1 are in
+// flight while stage s is multiplied.
 // VT: storage real type of the slots; MT: MFMA arithmetic type.
 template <typename VT, typename MT>
 __global__ void __launch_bounds__(256)
@@ -68,37 +76,106 @@ __global__ void __launch_bounds__(256)
                int64_t kchunk, double2* __restrict__ part, int ldn, int ldm) {
   using MF = Mfma<MT>;
   using acc_t = typename MF::acc_t;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  constexpr int PADE = 16 / (int)sizeof(MT);       // pad elements per LDS row
+  constexpr int LDR = kStep + PADE;                 // LDS row stride (MT elements)
+  constexpr int WPR = kStep * (int)sizeof(VT) / 16; // 16-byte global words per row segment
+  constexpr int EPW = 16 / (int)sizeof(VT);         // VT elements per 16-byte word
+  constexpr int NW = 2 * 64 * WPR / 256;            // global words per thread per stage (A+B)
+  __shared__ MT lds[2][2][64 * LDR];                // [stage][A|B][row * LDR + k]
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
   const int g = lane >> 4;
   const int i = lane & 15;
-  const int n0 = blockIdx.y * 64 + (wave >> 1) * 32;
-  const int m0 = blockIdx.x * 64 + (wave & 1) * 32;
+  const int tile_n = blockIdx.y * 64, tile_m = blockIdx.x * 64;
+  const int n0 = (wave >> 1) * 32;
+  const int m0 = (wave & 1) * 32;
   const int64_t kbeg = (int64_t)blockIdx.z * kchunk;
   const int64_t kend = min(kbeg + kchunk, K2);
+  const int nstage = (int)((kend - kbeg) / kStep);
 
-  int na[2], mb[2];
+  // this thread's global words: word w -> (operand, row, word-in-row)
+  const VT* src[NW];
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    na[t] = min(n0 + 16 * t + i, nR - 1);
-    mb[t] = min(m0 + 16 * t + i, nL - 1);
+  for (int q = 0; q < NW; ++q) {
+    const int w = tid + 256 * q;
+    const int op = w / (64 * WPR);  // 0: R (A operand), 1: L (B operand)
+    const int rem = w % (64 * WPR);
+    const int row = rem / WPR;
+    const int wi = rem % WPR;
+    const int grow = op == 0 ? min(tile_n + row, nR - 1) : min(tile_m + row, nL - 1);
+    src[q] = (op == 0 ? R : L) + (size_t)grow * K2 + kbeg + wi * EPW;
   }
+  auto gload = [&](int stage, float4* regs) {
+#pragma unroll
+    for (int q = 0; q < NW; ++q) regs[q] = *reinterpret_cast<const float4*>(src[q] + (int64_t)stage * kStep);
+  };
+  auto lstore = [&](int buf, const float4* regs) {
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+      const int w = tid + 256 * q;
+      const int op = w / (64 * WPR);
+      const int rem = w % (64 * WPR);
+      const int row = rem / WPR;
+      const int wi = rem % WPR;
+      const VT* e = reinterpret_cast<const VT*>(&regs[q]);
+      MT* dst = &lds[buf][op][row * LDR + wi * EPW];
+#pragma unroll
+      for (int t = 0; t < EPW; ++t) dst[t] = (MT)e[t];
+    }
+  };
+
   acc_t are[2][2], aim[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) are[a][b] = aim[a][b] = MF::zero();
 
-  for (int64_t k = kbeg; k < kend; k += kStep) {
+  // FP32 MFMA mode: the FP32 accumulators are added into FP64 ones every
+  // kFlush stages (256 reals of K), so FP32 rounding accumulates over short
+  // runs only; FP64 mode accumulates in the MFMA registers directly.
+  constexpr bool kPromote = sizeof(MT) == 4;
+  constexpr int kFlush = 8;
+  double fre[2][2][4], fim[2][2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) fre[a][b][v] = fim[a][b][v] = 0.0;
+  auto flush = [&]() {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          fre[a][b][v] += (double)are[a][b][v];
+          fim[a][b][v] += (double)aim[a][b][v];
+        }
+        are[a][b] = aim[a][b] = MF::zero();
+      }
+  };
+
+  float4 regs[NW];
+  if (nstage > 0) {
+    gload(0, regs);
+    lstore(0, regs);
+  }
+  __syncthreads();
+  for (int st = 0; st < nstage; ++st) {
+    const int buf = st & 1;
+    if (st + 1 < nstage) gload(st + 1, regs);  // next stage in flight during the MFMAs
     MT av[2][8], bv[2][8];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-      const VT* pa = R + (size_t)na[t] * K2 + k + 8 * g;
-      const VT* pb = L + (size_t)mb[t] * K2 + k + 8 * g;
+      const MT* pa = &lds[buf][0][(n0 + 16 * t + i) * LDR + 8 * g];
+      const MT* pb = &lds[buf][1][(m0 + 16 * t + i) * LDR + 8 * g];
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        av[t][j] = (MT)pa[j];
-        bv[t][j] = (MT)pb[j];
+        av[t][j] = pa[j];
+        bv[t][j] = pb[j];
       }
     }
 #pragma unroll
@@ -114,7 +191,11 @@ __global__ void __launch_bounds__(256)
         }
       }
     }
+    if (kPromote && (st % kFlush) == kFlush - 1) flush();
+    if (st + 1 < nstage) lstore(buf ^ 1, regs);
+    __syncthreads();
   }
+  if (kPromote) flush();
   double2* out = part + (size_t)blockIdx.z * ldn * ldm;
 #pragma unroll
   for (int tn = 0; tn < 2; ++tn)
@@ -122,10 +203,12 @@ __global__ void __launch_bounds__(256)
     for (int tm = 0; tm < 2; ++tm)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int n = n0 + 16 * tn + MF::row(g, v);
-        const int m = m0 + 16 * tm + i;
+        const int n = tile_n + n0 + 16 * tn + MF::row(g, v);
+        const int m = tile_m + m0 + 16 * tm + i;
         if (n < nR && m < nL)
-          out[(size_t)n * ldm + m] = make_double2((double)are[tn][tm][v], (double)aim[tn][tm][v]);
+          out[(size_t)n * ldm + m] =
+              kPromote ? make_double2(fre[tn][tm][v], fim[tn][tm][v])
+                       : make_double2((double)are[tn][tm][v], (double)aim[tn][tm][v]);
       }
 }
 
