@@ -158,6 +158,26 @@ int kpm_3d(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, const kpm_csr* Jb,
            const double* psi_r, int64_t K_left, int64_t K_right, int vec_prec, int gemm_mode,
            double* mu);
 
+/* ---- matrix-function action: src/utils/chebyshev_action.jl ------------------
+ * chebyshev_action! (src/utils/chebyshev_action.jl:47-125) with the fused
+ * accumulation broadcast_assign! (src/utils/chebyshev_action.jl:163-178;
+ * ext/KPMCUDAExt.jl:273-299) and the relative stability guard
+ * _check_chebyshev_columns (src/utils/chebyshev_iteration.jl:119-132):
+ *   out[:, :, k] = sum_{n=1..NC} C[n, k] T_{n-1}(H) V,   k = 1..K.
+ * V: NH x NR column-major complex; C: NC x K column-major complex, used
+ * verbatim; out: NH x NR x K column-major complex, overwritten.  The
+ * recurrence runs in vec_prec, the accumulation in FP64.  check_every = 0
+ * disables the guard; a tripped guard returns KPM_EUNSTABLE with the
+ * reference's message (iteration and growth) and leaves out unspecified. */
+int kpm_cheb_action(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t K, int64_t NR, int64_t NH,
+                    const double* V, const double* C, double* out, int vec_prec,
+                    int64_t check_every);
+/* Device-resident form (the reference's CuArray method with out resident
+ * with Hn): V a recurrence block (its precision is the recurrence's), out an
+ * FP64 block of (K*NH) x NR whose rows [k*NH, (k+1)*NH) receive out[:, :, k]. */
+int kpm_cheb_action_block(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t K, const kpm_block* V,
+                          const double* C, kpm_block* out, int64_t check_every);
+
 /* ---- reverse-mode gradient ---------------------------------------------------
  * No counterpart in the reference.  For the equal-vector moments
  * mu_n = (1/NR) sum_r <psi_r|T_n(H)|psi_r> (n < NC; the moments of

@@ -5,10 +5,14 @@ include/kpm_hip.h); this package is the host-side mirror of the reference's
 interface: the moment engine (src/moment.jl), its primitives
 (src/utils/chebyshev_iteration.jl, src/utils/vectors.jl), the device layer
 (src/device.jl), the rescaling (src/utils/external.jl), damping kernels
-(src/kernels), the DOS / Kubo–Bastin reconstruction heads and the typed front
-end (src/frontend.jl).
+(src/kernels), the DOS / Kubo–Bastin reconstruction heads, the typed front
+end (src/frontend.jl), and the matrix-function action with its consumers
+(src/utils/chebyshev_action.jl: evolve, fermi_projector, chern_marker).
 """
-from ._lib import ArgumentError, KPMHipError, KPMHipMissing, load_library
+from ._lib import ArgumentError, KPMHipError, KPMHipMissing, KPMUnstableError, load_library
+from .action import (chebyshev_action, chebyshev_action_, chern_marker, chern_marker_average,
+                     chern_marker_region, evolution_coefficients, evolution_order,
+                     evolution_tail, evolve, evolve_, fermi_coefficients, fermi_projector)
 from .adjoint import (bond_gradient, dos_loss, kpm_1d_adjoint, moments_rrule,
                       site_gradient)
 from .applications import (chebyshev_lin_trans, chebyshevT, chebyshevT_0, chebyshevT_xn,

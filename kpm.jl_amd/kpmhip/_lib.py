@@ -37,6 +37,10 @@ class KPMHipError(RuntimeError):
     """A HIP runtime / allocation failure inside libkpm_hip."""
 
 
+class KPMUnstableError(RuntimeError):
+    """Julia's ErrorException from the recurrence stability guard."""
+
+
 class KPMHipMissing(RuntimeError):
     """libkpm_hip.so is not built or no GPU is visible."""
 
@@ -79,6 +83,8 @@ _SIGS = {
                          C.c_int, C.c_int, C.c_int, _pd]),
     "kpm_3d": (C.c_int, [_p, _p, _p, _p, _p, _i64, _i64, _i64, _pd, _pd, _i64, _i64, C.c_int,
                          C.c_int, _pd]),
+    "kpm_cheb_action": (C.c_int, [_p, _p, _i64, _i64, _i64, _i64, _pd, _pd, _pd, C.c_int, _i64]),
+    "kpm_cheb_action_block": (C.c_int, [_p, _p, _i64, _i64, _p, _pd, _p, _i64]),
     "kpm_1d_adjoint": (C.c_int, [_p, _p, _i64, _i64, _i64, _pd, _pd, _pd, C.c_int, _i64]),
     "kpm_batch_create": (C.c_int, [_p, C.c_int, _pi64, _pi64, _pi64, _pd, C.c_int, C.POINTER(_p)]),
     "kpm_batch_destroy": (C.c_int, [_p]),
@@ -140,6 +146,8 @@ def check(rc: int, what: str = "") -> None:
         raise NotImplementedError(msg)
     if rc == KPM_ENOMEM:
         raise MemoryError(msg)
+    if rc == KPM_EUNSTABLE:
+        raise KPMUnstableError(msg)
     raise KPMHipError(f"{what}: {msg} (status {rc})")
 
 

@@ -184,7 +184,8 @@ def dos0(mu, H_rescale_factor, *, NC: int = 0, kernel=JacksonKernel, dE_order: i
 
 
 def fermiFunction(E, E_f, beta):
-    return 1.0 / (np.exp((E - E_f) * beta) + 1.0)
+    with np.errstate(over="ignore"):  # exp overflow -> occupation 0, as intended
+        return 1.0 / (np.exp((E - E_f) * beta) + 1.0)
 
 
 def fermiFunctions(E_f: float, beta: float):

@@ -9,6 +9,9 @@ CSC ComplexF64, 0-based).  Each restates a model builder of the reference:
 - ``haldane_model``    test/ed_reference.jl:69-147 (with Jx, Jy, area)
 - ``cubic_model``      test/ed_reference.jl:396-442 (with Jx, Jy, Jz, volume)
 - ``chain``            the periodic 1-D chain of test/dos_test.jl:214-228
+- ``haldane_open_model`` test/ed_reference.jl:149-220 (open flake, site
+                       coordinates and cell area, for local Chern markers)
+- ``ring_model`` / ``flux_ring_model``  test/ed_reference.jl:530-570
 
 Disorder comes from the build's xoshiro256++ generator (kpmhip.rng), not from
 Julia's Xoshiro stream.
@@ -177,3 +180,67 @@ def chain(N: int, *, t: float = 1.0, shift: float = 0.0):
     cols = np.concatenate([i, (i + 1) % N, i])
     vals = np.concatenate([np.full(N, shift), np.full(N, -t), np.full(N, -t)]).astype(np.complex128)
     return _csc(rows, cols, vals, N)
+
+
+def haldane_open_model(Lx: int, Ly: int, *, t: float = 1.0, t2: float = 0.2,
+                       phi: float = math.pi / 2, m: float = 0.0):
+    """Haldane model on an open Lx x Ly honeycomb flake (hops leaving the grid
+    are dropped): returns (H, pos, cell_area) with pos the N x 2 site
+    coordinates r_A = (x-1) a1 + (y-1) a2, r_B = r_A + dAB.  Sites: cell
+    c = (y-1) + Ly (x-1), A = 2c, B = 2c + 1 (0-based)."""
+    a1 = np.array([math.sqrt(3), 0.0])
+    a2 = np.array([math.sqrt(3) / 2, 1.5])
+    dAB = np.array([math.sqrt(3) / 2, 0.5])
+    N = 2 * Lx * Ly
+    X, Y = np.meshgrid(np.arange(1, Lx + 1), np.arange(1, Ly + 1), indexing="ij")
+    X = X.ravel()
+    Y = Y.ravel()
+    cell = (Y - 1) + Ly * (X - 1)
+    pos = np.zeros((N, 2))
+    rA = (X - 1)[:, None] * a1[None, :] + (Y - 1)[:, None] * a2[None, :]
+    pos[2 * cell] = rA
+    pos[2 * cell + 1] = rA + dAB
+    I, J, V = [], [], []
+
+    def addhop(i, j, amp):
+        amp = np.broadcast_to(np.asarray(amp, dtype=np.complex128), i.shape)
+        I.extend([i, j])
+        J.extend([j, i])
+        V.extend([amp, np.conj(amp)])
+
+    def inb(x, y):
+        return (x >= 1) & (x <= Lx) & (y >= 1) & (y <= Ly)
+
+    def sA(x, y):
+        return 2 * ((y - 1) + Ly * (x - 1))
+
+    addhop(sA(X, Y) + 1, sA(X, Y), -t + 0j)
+    for xo, yo in ((1, 0), (0, 1)):
+        k = inb(X + xo, Y + yo)
+        addhop(sA(X[k], Y[k]) + 1, sA(X[k] + xo, Y[k] + yo), -t + 0j)
+    for xo, yo in ((1, 0), (-1, 1), (0, -1)):  # a1, a2 - a1, -a2
+        k = inb(X + xo, Y + yo)
+        addhop(sA(X[k] + xo, Y[k] + yo), sA(X[k], Y[k]), t2 * np.exp(1j * phi))
+        addhop(sA(X[k] + xo, Y[k] + yo) + 1, sA(X[k], Y[k]) + 1, t2 * np.exp(-1j * phi))
+    I.extend([2 * cell, 2 * cell + 1])
+    J.extend([2 * cell, 2 * cell + 1])
+    V.extend([np.full(cell.shape, m + 0j), np.full(cell.shape, -m + 0j)])
+    H = _csc(np.concatenate(I), np.concatenate(J), np.concatenate(V), N)
+    return H, pos, abs(a1[0] * a2[1] - a1[1] * a2[0])
+
+
+def ring_model(N: int, *, t: float = 1.0):
+    """N-site periodic ring with hopping -t (real symmetric)."""
+    i = np.arange(N)
+    rows = np.concatenate([i, i])
+    cols = np.concatenate([(i + 1) % N, (i - 1) % N])
+    return _csc(rows, cols, np.full(2 * N, -float(t)), N)
+
+
+def flux_ring_model(N: int, *, t: float = 1.0, phi: float = 0.35):
+    """N-site periodic ring with h[i, i+1] = -t e^{i phi} and its conjugate."""
+    i = np.arange(N)
+    j = (i + 1) % N
+    hop = -float(t) * np.exp(1j * phi)
+    return _csc(np.concatenate([i, j]), np.concatenate([j, i]),
+                np.concatenate([np.full(N, hop), np.full(N, np.conj(hop))]), N)

@@ -17,6 +17,9 @@
  *   - kpm_2d_blocking (integer planner)   src/moment.jl:759-810
  *   - _kpm_2d_blocked!                    src/moment.jl:987-1088
  *   - kpm_3d!                             src/moment.jl:1106-1227
+ *   - chebyshev_action! + broadcast_assign! + the relative stability guard
+ *       src/utils/chebyshev_action.jl:47-125,163-178
+ *       src/utils/chebyshev_iteration.jl:119-132
  *
  * Layout conventions follow the Julia caller: probe blocks are NH x NR
  * column-major interleaved complex doubles; mu_all is NR x NC column-major;
@@ -28,6 +31,7 @@
  * Parity pins (closed forms, dense references, ED) live in tests/.
  */
 #include <complex.h>
+#include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -457,4 +461,84 @@ int kpm_oracle_1d_tangent(int64_t NH, const int64_t* colptr, const int64_t* rowv
     }
     free(w);
     return 0;
+}
+
+/* chebyshev_action! (chebyshev_action.jl:47-125): two-slot recurrence,
+ * out[:, :, k] += C[n, k] * T_{n-1} V after every step (broadcast_assign!,
+ * chebyshev_action.jl:163-178), out zeroed first.  V NH x NR, C NC x K and
+ * out NH x NR x K, all column-major complex.  The relative guard
+ * (chebyshev_iteration.jl:119-132) runs at iteration 1 when NC == 2 and at
+ * every check_every-th iteration and the last one otherwise; on a trip the
+ * function returns -4 and stores the iteration and sqrt(max growth) in
+ * guard[0], guard[1].  check_every == 0 disables it. */
+int kpm_oracle_cheb_action(int64_t NH, const int64_t* colptr, const int64_t* rowval,
+                           const double* nzval, int64_t NC, int64_t K, int64_t NR,
+                           const double* V, const double* Cc, double* out_, int64_t check_every,
+                           double* guard) {
+    if (NH <= 0 || NR <= 0 || NC < 1 || K < 1 || check_every < 0) return OR_EINVAL;
+    csc_t H = {NH, colptr, rowval, (const cplx*)nzval};
+    const cplx* C = (const cplx*)Cc;
+    cplx* out = (cplx*)out_;
+    const size_t blk = (size_t)(NH * NR);
+    cplx* s1 = (cplx*)malloc(sizeof(cplx) * blk);
+    cplx* s2 = (cplx*)malloc(sizeof(cplx) * blk);
+    double* ref = (double*)malloc(sizeof(double) * (size_t)NR);
+    if (!s1 || !s2 || !ref) {
+        free(s1);
+        free(s2);
+        free(ref);
+        return OR_ENOMEM;
+    }
+    memcpy(s1, V, sizeof(cplx) * blk);
+    for (int64_t c = 0; c < NR; ++c) ref[c] = creal(vdot(NH, s1 + c * NH, s1 + c * NH));
+    memset(out, 0, sizeof(cplx) * blk * (size_t)K);
+    cplx* slots[2] = {s1, s2};
+    int rc = 0;
+    /* broadcast_assign!: out_k .+= x .* c_k */
+#define ACCUM(x, n)                                                        \
+    for (int64_t k = 0; k < K; ++k) {                                     \
+        const cplx ck = C[(n) + NC * k];                                   \
+        cplx* o = out + (size_t)k * blk;                                   \
+        for (size_t e = 0; e < blk; ++e) o[e] += (x)[e] * ck;              \
+    }
+    /* _check_chebyshev_columns(slot, iteration, ref_sq) */
+#define GUARD(x, it)                                                       \
+    do {                                                                   \
+        double g2 = -INFINITY;                                             \
+        int nan = 0;                                                       \
+        for (int64_t c = 0; c < NR; ++c) {                                 \
+            const double d = ref[c] == 0.0 ? 1.0 : ref[c];                 \
+            const double r = creal(vdot(NH, (x) + c * NH, (x) + c * NH)) / d; \
+            if (r != r) nan = 1;                                           \
+            if (r > g2) g2 = r;                                            \
+        }                                                                  \
+        if (nan || !(g2 <= 2.25)) {                                        \
+            if (guard) {                                                   \
+                guard[0] = (double)(it);                                   \
+                guard[1] = nan ? NAN : sqrt(g2);                           \
+            }                                                              \
+            rc = -4;                                                       \
+        }                                                                  \
+    } while (0)
+    ACCUM(slots[0], 0);
+    if (NC >= 2) {
+        block_mul5(&H, NH, NR, slots[0], slots[1], 1.0, 0.0);
+        ACCUM(slots[1], 1);
+        if (NC == 2 && check_every > 0) GUARD(slots[1], 1);
+        int ip = 1, ipp = 0;
+        for (int64_t n = 3; n <= NC && rc == 0; ++n) {
+            cheb_single(&H, NH, NR, slots[ipp], slots[ip]);
+            ACCUM(slots[ipp], n - 1);
+            const int64_t it = n - 1;
+            if (check_every > 0 && (it % check_every == 0 || n == NC)) GUARD(slots[ipp], it);
+            ip = 1 - ip;
+            ipp = 1 - ipp;
+        }
+    }
+#undef ACCUM
+#undef GUARD
+    free(s1);
+    free(s2);
+    free(ref);
+    return rc;
 }

@@ -55,6 +55,8 @@ def lib():
             "kpm_oracle_cheb_iter": (C.c_int, [_i, _pi, _pi, _pd, _i, _i, _pd]),
             "kpm_oracle_1d_steps": (C.c_int, [_i, _pi, _pi, _pd, _i, _i, _pd, _pd, _pd]),
             "kpm_oracle_1d_tangent": (C.c_int, [_i, _pi, _pi, _pd, _pd, _i, _i, _pd, _pd, _pd]),
+            "kpm_oracle_cheb_action": (C.c_int, [_i, _pi, _pi, _pd, _i, _i, _i, _pd, _pd, _pd, _i,
+                                                 _pd]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -185,6 +187,38 @@ def kpm_3d(H, Ja, Jb, Jg, NC: int, psi_l, psi_r, K_left: int = 3, K_right: int |
     return mu
 
 
+class OracleUnstable(RuntimeError):
+    """The oracle's restatement of the recurrence stability guard tripped;
+    .iteration and .growth are the reference's error() payload."""
+
+    def __init__(self, iteration: int, growth: float):
+        super().__init__(f"Chebyshev recurrence is unstable at iteration {iteration} "
+                         f"(maximum column growth {growth} > 1.5)")
+        self.iteration = iteration
+        self.growth = growth
+
+
+def cheb_action(H, V, C, check_every: int = 16) -> np.ndarray:
+    """chebyshev_action! (src/utils/chebyshev_action.jl:47-125):
+    out[:, :, k] = sum_n C[n, k] T_{n-1}(H) V, NH x NR x K complex."""
+    A = CSC(H)
+    v = _blk(V)
+    c = np.asarray(C, dtype=np.complex128)
+    if c.ndim == 1:
+        c = c.reshape(-1, 1)
+    c = np.asfortranarray(c)
+    NC, K = c.shape
+    NH, NR = v.shape
+    out = np.zeros((NH, NR, K), dtype=np.complex128, order="F")
+    g = np.zeros(2)
+    rc = lib().kpm_oracle_cheb_action(*A.args(), NC, K, NR, _dp(v), _dp(c), _dp(out),
+                                      int(check_every), _dp(g))
+    if rc == -4:
+        raise OracleUnstable(int(g[0]), float(g[1]))
+    _check(rc, "cheb_action")
+    return out
+
+
 def cheb_iter(H, slots: np.ndarray) -> np.ndarray:
     """chebyshev_iter over an NH x NR x n workspace (slots 1, 2 given)."""
     A = CSC(H)
@@ -224,6 +258,41 @@ def chebyshev_matrices(H, NC: int):
     for _ in range(2, NC):
         T.append(2 * Hd @ T[-1] - T[-2])
     return T
+
+
+def action_dense(H, V, C):
+    """Column-by-column three-term recurrence reference of the action
+    (test/chebyshev_action_test.jl:20-41): out[:, j, k] = sum_n C[n,k] T_n V[:, j]."""
+    Hd = H.toarray() if sp.issparse(H) else np.asarray(H, dtype=np.complex128)
+    v = _blk(V)
+    c = np.asarray(C, dtype=np.complex128).reshape(np.shape(C)[0], -1)
+    NC, K = c.shape
+    out = np.zeros(v.shape + (K,), dtype=np.complex128)
+    for j in range(v.shape[1]):
+        prev = v[:, j].copy()
+        cur = Hd @ prev
+        for k in range(K):
+            out[:, j, k] += c[0, k] * prev
+            if NC >= 2:
+                out[:, j, k] += c[1, k] * cur
+        for n in range(2, NC):
+            prev, cur = cur, 2 * (Hd @ cur) - prev
+            for k in range(K):
+                out[:, j, k] += c[n, k] * cur
+    return out
+
+
+def action_ed(H, V, C):
+    """ED form of the action: U diag(sum_n C[n,k] cos(n theta)) U^H V
+    (test/chebyshev_action_test.jl:53-59)."""
+    Hd = H.toarray() if sp.issparse(H) else np.asarray(H, dtype=np.complex128)
+    lam, U = np.linalg.eigh(Hd)
+    th = np.arccos(np.clip(lam, -1, 1))
+    v = _blk(V)
+    c = np.asarray(C, dtype=np.complex128).reshape(np.shape(C)[0], -1)
+    f = np.cos(np.arange(c.shape[0])[:, None] * th[None, :]).T @ c      # (N, K)
+    proj = U.conj().T @ v
+    return np.stack([U @ (f[:, k:k + 1] * proj) for k in range(c.shape[1])], axis=2)
 
 
 def kpm1d_dense(H, psi, NC):
@@ -288,3 +357,30 @@ def ed_moments(H, psi, NC):
     th = np.arccos(np.clip(lam, -1, 1))
     Tn = np.cos(np.arange(NC)[:, None] * th[None, :])
     return (Tn @ w).T
+
+
+def ed_evolve(H, psi, t):
+    """exp(-iHt) psi = U e^{-i lam t} U^H psi by dense eigendecomposition
+    (test/ed_reference.jl:50-59), broadcast over the columns of psi."""
+    Hd = H.toarray() if sp.issparse(H) else np.asarray(H)
+    lam, U = np.linalg.eigh(Hd)
+    p = np.asarray(psi, dtype=np.complex128)
+    ph = np.exp(-1j * lam * t)
+    proj = U.conj().T @ p
+    return U @ (ph[:, None] * proj if p.ndim == 2 else ph * proj)
+
+
+def ed_chern_marker(H, x, y, Ef, beta=np.inf):
+    """m_i = -4 pi Im <i|P X Q Y P|i> with the exact projector P = U f(e - Ef) U^H,
+    midpoint occupation at Ef for beta = inf (test/ed_reference.jl:222-247)."""
+    Hd = H.toarray() if sp.issparse(H) else np.asarray(H)
+    ev, U = np.linalg.eigh(Hd)
+    if np.isinf(beta):
+        occ = ((ev < Ef).astype(float) + (ev <= Ef).astype(float)) / 2
+    else:
+        occ = 1.0 / (np.exp(beta * (ev - Ef)) + 1.0)
+    P = (U * occ[None, :]) @ U.conj().T
+    X = np.diag(np.asarray(x, dtype=float))
+    Y = np.diag(np.asarray(y, dtype=float))
+    M = P @ X @ (np.eye(len(ev)) - P) @ Y @ P
+    return -4 * np.pi * np.imag(np.diag(M))

@@ -94,6 +94,17 @@ def main():
                                                Jb @ T[n2] @ Jg @ T[n1] @ Ja @ pr[:, i]) / NR
     np.savez(os.path.join(HERE, "kpm3d_dense.npz"), H=H, Ja=Ja, Jb=Jb, Jg=Jg, psi_l=pl, psi_r=pr,
              NC=NC, mu=mu3)
+    # g4: chebyshev_action (test/chebyshev_action_test.jl:20-41 reference form)
+    NH, NR, NC, K = 20, 2, 12, 3
+    H = sparsify(rng, herm(rng, NH, 1.05))
+    V = (rng.rand(NH, NR) - 0.5) + 1j * (rng.rand(NH, NR) - 0.5)
+    Cc = (rng.rand(NC, K) - 0.5) + 1j * (rng.rand(NC, K) - 0.5)
+    T = cheb(H, NC)
+    out = np.zeros((NH, NR, K), dtype=complex)
+    for k in range(K):
+        for n in range(NC):
+            out[:, :, k] += Cc[n, k] * (T[n] @ V)
+    np.savez(os.path.join(HERE, "action_dense.npz"), H=H, V=V, C=Cc, out=out)
     print("wrote fixtures to", HERE)
 
 

@@ -33,6 +33,9 @@ MUTANTS = {
     "ring_slot": ("const int ri = (int)((n - 1) % 3), rip = (int)((n + 1) % 3), ripp = (int)(n % 3);",
                   "const int ri = (int)((n - 1) % 3), rip = (int)(n % 3), ripp = (int)((n + 1) % 3);"),
     "current_no_adjoint": ("acc += conj(nz[k]) * x[rowval[k]];", "acc += nz[k] * x[rowval[k]];"),
+    "action_skip_seed": ("    ACCUM(slots[0], 0);\n", "\n"),
+    "action_conj_coef": ("for (size_t e = 0; e < blk; ++e) o[e] += (x)[e] * ck;",
+                         "for (size_t e = 0; e < blk; ++e) o[e] += (x)[e] * conj(ck);"),
 }
 
 
@@ -71,6 +74,12 @@ def pins_pass(lib_path) -> bool:
             NC2 = int(g2["NC"])
             ok &= np.allclose(oracle.kpm_2d(*args, NC2, g2["psi_l"], g2["psi_r"], 3, 2), g2["mu"],
                               rtol=1e-12, atol=1e-12)
+            g4 = dict(np.load(os.path.join(GOLD, "action_dense.npz"), allow_pickle=False))
+            try:
+                ok &= np.allclose(oracle.cheb_action(sp.csc_matrix(g4["H"]), g4["V"], g4["C"]),
+                                  g4["out"], rtol=1e-12, atol=1e-12)
+            except oracle.OracleUnstable:  # a mutant can make the recurrence grow
+                ok = False
             return bool(ok)
         finally:
             oracle.LIB = old_lib_path
