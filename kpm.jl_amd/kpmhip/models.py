@@ -12,6 +12,9 @@ CSC ComplexF64, 0-based).  Each restates a model builder of the reference:
 - ``haldane_open_model`` test/ed_reference.jl:149-220 (open flake, site
                        coordinates and cell area, for local Chern markers)
 - ``ring_model`` / ``flux_ring_model``  test/ed_reference.jl:530-570
+- ``square_disordered`` the periodic square lattice with on-site eps_i ~ U[-1, 1]
+                       and Hermitian-tied bond hoppings t_ij ~ 1 + U[-0.1, 0.1]
+                       (the trainable-parameter lattice of the gradient workload)
 
 Disorder comes from the build's xoshiro256++ generator (kpmhip.rng), not from
 Julia's Xoshiro stream.
@@ -244,3 +247,23 @@ def flux_ring_model(N: int, *, t: float = 1.0, phi: float = 0.35):
     hop = -float(t) * np.exp(1j * phi)
     return _csc(np.concatenate([i, j]), np.concatenate([j, i]),
                 np.concatenate([np.full(N, hop), np.full(N, np.conj(hop))]), N)
+
+
+def square_disordered(L: int, *, seed: int = 42, eps: float = 1.0, dt: float = 0.1):
+    """Periodic L x L square lattice, H_ii = eps_i ~ U[-eps, eps] and
+    H_ij = H_ji = -t_ij with t_ij ~ 1 + U[-dt, dt] per bond (real)."""
+    rng = Xoshiro256pp(seed)
+    N = L * L
+    x, y = np.meshgrid(np.arange(L), np.arange(L), indexing="ij")
+    x = x.T.ravel()
+    y = y.T.ravel()
+    site = x + L * y
+    right = (x + 1) % L + L * y
+    up = x + L * ((y + 1) % L)
+    tr = 1.0 + dt * (2 * rng.rand(N) - 1)
+    tu = 1.0 + dt * (2 * rng.rand(N) - 1)
+    e = eps * (2 * rng.rand(N) - 1)
+    rows = np.concatenate([site, right, site, up, site])
+    cols = np.concatenate([right, site, up, site, site])
+    vals = np.concatenate([-tr, -tr, -tu, -tu, e])
+    return _csc(rows, cols, vals, N)

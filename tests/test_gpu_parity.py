@@ -86,6 +86,30 @@ def test_determinism_bitwise():
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_autotuner_choice_invariance(prec):
+    """Every launch shape the tuner may record (rows per lane x workgroups)
+    gives the same moments; only the partial-sum grouping moves, so they agree
+    to roundoff, and the tuner's own pick is one of them."""
+    H = (models.graphene(40, 40, disorder=0.5, seed=9) / 3.6).tocsc()
+    NR, NC = 16, 64
+    psi = phases(19, H.shape[0], NR)
+    K.tune_set(H, NR, prec, unroll=2, grid=0)
+    base = K.kpm_1d(H, NC, NR, psi_in=psi, avg_output=False, precision=prec)
+    tol = 1e-13  # the stored vectors are identical; only FP64 dot grouping moves
+    for unroll in (1, 2):
+        for grid in (8, 64, 512, 2048):
+            print(f"tune prec={prec} unroll={unroll} grid={grid}", flush=True)
+            K.tune_set(H, NR, prec, unroll=unroll, grid=grid)
+            got = K.kpm_1d(H, NC, NR, psi_in=psi, avg_output=False, precision=prec)
+            np.testing.assert_allclose(got, base, rtol=0, atol=tol)
+    picked = K.autotune(H, NR, prec)
+    assert picked["unroll"] in (1, 2) and picked["grid"] % 8 == 0
+    got = K.kpm_1d(H, NC, NR, psi_in=psi, avg_output=False, precision=prec)
+    np.testing.assert_allclose(got, base, rtol=0, atol=tol)
+    K.tune_set(H, NR, prec, unroll=2, grid=0)
+
+
 @pytest.mark.parametrize("NR", [1, 2, 3, 5, 16, 33, 64, 65, 130])
 def test_kpm1d_probe_widths(NR):
     """padded probe widths (NRp > NR) and multi-chunk rows (NR > 64)."""
