@@ -178,6 +178,22 @@ int kpm_cheb_action(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t K, int64
 int kpm_cheb_action_block(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t K, const kpm_block* V,
                           const double* C, kpm_block* out, int64_t check_every);
 
+/* ---- reconstruction heads (FP64, device) ------------------------------------
+ * chebyshev_lin_trans (src/utils/chebyshev_lintrans.jl:12-27; the device
+ * method ext/KPMCUDAExt.jl:207-236) as used by dos (src/applications/dos.jl:
+ * 128-133): y[i] = sum_j mu_tilde[j] cos(n_grid[j] acos(x[i])), x in [-1, 1];
+ * mu_tilde and y complex (interleaved), host arrays. */
+int kpm_cheb_lin_trans(kpm_ctx* ctx, int64_t Nx, const double* x, int64_t Nn, const int64_t* n_grid,
+                       const double* mu_tilde, double* y);
+/* The quadrature-node contraction of kubo_bastin_cond
+ * (src/applications/conductivity.jl:133-155): for the kernel-weighted table
+ * mu_tilde (NC x NC complex column-major) and K nodes x_k in (-1, 1) with
+ * weights phi_k, acc = sum_k phi_k sum_nm mu_tilde[n,m] Gamma_nm(x_k), with
+ * Gamma_nm(x) = (x - i m s) e^{i m th} T_n(x) + (x + i n s) e^{-i n th} T_m(x).
+ * acc: one complex value.  The caller applies -2 NH / (area a^2) Re(acc). */
+int kpm_kubo_bastin_nodes(kpm_ctx* ctx, int64_t NC, const double* mu_tilde, int64_t K,
+                          const double* x, const double* phi, double* acc);
+
 /* ---- reverse-mode gradient ---------------------------------------------------
  * No counterpart in the reference.  For the equal-vector moments
  * mu_n = (1/NR) sum_r <psi_r|T_n(H)|psi_r> (n < NC; the moments of

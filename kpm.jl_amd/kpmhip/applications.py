@@ -1,8 +1,15 @@
-"""Host FP64 reconstruction heads that consume the moments: the DOS
+"""FP64 reconstruction heads that consume the moments: the DOS
 (src/applications/dos.jl:76-199 with src/utils/chebyshev.jl:7-39 and
 src/utils/chebyshev_lintrans.jl:6-27) and the Kubo–Bastin conductivity
 (src/applications/conductivity.jl:101-157), plus the Fermi functions
 (src/utils/physics.jl:15-35).
+
+The heavy parts run on the device in FP64, as the reference's device methods
+do: ``chebyshev_lin_trans`` (ext/KPMCUDAExt.jl:207-236) through
+``kpm_cheb_lin_trans`` and the Kubo–Bastin node contraction through
+``kpm_kubo_bastin_nodes``.  Node selection, weights and the energy-derivative
+path (which the reference also keeps on the host, dos.jl:134-145) stay on the
+host.
 """
 from __future__ import annotations
 
@@ -11,7 +18,8 @@ import warnings
 
 import numpy as np
 
-from ._lib import ArgumentError
+from ._lib import ArgumentError, check, dptr, i64ptr
+from .device import get_context
 from .kernels import JacksonKernel, hn, mu2D_apply_kernel_and_h
 
 
@@ -34,18 +42,21 @@ def chebyshevT_xn(x_grid, n_grid):
 
 
 def chebyshev_lin_trans(x_grid, n_grid, mu_tilde):
-    """Σ_n μ̃_n T_n(x) on a point or grid (src/utils/chebyshev_lintrans.jl:6-27)."""
-    mu_tilde = np.asarray(mu_tilde)
-    if np.ndim(x_grid) == 0:
-        return np.sum(chebyshevT_xn(x_grid, n_grid) @ mu_tilde)
-    x = np.asarray(x_grid, dtype=np.float64)
-    Nx, Nn = len(x), len(n_grid)
-    if Nx * Nn / 65536 < 1000:
-        return chebyshevT_xn(x, n_grid) @ mu_tilde
-    y = np.zeros(Nx, dtype=np.result_type(mu_tilde, np.complex128))
-    for i in range(Nx):
-        y[i] = np.dot(chebyshevT(np.asarray(n_grid), x[i]), mu_tilde)
-    return y
+    """Σ_n μ̃_n T_n(x) on a point or grid (src/utils/chebyshev_lintrans.jl:6-27),
+    evaluated on the device (one wave per point, fixed-order sums)."""
+    scalar = np.ndim(x_grid) == 0
+    x = np.ascontiguousarray(np.atleast_1d(np.asarray(x_grid, dtype=np.float64)))
+    n = np.ascontiguousarray(np.asarray(n_grid, dtype=np.int64).reshape(-1))
+    m = np.ascontiguousarray(np.asarray(mu_tilde, dtype=np.complex128).reshape(-1))
+    if m.shape[0] != n.shape[0]:
+        raise ArgumentError("n_grid and mu_tilde must have the same length")
+    y = np.zeros(x.shape[0], dtype=np.complex128)
+    ctx = get_context()
+    check(ctx.lib.kpm_cheb_lin_trans(ctx.handle, x.shape[0], dptr(x), n.shape[0], i64ptr(n),
+                                     dptr(m), dptr(y)), "chebyshev_lin_trans")
+    if not np.iscomplexobj(mu_tilde):
+        y = y.real.copy()
+    return y[0] if scalar else y
 
 
 # ---- truncated Taylor arithmetic for dE_order derivatives ---------------------
@@ -234,19 +245,12 @@ def kubo_bastin_cond(mu2D, a: float, Ef: float, *, b: float = 0.0, NH: int, area
         return 0.0
     x = nodes[keep]
     phi = weights[keep] * fv[keep] / (1 - x ** 2) ** 1.5
-    ns = np.arange(NC)[:, None]
-    acc = 0.0 + 0.0j
-    chunk = max(1, min(len(x), -(-1_000_000 // NC)))
-    for lo in range(0, len(x), chunk):
-        xs = x[lo:lo + chunk][None, :]
-        ps = phi[lo:lo + chunk][None, :]
-        th = np.arccos(xs)
-        s = np.sqrt(1 - xs ** 2)
-        Cm = np.cos(ns * th)
-        Em = np.exp(1j * ns * th)
-        WM = (xs - 1j * ns * s) * Em
-        WN = (xs + 1j * ns * s) * np.conj(Em)
-        U = mt.T @ Cm
-        V = mt @ Cm
-        acc += np.sum(ps * (U * WM)) + np.sum(ps * (V * WN))
+    mtf = np.asfortranarray(mt, dtype=np.complex128)
+    xs = np.ascontiguousarray(x, dtype=np.float64)
+    ps = np.ascontiguousarray(phi, dtype=np.float64)
+    out = np.zeros(2)
+    ctx = get_context()
+    check(ctx.lib.kpm_kubo_bastin_nodes(ctx.handle, NC, dptr(mtf), xs.shape[0], dptr(xs), dptr(ps),
+                                        dptr(out)), "kubo_bastin_cond")
+    acc = complex(out[0], out[1])
     return float(-2 * NH / (area * a ** 2) * acc.real)

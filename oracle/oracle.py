@@ -6,6 +6,11 @@ references used to pin it.  Only tests/, __graft_entry__.smoke() and
 bench.py's cpu_baseline leg may import this module; the product path
 (kpmhip + libkpm_hip.so) never does.
 
+It also restates the reference's host reconstruction heads in numpy FP64
+(``dos`` after src/applications/dos.jl:76-149 and src/utils/chebyshev_lintrans.jl,
+``kubo_bastin_cond`` after src/applications/conductivity.jl:101-157), against
+which the device reconstruction kernels are checked.
+
 Dense references (numpy, tiny sizes) restate the reference's own test
 references: _chebyshev_matrices / _kpm2d_dense_reference /
 _kpm3d_dense_reference (test/kpm2d_blocking_test.jl:25-58) and the ED moment
@@ -384,3 +389,97 @@ def ed_chern_marker(H, x, y, Ef, beta=np.inf):
     Y = np.diag(np.asarray(y, dtype=float))
     M = P @ X @ (np.eye(len(ev)) - P) @ Y @ P
     return -4 * np.pi * np.imag(np.diag(M))
+
+
+# ---- host reconstruction heads (restated from the reference) -------------------------
+
+
+def jackson(n, N):
+    """Jackson kernel g_n (src/kernels/jackson_kernel.jl:7-8), zero for n >= N."""
+    n = np.asarray(n, dtype=np.float64)
+    q = np.pi / (N + 1.0)
+    return ((N + 1.0 - n) * np.cos(q * n) + np.sin(q * n) / np.tan(q)) / (N + 1.0) * (n < N)
+
+
+def hn(n):
+    """h_0 = 1, h_n = 2 (src/kernels/kernels.jl:2)."""
+    return np.where(np.asarray(n) == 0, 1.0, 2.0)
+
+
+def lin_trans(x, n_grid, mu_tilde):
+    """sum_j mu~_j cos(n_j acos x) (src/utils/chebyshev_lintrans.jl:12-27)."""
+    x = np.atleast_1d(np.asarray(x, dtype=np.float64))
+    return np.cos(np.asarray(n_grid)[None, :] * np.arccos(x)[:, None]) @ np.asarray(mu_tilde)
+
+
+def dos(mu, a, *, b=0.0, E_grid=None, N_tilde=0, E_range=None, NC=0, kernel=jackson):
+    """(E_grid, rho), rho = sum_n h_n g_n mu_n T_n(x) / (a pi sqrt(1-x^2)) for
+    |E - b| < a, zero elsewhere (src/applications/dos.jl:76-149, dE_order = 0)."""
+    mu = np.asarray(mu)
+    NC = len(mu) if NC == 0 else min(NC, len(mu))
+    if E_grid is None:
+        N_tilde = 2 * NC if N_tilde == 0 else N_tilde
+        lo, hi = (b - 0.99 * a, b + 0.99 * a) if E_range is None else E_range
+        E_grid = np.arange(N_tilde + 1) * (hi - lo) / N_tilde + lo
+    E_grid = np.asarray(E_grid, dtype=np.float64)
+    rho = np.zeros(len(E_grid))
+    idx = np.abs(E_grid - b) < a
+    x = (E_grid[idx] - b) / a
+    ns = np.arange(NC)
+    mt = mu[:NC] * np.asarray(kernel(ns, NC)) * hn(ns)
+    rho[idx] = np.real(lin_trans(x, ns, mt) / (a * np.pi * np.sqrt(1 - x * x)))
+    return E_grid, rho
+
+
+def gauss_chebyshev(n):
+    """First-kind Gauss-Chebyshev nodes cos((2k-1)pi/(2n)) (ascending), weights pi/n."""
+    k = np.arange(1, n + 1)
+    return np.sort(np.cos((2 * k - 1) * np.pi / (2 * n))), np.full(n, np.pi / n)
+
+
+def kubo_bastin_cond(mu2D, a, Ef, *, b=0.0, NH, area, beta=np.inf, kernel=jackson, NC=None,
+                     quad_N=None, edge_cutoff=1e-3):
+    """sigma_ab(Ef) (src/applications/conductivity.jl:101-157): Gauss-Chebyshev
+    nodes inside the edge cutoff with nonzero Fermi weight, and
+    sum_k phi_k sum_nm mu~[n,m] Gamma_nm(x_k) through the factorised GEMMs."""
+    mu2D = np.asarray(mu2D)
+    NC = mu2D.shape[0] if NC is None else min(NC, *mu2D.shape)
+    quad_N = 8 * NC if quad_N is None else quad_N
+    w = np.asarray(kernel(np.arange(NC), NC)) * hn(np.arange(NC))
+    mt = mu2D[:NC, :NC] * w[:, None] * w[None, :]
+    nodes, weights = gauss_chebyshev(quad_N)
+    E = a * nodes + b
+    if np.isinf(beta):
+        fv = ((E < Ef).astype(float) + (E <= Ef).astype(float)) / 2
+    else:
+        with np.errstate(over="ignore"):
+            fv = 1.0 / (np.exp((E - Ef) * beta) + 1.0)
+    keep = (np.abs(nodes) < 1 - edge_cutoff) & (fv != 0)
+    if not np.any(keep):
+        return 0.0
+    x = nodes[keep]
+    phi = weights[keep] * fv[keep] / (1 - x ** 2) ** 1.5
+    ns = np.arange(NC)[:, None]
+    th = np.arccos(x)[None, :]
+    s = np.sqrt(1 - x ** 2)[None, :]
+    C = np.cos(ns * th)
+    Em = np.exp(1j * ns * th)
+    WM = (x[None, :] - 1j * ns * s) * Em
+    WN = (x[None, :] + 1j * ns * s) * np.conj(Em)
+    acc = np.sum(phi[None, :] * ((mt.T @ C) * WM)) + np.sum(phi[None, :] * ((mt @ C) * WN))
+    return float(-2 * NH / (area * a ** 2) * acc.real)
+
+
+def kubo_gamma_direct(mt, x):
+    """sum_nm mu~[n,m] Gamma_nm(x) with the unfactorised kernel of Garcia et al.
+    (PRL 114, 116602): Gamma_nm = (x - i m s) e^{i m th} T_n + (x + i n s) e^{-i n th} T_m."""
+    NC = mt.shape[0]
+    th = np.arccos(x)
+    s = np.sqrt(1 - x * x)
+    tot = 0j
+    for n in range(NC):
+        for m in range(NC):
+            g = ((x - 1j * m * s) * np.exp(1j * m * th) * np.cos(n * th) +
+                 (x + 1j * n * s) * np.exp(-1j * n * th) * np.cos(m * th))
+            tot += mt[n, m] * g
+    return tot

@@ -1,6 +1,7 @@
-"""Host-side logic of the package (no GPU): damping kernels, DOS and
-Kubo–Bastin reconstruction, rescaling, the planner, the probe generator and
-the model builders.  Moments for the physics checks come from the oracle."""
+"""Host-side logic of the package (no GPU): damping kernels, the DOS
+derivative path, rescaling, the planner, the probe generator and the model
+builders; and the oracle's reconstruction heads (dos, kubo_bastin_cond)
+against the reference's physics checks.  Moments come from the oracle."""
 import math
 
 import numpy as np
@@ -79,7 +80,7 @@ def test_chain_dos_analytic_and_sum_rule():
     NC, NR = 256, 24
     psi = K.random_phase_vectors(Xoshiro256pp(7), N, NR)
     mu = oracle.kpm_1d(Hn, NC, psi).mean(axis=0).real
-    E, rho = K.dos_raw(mu, a, b=b, N_tilde=1024)
+    E, rho = oracle.dos(mu, a, b=b, N_tilde=1024)
     dE = E[1] - E[0]
     assert abs(np.sum(rho) * dE - 1.0) < 2e-2
     exact = lambda e: 1 / (np.pi * np.sqrt(4 * t * t - (e - shift) ** 2))  # noqa: E731
@@ -94,7 +95,7 @@ def test_dos_derivatives_vs_finite_differences_and_dos0():
     mu = rng.rand(NC) - 0.5
     Eg = b + a * np.array([-0.45, 0.2, 0.55])
     h = 1e-3 * a
-    rho0 = lambda E: K.dos_raw(mu, a, b=b, E_grid=[E], N_tilde=1)[1][0]  # noqa: E731
+    rho0 = lambda E: oracle.dos(mu, a, b=b, E_grid=[E], N_tilde=1)[1][0]  # noqa: E731
     stencil = [(-rho0(E + 2 * h) + 16 * rho0(E + h) - 30 * rho0(E) + 16 * rho0(E - h) -
                 rho0(E - 2 * h)) / (12 * h * h) for E in Eg]
     _, d2 = K.dos_raw(mu, a, b=b, E_grid=Eg, N_tilde=3, dE_order=2)
@@ -105,7 +106,7 @@ def test_dos_derivatives_vs_finite_differences_and_dos0():
     np.testing.assert_allclose(d1, fd, rtol=1e-5)
     _, d2c = K.dos_raw(mu, a, b=b, E_grid=[b], N_tilde=1, dE_order=2)
     assert d2c[0] == pytest.approx(K.dos0_raw(mu, a, dE_order=2), rel=1e-10)
-    _, r0 = K.dos_raw(mu, a, b=b, E_grid=[b], N_tilde=1)
+    _, r0 = oracle.dos(mu, a, b=b, E_grid=[b], N_tilde=1)
     assert r0[0] == pytest.approx(K.dos0_raw(mu, a), rel=1e-10)
 
 
@@ -184,16 +185,35 @@ def test_haldane_sigma_xy_quantised():
     assert abs(C) == 1 and abs(C_fhs - C) < 1e-10
     s_ed = _ed_hall_T0(H, Jx, Jy, area, 0.0)
     assert abs(s_ed - C) < 0.02
-    s = K.kubo_bastin_cond_raw(mu, a, 0.0, b=b, NH=D, area=area)
+    s = oracle.kubo_bastin_cond(mu, a, 0.0, b=b, NH=D, area=area)
     assert s == pytest.approx(s_ed, abs=0.03)
     assert s == pytest.approx(C, abs=0.05)
-    sT = K.kubo_bastin_cond_raw(mu, a, 0.0, b=b, NH=D, area=area, beta=50.0)
+    sT = oracle.kubo_bastin_cond(mu, a, 0.0, b=b, NH=D, area=area, beta=50.0)
     assert sT == pytest.approx(s, abs=0.01)
     # a^2 counterfactual: currents from H_norm divide sigma by a^2
     Jxn, Jyn = Jx / a, Jy / a
     mu_n = oracle.kpm_2d(Hn, Jxn, Jyn, NC, psi, psi, NC, NC)
-    sn = K.kubo_bastin_cond_raw(mu_n, a, 0.0, b=b, NH=D, area=area)
+    sn = oracle.kubo_bastin_cond(mu_n, a, 0.0, b=b, NH=D, area=area)
     assert sn == pytest.approx(s / a ** 2, rel=1e-8)
+
+
+def test_oracle_kubo_factorisation_vs_direct_gamma():
+    """The oracle's factorised node contraction (two GEMMs per chunk,
+    conductivity.jl:143-155) equals the direct double sum of Gamma_nm."""
+    rng = np.random.default_rng(5)
+    NC, a, b, NH, area = 7, 1.3, 0.1, 10, 2.0
+    mu = rng.standard_normal((NC, NC)) + 1j * rng.standard_normal((NC, NC))
+    quad_N = 16
+    s = oracle.kubo_bastin_cond(mu, a, 0.2, b=b, NH=NH, area=area, quad_N=quad_N)
+    w = oracle.jackson(np.arange(NC), NC) * oracle.hn(np.arange(NC))
+    mt = mu * w[:, None] * w[None, :]
+    x, wt = oracle.gauss_chebyshev(quad_N)
+    E = a * x + b
+    f = ((E < 0.2).astype(float) + (E <= 0.2).astype(float)) / 2
+    keep = (np.abs(x) < 1 - 1e-3) & (f != 0)
+    acc = sum(wt[k] * f[k] / (1 - x[k] ** 2) ** 1.5 * oracle.kubo_gamma_direct(mt, x[k])
+              for k in np.where(keep)[0])
+    assert s == pytest.approx(-2 * NH / (area * a ** 2) * acc.real, rel=1e-12)
 
 
 def test_gauss_chebyshev_sum_rule():
