@@ -178,6 +178,15 @@ int kpm_cheb_action(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t K, int64
 int kpm_cheb_action_block(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t K, const kpm_block* V,
                           const double* C, kpm_block* out, int64_t check_every);
 
+/* ---- spectral edges for normalizeH (src/utils/external.jl:41-76) -------------
+ * The reference estimates the extreme eigenvalues with Arpack eigs on the
+ * host (tol 1e-3, maxiter 300); this runs an FP64 Lanczos recurrence on the
+ * device from a seeded random start vector and returns the extreme Ritz
+ * values once both move by less than tol * max(|emin|, |emax|) over 10 steps
+ * (or at an invariant subspace, or after max_iter steps).  iters may be NULL. */
+int kpm_spectral_edges(kpm_ctx* ctx, const kpm_csr* H, int64_t max_iter, double tol, uint64_t seed,
+                       double* emin, double* emax, int64_t* iters);
+
 /* ---- reconstruction heads (FP64, device) ------------------------------------
  * chebyshev_lin_trans (src/utils/chebyshev_lintrans.jl:12-27; the device
  * method ext/KPMCUDAExt.jl:207-236) as used by dos (src/applications/dos.jl:

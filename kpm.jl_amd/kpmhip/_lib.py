@@ -85,6 +85,7 @@ _SIGS = {
                          C.c_int, _pd]),
     "kpm_cheb_action": (C.c_int, [_p, _p, _i64, _i64, _i64, _i64, _pd, _pd, _pd, C.c_int, _i64]),
     "kpm_cheb_action_block": (C.c_int, [_p, _p, _i64, _i64, _p, _pd, _p, _i64]),
+    "kpm_spectral_edges": (C.c_int, [_p, _p, _i64, C.c_double, C.c_uint64, _pd, _pd, _pi64]),
     "kpm_cheb_lin_trans": (C.c_int, [_p, _i64, _pd, _i64, _pi64, _pd, _pd]),
     "kpm_kubo_bastin_nodes": (C.c_int, [_p, _i64, _pd, _i64, _pd, _pd, _pd]),
     "kpm_1d_adjoint": (C.c_int, [_p, _p, _i64, _i64, _i64, _pd, _pd, _pd, C.c_int, _i64]),

@@ -94,12 +94,13 @@ def npairs(m: GreenMoments) -> int:
     return int(m.mu.shape[1])
 
 
-def rescale(H, *, center: bool = False, eps: float = 0.1, fixed_a: float = 0.0):
-    """src/frontend.jl:149-161"""
+def rescale(H, *, center: bool = False, eps: float = 0.1, fixed_a: float = 0.0,
+            method: str = "lanczos"):
+    """src/frontend.jl:149-161 (``method``: the normalizeH edge estimator)"""
     if center:
-        a, b, Hn = normalizeH(H, eps=eps, fixed_a=fixed_a, center=True)
+        a, b, Hn = normalizeH(H, eps=eps, fixed_a=fixed_a, center=True, method=method)
     else:
-        a, Hn = normalizeH(H, eps=eps, fixed_a=fixed_a)
+        a, Hn = normalizeH(H, eps=eps, fixed_a=fixed_a, method=method)
         b = 0.0
     af, bf = float(a), float(b)
     if not (np.isfinite(af) and np.isfinite(bf)):

@@ -75,7 +75,7 @@ def test_chain_dos_analytic_and_sum_rule():
     """test/dos_test.jl:214-249 with oracle moments."""
     N, t, shift = 4000, 1.0, 0.7
     H = models.chain(N, t=t, shift=shift)
-    a, b, Hn = K.normalizeH(H, center=True)
+    a, b, Hn = K.normalizeH(H, center=True, method="arpack")
     assert abs(b - shift) < 0.05
     NC, NR = 256, 24
     psi = K.random_phase_vectors(Xoshiro256pp(7), N, NR)
@@ -118,18 +118,18 @@ def test_normalizeH_hermiticity_and_center():
     mask = rng.rand(n, n) < 0.05
     A = sp.csc_matrix(np.where(mask, (rng.rand(n, n) - 0.5) + 1j * (rng.rand(n, n) - 0.5), 0))
     with pytest.raises(K.ArgumentError):
-        K.normalizeH(A + 2 * A.conj().T)
+        K.normalizeH(A + 2 * A.conj().T, method="arpack")
     H = A + A.conj().T
-    a, Hn = K.normalizeH(H)
+    a, Hn = K.normalizeH(H, method="arpack")
     assert np.max(np.abs(np.linalg.eigvalsh(Hn.toarray()))) < 1
     Hs = H + 0.7 * sp.identity(n)
-    a, b, Hsn = K.normalizeH(Hs, center=True)
+    a, b, Hsn = K.normalizeH(Hs, center=True, method="arpack")
     ev = np.linalg.eigvalsh(Hsn.toarray())
     assert abs(b - 0.7) < 0.1 * a
     assert ev.min() > -1 and ev.max() < 1
     assert abs(ev.max() + ev.min()) < 0.02
     with pytest.raises(K.ArgumentError):
-        K.normalizeH(H, fixed_a=2.0, center=True)
+        K.normalizeH(H, fixed_a=2.0, center=True, method="arpack")
 
 
 # ---- Kubo–Bastin (test/kubo_bastin_test.jl) ----------------------------------------------------
@@ -176,7 +176,7 @@ def test_haldane_sigma_xy_quantised():
     t2 = 0.2
     H, Jx, Jy, area = models.haldane_model(6, 6, t=1.0, t2=t2)
     D = H.shape[0]
-    a, b, Hn = K.normalizeH(H, center=True)
+    a, b, Hn = K.normalizeH(H, center=True, method="arpack")
     NC = 96
     psi = np.eye(D, dtype=complex)
     mu = oracle.kpm_2d(Hn, Jx, Jy, NC, psi, psi, NC, NC)
