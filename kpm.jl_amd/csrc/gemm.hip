@@ -283,3 +283,43 @@ int launch_contract(kpm_ctx* ctx, const void* Lslots, int nL, const void* Rslots
 }
 
 }  // namespace kpm
+
+using namespace kpm;
+
+extern "C" {
+
+int kpm_bench_contract(kpm_ctx* ctx, int64_t nL, int64_t nR, int64_t K, int vec_prec, int gemm_mode,
+                       int64_t reps, float* ms) {
+  if (!ctx || !ms || nL < 1 || nR < 1 || reps < 1 || K < 16 || K % 16 != 0)
+    return fail(KPM_EINVAL, "bad arguments (K must be a positive multiple of 16)");
+  if (vec_prec != KPM_VEC_F64 && vec_prec != KPM_VEC_F32) return fail(KPM_EINVAL, "bad vec_prec");
+  if (gemm_mode != KPM_GEMM_F64 && gemm_mode != KPM_GEMM_F32) return fail(KPM_EINVAL, "bad gemm_mode");
+  KPM_HIP_TRY(hipSetDevice(ctx->device));
+  const size_t eb = (size_t)elem_bytes(vec_prec);
+  const int64_t NC = std::max(nL, nR);
+  Buffer lb, rb, mb;
+  int rc = ensure(ctx, lb, (size_t)(nL * K) * eb);
+  if (rc == KPM_OK) rc = ensure(ctx, rb, (size_t)(nR * K) * eb);
+  if (rc == KPM_OK) rc = ensure(ctx, mb, (size_t)(NC * NC) * sizeof(double2));
+  if (rc == KPM_OK && (hipMemsetAsync(lb.ptr, 0, lb.bytes, ctx->stream) != hipSuccess ||
+                       hipMemsetAsync(rb.ptr, 0, rb.bytes, ctx->stream) != hipSuccess ||
+                       hipMemsetAsync(mb.ptr, 0, mb.bytes, ctx->stream) != hipSuccess))
+    rc = fail(KPM_EHIP, "workspace clear failed");
+  if (rc == KPM_OK) rc = launch_contract(ctx, lb.ptr, (int)nL, rb.ptr, (int)nR, K, vec_prec, gemm_mode,
+                                         1.0, mb.ptr, NC, 0, 0, KPM_PARITY_NONE);  // warm-up
+  if (rc == KPM_OK && hipEventRecord(ctx->events[14], ctx->stream) != hipSuccess)
+    rc = fail(KPM_EHIP, "event record failed");
+  for (int64_t r = 0; r < reps && rc == KPM_OK; ++r)
+    rc = launch_contract(ctx, lb.ptr, (int)nL, rb.ptr, (int)nR, K, vec_prec, gemm_mode, 1.0, mb.ptr, NC,
+                         0, 0, KPM_PARITY_NONE);
+  if (rc == KPM_OK && (hipEventRecord(ctx->events[15], ctx->stream) != hipSuccess ||
+                       hipEventSynchronize(ctx->events[15]) != hipSuccess ||
+                       hipEventElapsedTime(ms, ctx->events[14], ctx->events[15]) != hipSuccess))
+    rc = fail(KPM_EHIP, "event timing failed");
+  (void)hipStreamSynchronize(ctx->stream);
+  for (Buffer* b : {&lb, &rb, &mb})
+    if (b->ptr) (void)hipFree(b->ptr);
+  return rc;
+}
+
+}  // extern "C"
