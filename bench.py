@@ -75,7 +75,6 @@ def main() -> int:
 
     import kpmhip as K
     from kpmhip import models
-    from kpmhip.rng import Xoshiro256pp
 
     if not K.activate(0):
         raise SystemExit("no HIP device visible")
@@ -85,7 +84,10 @@ def main() -> int:
     H = (models.graphene(L, L, disorder=0.5, seed=42) / 3.6).tocsc()
     NH, nnz = H.shape[0], H.nnz
     print(f"[bench] NH={NH} nnz={nnz}; probes NR={NR} ...", file=sys.stderr, flush=True)
-    psi = K.random_phase_vectors(Xoshiro256pp(42), NH, NR)
+    # probes drawn on the device from the same xoshiro256++ stream as
+    # random_phase_vectors(Xoshiro256pp(42), NH, NR); one host copy feeds the
+    # FP32 leg and the CPU baseline, so every leg sees identical probes
+    psi = K.DeviceBlock(NH, NR, "f64").random_phase(42).download()
     Hd = K.to_device(H)
 
     def run_leg(precision: str):

@@ -103,6 +103,13 @@ int kpm_block_upload(kpm_block* b, const double* host);
 int kpm_block_download(const kpm_block* b, double* host);
 int kpm_block_copy(kpm_block* dst, const kpm_block* src);
 int kpm_block_zero(kpm_block* b);
+/* random_phase_vectors + normalize_by_col on the device (src/frontend.jl:
+ * 212-216, src/utils/vectors.jl:14-20; kpm_1d! draws its random probes on the
+ * device, src/moment.jl:493-497): b = exp(2 pi i U) with unit-norm columns.
+ * U is the lane-parallel xoshiro256++ stream of kpmhip.rng (lane j seeded by
+ * splitmix64 outputs 4j..4j+3 of `seed`, value i of the column-major fill
+ * from lane i % lanes), not Julia's Xoshiro. */
+int kpm_block_random_phase(kpm_block* b, uint64_t seed, int64_t lanes);
 
 /* ---- recurrence primitives: ext/KPMCUDAExt.jl:84-121 ---------------------
  * chebyshev_iter_single(H, V_pp, V_p[, V_out]) (chebyshev_iteration.jl:39-91,

@@ -70,6 +70,7 @@ _SIGS = {
     "kpm_block_download": (C.c_int, [_p, _pd]),
     "kpm_block_copy": (C.c_int, [_p, _p]),
     "kpm_block_zero": (C.c_int, [_p]),
+    "kpm_block_random_phase": (C.c_int, [_p, C.c_uint64, _i64]),
     "kpm_cheb_step": (C.c_int, [_p, _p, _p, _p, _p]),
     "kpm_spmm": (C.c_int, [_p, _p, C.c_double, _p, C.c_double, _p]),
     "kpm_colwise_dot": (C.c_int, [_p, _pd, _i64, _i64, _p, _p, C.c_double, _i64, C.c_double]),

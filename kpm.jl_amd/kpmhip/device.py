@@ -260,6 +260,14 @@ class DeviceBlock:
     def zero(self):
         check(self.ctx.lib.kpm_block_zero(self.handle), "block zero")
 
+    def random_phase(self, seed: int, lanes: int = 1024):
+        """Fill with exp(2 pi i U), unit-norm columns, drawn on the device from
+        the same lane-parallel xoshiro256++ stream as
+        ``random_phase_vectors(Xoshiro256pp(seed, lanes), NH, NR)``."""
+        check(self.ctx.lib.kpm_block_random_phase(self.handle, int(seed) & 0xFFFFFFFFFFFFFFFF,
+                                                  int(lanes)), "random_phase")
+        return self
+
     def __del__(self):
         try:
             if getattr(self, "handle", None) and not _shutdown[0]:

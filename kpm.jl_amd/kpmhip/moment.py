@@ -69,7 +69,16 @@ def kpm_1d_(H, NC: int, NR: int, NH: int, mu_all: np.ndarray, psi_in=None, psi_i
     prec = prec_code(precision)
     out = np.zeros((NR, NC), dtype=np.complex128, order="F")
     if psi_in is None:
-        psi_in = random_probes(NH, NR)
+        # random probes drawn on the device (src/moment.jl:493-497)
+        if NC % 2 != 0:
+            raise AssertionError("Invalid NC: NC should be even.")
+        if Hd.shape[0] != NH:
+            raise AssertionError("H and psi_in disagree on NH")
+        blk = DeviceBlock(NH, NR, precision, ctx)
+        blk.random_phase(int(np.random.SeedSequence().entropy) & 0xFFFFFFFFFFFFFFFF)
+        check(ctx.lib.kpm_1d_block(ctx.handle, Hd.handle, NC, blk.handle, dptr(out)), "kpm_1d!")
+        mu_all[...] = out
+        return None
     if psi_in_r is None:
         if NC % 2 != 0:
             raise AssertionError("Invalid NC: NC should be even.")
