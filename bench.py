@@ -314,11 +314,12 @@ def bench_adjoint(args) -> int:
     4 * 1.1 + 1 = 5.4), NC = 1024, NR = 16, Jackson kernel; loss = sum over 256
     energies of (rho(E) - rho*(E))^2 with rho* the DOS of a second disorder
     seed.  One step = forward moments + loss + reverse Chebyshev gradient
-    (sqrt(NC) checkpoints) + per-site / per-bond gradients, host H in.
-    Algorithmic bytes (FP64, per step): forward NC/2 fused passes; adjoint
-    (NC-1) checkpoint passes + (NC-1) recomputed passes + (NC-1) reverse
-    passes (one extra source read) + (NC-1) gradient contractions (read beta,
-    gather alpha once, read-modify-write the nnz gradient)."""
+    (every forward vector kept in HBM, 69 GB here) + per-site / per-bond
+    gradients, host H in.  Algorithmic bytes (FP64, per step): forward
+    moments NC/2 fused passes; adjoint NC-2 forward passes that keep alpha,
+    NC-1 reverse passes (one extra source read), and the gradient terms:
+    beta and alpha read once each per reverse step, the nnz gradient read and
+    written once per 8-step window."""
     import numpy as np
 
     import kpmhip as K
@@ -363,7 +364,9 @@ def bench_adjoint(args) -> int:
     vec = NH * nrp * 16
     hb = nnz * (8 + 4) + (NH + 1) * 4
     fwd = (NC // 2) * (hb + 3 * vec)
-    adj = (NC - 1) * ((hb + 3 * vec) * 2 + (hb + 4 * vec) + (2 * vec + nnz * (4 + 32) + (NH + 1) * 4))
+    windows = -(-(NC - 1) // 8)
+    adj = ((NC - 2) * (hb + 3 * vec) + (NC - 1) * (hb + 4 * vec) + (NC - 1) * 2 * vec
+           + windows * (nnz * (4 + 32) + (NH + 1) * 4))
     total = fwd + adj
     out = {"metric": "DOS-loss gradient step time (forward + reverse Chebyshev)",
            "value": round(wall * 1e3, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps,

@@ -222,6 +222,14 @@ int kpm_kubo_bastin_nodes(kpm_ctx* ctx, int64_t NC, const double* mu_tilde, int6
 int kpm_1d_adjoint(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t NR, int64_t NH,
                    const double* psi, const double* g, double* grad, int vec_prec,
                    int64_t ckpt_stride);
+/* Host bookkeeping for per-bond gradients (no reference counterpart): for a
+ * CSC pattern with sorted rows holding both (i, j) and (j, i), and grad on that
+ * pattern (the output of kpm_1d_adjoint), write every pair i < j with
+ * dL/dt_ij = Re(u M_ij + conj(u) M_ji) / a, u = H_ij / |H_ij|.  bi, bj, val
+ * need room for nnz entries; nbonds receives the count (0-based indices). */
+int kpm_bond_gradient(int64_t n, const int64_t* colptr, const int64_t* rowval, const double* h,
+                      const double* grad, double a, int64_t* bi, int64_t* bj, double* val,
+                      int64_t* nbonds);
 
 /* ---- batched realisations ---------------------------------------------------
  * No single-call counterpart in the reference, which runs one disorder

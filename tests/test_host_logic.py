@@ -273,3 +273,28 @@ def test_models_hermitian_and_sizes():
     ev = np.linalg.eigvalsh(models.chain(10, shift=0.3).toarray())
     np.testing.assert_allclose(np.sort(ev), np.sort(0.3 - 2 * np.cos(2 * np.pi * np.arange(10) / 10)),
                                atol=1e-12)
+
+
+def test_bond_gradient_bookkeeping():
+    """kpm_bond_gradient (host, in libkpm_hip) equals the per-pair formula
+    Re(u M_ij + conj(u) M_ji)/a evaluated by scipy indexing."""
+    H = models.square_disordered(6, seed=4).tolil()
+    H[0, 1] = H[0, 1] * np.exp(0.7j)
+    H[1, 0] = np.conj(H[0, 1])
+    H = sp.csc_matrix(H)
+    A = sp.csc_matrix(H, dtype=np.complex128)
+    A.sort_indices()
+    rng = np.random.default_rng(2)
+    G = sp.csc_matrix((rng.random(A.nnz) + 1j * rng.random(A.nnz), A.indices.copy(),
+                       A.indptr.copy()), shape=A.shape)
+    got = K.bond_gradient(H, G, 2.5).tocsr()
+    C = sp.coo_matrix(H)
+    m = C.row < C.col
+    i, j, h = C.row[m], C.col[m], C.data[m]
+    Gr = G.tocsr()
+    gij, gji = np.asarray(Gr[i, j]).ravel(), np.asarray(Gr[j, i]).ravel()
+    u = h / np.abs(h)
+    ref = sp.coo_matrix((np.real(u * gij + np.conj(u) * gji) / 2.5, (i, j)), shape=A.shape).tocsr()
+    assert got.nnz == ref.nnz and abs(got - ref).max() < 1e-15
+    with pytest.raises(K.ArgumentError):  # gradient on a different pattern
+        K.bond_gradient(H, sp.identity(A.shape[0], format="csc"), 2.5)
