@@ -57,28 +57,71 @@ static int upload_into(kpm_ctx* ctx, const double* host, kpm_block* b) {
   return launch_upload(ctx, ctx->staging.ptr, b);
 }
 
+// Dot steps alternate between two halves of ctx->partials; each launch also
+// finishes the previous step's moment columns (PendingCombine), and only the
+// last step's columns take a separate combine launch.
+struct DotSteps {
+  kpm_ctx* ctx;
+  const kpm_csr* H;
+  int64_t NR, NRp;
+  double2* half[2];
+  PendingCombine pend;
+  bool has_pend = false;
+  int k = 0;
+
+  int init(int nd) {
+    const size_t per = (size_t)kMaxGridPerCU * std::max(ctx->num_cus, 1) * nd * NRp;
+    KPM_TRY(ensure(ctx, ctx->partials, 2 * per * sizeof(double2)));
+    half[0] = (double2*)ctx->partials.ptr;
+    half[1] = half[0] + per;
+    return KPM_OK;
+  }
+  // one fused pass whose moment columns are described by c
+  int step(const kpm_block* x, const kpm_block* z, kpm_block* out, const kpm_block* l, double alpha,
+           double beta, int mode, const Combine& c) {
+    StepLaunch s{H, x, z, out, l, alpha, beta, mode};
+    s.part_out = half[k & 1];
+    s.pend = has_pend ? &pend : nullptr;
+    int G = 0;
+    KPM_TRY(launch_step(ctx, s, tuned_config(ctx, H, x->NR, x->prec, mode), &G));
+    pend.part = half[k & 1];
+    pend.mu = (double2*)ctx->mu.ptr;
+    pend.cb = c;
+    pend.G = G;
+    pend.NRp = (int)NRp;
+    pend.NR = (int)NR;
+    has_pend = true;
+    ++k;
+    return KPM_OK;
+  }
+  int finish() {
+    if (!has_pend) return KPM_OK;
+    has_pend = false;
+    return launch_combine_from(ctx, pend.part, pend.G, NR, NRp, pend.cb, ctx->mu.ptr);
+  }
+};
+
 // ---- kpm_1d! equal-vector, on a device probe block -------------------------
 // w0 holds psi on entry; w1 is scratch.  mu (device) receives NR x NC.
 static int run_1d_doubling(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, kpm_block* w0,
                            kpm_block* w1) {
   const int64_t NR = w0->NR, NRp = w0->NRp;
   KPM_TRY(mu_buffer(ctx, NR, NC));
+  DotSteps ds{ctx, H, NR, NRp, {nullptr, nullptr}};
+  KPM_TRY(ds.init(2));
   kpm_block* v[2] = {w0, w1};
-  int G = 0;
   // alpha_1 = H alpha_0 with mu0 = <a0|a0>, mu1 = <a0|a1> (:442-449)
-  KPM_TRY(step(ctx, H, v[0], nullptr, v[1], nullptr, 1.0, 0.0, MODE_DBL, &G));
   Combine c0{2, {0, 1}, {1.0, 1.0}, {-1, -1}, {0.0, 0.0}};
-  KPM_TRY(launch_combine(ctx, G, NR, NRp, c0, ctx->mu.ptr, NC));
+  KPM_TRY(ds.step(v[0], nullptr, v[1], nullptr, 1.0, 0.0, MODE_DBL, c0));
   int ip = 1, ipp = 0;
   for (int64_t n = 2; n <= NC / 2; ++n) {  // :460-487
-    KPM_TRY(step(ctx, H, v[ip], v[ipp], v[ipp], nullptr, 2.0, -1.0, MODE_DBL, &G));
     // mu_{2n-2} = 2<a_{n-1}|a_{n-1}> - mu0 ; mu_{2n-1} = 2<a_{n-1}|a_n> - mu1
     Combine c{2, {(int)(2 * n - 2), (int)(2 * n - 1)}, {2.0, 2.0}, {0, 1}, {-1.0, -1.0}};
-    KPM_TRY(launch_combine(ctx, G, NR, NRp, c, ctx->mu.ptr, NC));
+    KPM_TRY(ds.step(v[ip], v[ipp], v[ipp], nullptr, 2.0, -1.0, MODE_DBL, c));
     ip = 1 - ip;
     ipp = 1 - ipp;
   }
-  return KPM_OK;
+  return ds.finish();
 }
 
 // ---- bra/ket and current: one dot per step against l ------------------------
@@ -90,19 +133,19 @@ static int run_1d_lr(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, const kpm_block
   KPM_TRY(launch_dot(ctx, l, w0, &G));  // mu0 = <l|a0>
   Combine c0{1, {0, 0}, {1.0, 0.0}, {-1, -1}, {0.0, 0.0}};
   KPM_TRY(launch_combine(ctx, G, NR, NRp, c0, ctx->mu.ptr, NC));
+  DotSteps ds{ctx, H, NR, NRp, {nullptr, nullptr}};
+  KPM_TRY(ds.init(1));
   kpm_block* v[2] = {w0, w1};
-  KPM_TRY(step(ctx, H, v[0], nullptr, v[1], l, 1.0, 0.0, MODE_LR, &G));  // a1 = H a0, mu1
   Combine c1{1, {1, 0}, {1.0, 0.0}, {-1, -1}, {0.0, 0.0}};
-  KPM_TRY(launch_combine(ctx, G, NR, NRp, c1, ctx->mu.ptr, NC));
+  KPM_TRY(ds.step(v[0], nullptr, v[1], l, 1.0, 0.0, MODE_LR, c1));  // a1 = H a0, mu1
   int ip = 1, ipp = 0;
   for (int64_t n = 3; n <= NC; ++n) {
-    KPM_TRY(step(ctx, H, v[ip], v[ipp], v[ipp], l, 2.0, -1.0, MODE_LR, &G));
     Combine c{1, {(int)(n - 1), 0}, {1.0, 0.0}, {-1, -1}, {0.0, 0.0}};
-    KPM_TRY(launch_combine(ctx, G, NR, NRp, c, ctx->mu.ptr, NC));
+    KPM_TRY(ds.step(v[ip], v[ipp], v[ipp], l, 2.0, -1.0, MODE_LR, c));
     ip = 1 - ip;
     ipp = 1 - ipp;
   }
-  return KPM_OK;
+  return ds.finish();
 }
 
 }  // namespace kpm

@@ -119,6 +119,7 @@ enum StepMode { MODE_NONE = 0, MODE_DBL = 1, MODE_LR = 2, MODE_DBL_SEG = 3, MODE
 // One fused pass out <- alpha*A*x + beta*z with optional dot partials:
 //   MODE_DBL: d0 = <x|x>, d1 = <x|out>;  MODE_LR: d0 = <l|out>.
 // z may alias out (in place) and may be null when beta == 0.
+struct PendingCombine;
 struct StepLaunch {
   const kpm_csr* A;
   const kpm_block* x;
@@ -127,7 +128,9 @@ struct StepLaunch {
   const kpm_block* l;
   double alpha, beta;
   int mode;
-  double gamma = 0.0;  // MODE_SRC only
+  double gamma = 0.0;                     // MODE_SRC only
+  double2* part_out = nullptr;            // dot partials (null: ctx->partials)
+  const PendingCombine* pend = nullptr;   // previous step's combine, run in this launch
 };
 // Launch geometry picked by the tuner for (LPR, rows per lane, grid size).
 struct StepConfig {
@@ -161,6 +164,21 @@ struct Combine {
 };
 int launch_combine(kpm_ctx* ctx, int grid, int64_t NR, int64_t NRp, const Combine& c,
                    void* mu_dev, int64_t NCmu);
+// the same reduction over an explicit partials buffer
+int launch_combine_from(kpm_ctx* ctx, const double2* partials, int grid, int64_t NR, int64_t NRp,
+                        const Combine& c, void* mu_dev);
+// A combine deferred into the next recurrence launch of the same engine: the
+// first nd*NR workgroups of that launch reduce these partials (written by the
+// previous launch, complete at the kernel boundary) before their own tiles,
+// which saves one launch per step.  Dot steps then alternate between two
+// partials buffers of at most kMaxGridPerCU * CUs workgroups each.
+struct PendingCombine {
+  const double2* part = nullptr;
+  double2* mu = nullptr;
+  Combine cb{0, {0, 0}, {0.0, 0.0}, {-1, -1}, {0.0, 0.0}};
+  int G = 0, NRp = 0, NR = 0;
+};
+constexpr int kMaxGridPerCU = 16;
 // layout conversions between the host column-major ComplexF64 block (in the
 // staging buffer, already on the device) and a device recurrence block
 int launch_upload(kpm_ctx* ctx, const void* staging, kpm_block* b);

@@ -67,36 +67,8 @@ __global__ void __launch_bounds__(kBlock)
 __global__ void __launch_bounds__(kBlock)
     k_combine(const double2* __restrict__ partials, int G, int nd, int NRp, int NR,
               double2* __restrict__ mu, Combine cb) {
-  const int d = blockIdx.x / NR;
-  const int col = blockIdx.x % NR;
-  double sr = 0.0, si = 0.0;
-  for (int g = threadIdx.x; g < G; g += kBlock) {
-    const double2 v = partials[((size_t)g * nd + d) * NRp + col];
-    sr += v.x;
-    si += v.y;
-  }
   __shared__ double2 sm[kBlock];
-  sm[threadIdx.x] = make_double2(sr, si);
-  __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      const double2 o = sm[threadIdx.x + w];
-      sm[threadIdx.x].x += o.x;
-      sm[threadIdx.x].y += o.y;
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const double2 tot = sm[0];
-    double rr = cb.alpha[d] * tot.x;
-    double ri = cb.alpha[d] * tot.y;
-    if (cb.bcol[d] >= 0) {
-      const double2 bv = mu[col + (size_t)NR * cb.bcol[d]];
-      rr = fma(cb.bscale[d], bv.x, rr);
-      ri = fma(cb.bscale[d], bv.y, ri);
-    }
-    mu[col + (size_t)NR * cb.tcol[d]] = make_double2(rr, ri);
-  }
+  combine_pair(partials, G, nd, NRp, NR, mu, cb, blockIdx.x / NR, blockIdx.x % NR, sm);
 }
 
 // host column-major ComplexF64 (NH x NR) -> row-interleaved block (NH x NRp)
@@ -201,12 +173,19 @@ int launch_step(kpm_ctx* ctx, const StepLaunch& s, const StepConfig& cfg, int* g
   if (ntiles > INT32_MAX) return fail(KPM_EINVAL, "row count too large for the recurrence kernel");
   int G = cfg.grid > 0 ? (cfg.grid + 7) / 8 * 8 : default_grid(ctx, ntiles);
   const int nd = s.mode == MODE_DBL ? 2 : (s.mode == MODE_LR ? 1 : 0);
-  if (nd > 0) KPM_TRY(ensure(ctx, ctx->partials, (size_t)G * nd * x->NRp * sizeof(double2)));
+  double2* part = s.part_out;
+  if (part) {  // caller-owned buffer sized for kMaxGridPerCU * CUs workgroups
+    G = std::min(G, kMaxGridPerCU * std::max(ctx->num_cus, 1) / 8 * 8);
+  } else {
+    if (nd > 0) KPM_TRY(ensure(ctx, ctx->partials, (size_t)G * nd * x->NRp * sizeof(double2)));
+    part = (double2*)ctx->partials.ptr;
+  }
   if ((s.mode == MODE_LR || s.mode == MODE_SRC) && !s.l) return fail(KPM_EINVAL, "missing operand block");
+  const PendingCombine pc = s.pend ? *s.pend : PendingCombine{};
   StepLauncher fn = pick(prec, g, s.A->is_real != 0, s.mode);
   fn(dim3(G, g.chunks), ctx->stream, s.A->rowptr, s.A->colind, s.A->vals, x->d, s.z ? s.z->d : x->d,
-     s.out->d, s.l ? s.l->d : nullptr, (double2*)ctx->partials.ptr, (int)x->NH, g.RW, (int)x->NRp,
-     (int)ntiles, s.alpha, s.beta, g.has_tail, nullptr, nullptr, nullptr, s.gamma);
+     s.out->d, s.l ? s.l->d : nullptr, part, (int)x->NH, g.RW, (int)x->NRp,
+     (int)ntiles, s.alpha, s.beta, g.has_tail, nullptr, nullptr, nullptr, s.gamma, pc);
   KPM_HIP_TRY(hipGetLastError());
   if (grid_used) *grid_used = G;
   return KPM_OK;
@@ -218,7 +197,7 @@ int launch_step_seg(kpm_ctx* ctx, const StepLaunch& s, const StepGeom& g, int G,
   StepLauncher fn = pick(x->prec, g, s.A->is_real != 0, MODE_DBL_SEG);
   fn(dim3(G, g.chunks), ctx->stream, s.A->rowptr, s.A->colind, s.A->vals, x->d, s.z ? s.z->d : x->d,
      s.out->d, nullptr, partials, (int)x->NH, g.RW, (int)x->NRp, (int)ntiles, s.alpha, s.beta,
-     g.has_tail, tile_lo, treal, seg0, 0.0);
+     g.has_tail, tile_lo, treal, seg0, 0.0, PendingCombine{});
   KPM_HIP_TRY(hipGetLastError());
   return KPM_OK;
 }
@@ -265,9 +244,13 @@ int launch_dot(kpm_ctx* ctx, const kpm_block* a, const kpm_block* b, int* grid_u
 
 int launch_combine(kpm_ctx* ctx, int grid, int64_t NR, int64_t NRp, const Combine& c, void* mu_dev,
                    int64_t /*NCmu*/) {
-  hipLaunchKernelGGL(k_combine, dim3((unsigned)(c.nd * NR)), dim3(kBlock), 0, ctx->stream,
-                     (const double2*)ctx->partials.ptr, grid, c.nd, (int)NRp, (int)NR,
-                     (double2*)mu_dev, c);
+  return launch_combine_from(ctx, (const double2*)ctx->partials.ptr, grid, NR, NRp, c, mu_dev);
+}
+
+int launch_combine_from(kpm_ctx* ctx, const double2* partials, int grid, int64_t NR, int64_t NRp,
+                        const Combine& c, void* mu_dev) {
+  hipLaunchKernelGGL(k_combine, dim3((unsigned)(c.nd * NR)), dim3(kBlock), 0, ctx->stream, partials,
+                     grid, c.nd, (int)NRp, (int)NR, (double2*)mu_dev, c);
   KPM_HIP_TRY(hipGetLastError());
   return KPM_OK;
 }

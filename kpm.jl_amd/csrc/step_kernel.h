@@ -120,6 +120,43 @@ __device__ inline void st_stream(float4* p, const float4& v) {
 #endif
 }
 
+// Fixed-order reduction of one (dot d, probe column col) moment over G
+// per-workgroup partials, then columnwise_dot!'s alpha * dot + beta_scale *
+// target[:, beta_col] (src/utils/vectors.jl:103-117).  sm: kBlock double2 of
+// LDS; every thread of the workgroup must call it.
+__device__ inline void combine_pair(const double2* __restrict__ partials, int G, int nd, int NRp,
+                                    int NR, double2* __restrict__ mu, const Combine& cb, int d,
+                                    int col, double2* sm) {
+  double sr = 0.0, si = 0.0;
+  for (int g = threadIdx.x; g < G; g += kBlock) {
+    const double2 v = partials[((size_t)g * nd + d) * NRp + col];
+    sr += v.x;
+    si += v.y;
+  }
+  sm[threadIdx.x] = make_double2(sr, si);
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const double2 o = sm[threadIdx.x + w];
+      sm[threadIdx.x].x += o.x;
+      sm[threadIdx.x].y += o.y;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double2 tot = sm[0];
+    double rr = cb.alpha[d] * tot.x;
+    double ri = cb.alpha[d] * tot.y;
+    if (cb.bcol[d] >= 0) {
+      const double2 bv = mu[col + (size_t)NR * cb.bcol[d]];
+      rr = fma(cb.bscale[d], bv.x, rr);
+      ri = fma(cb.bscale[d], bv.y, ri);
+    }
+    mu[col + (size_t)NR * cb.tcol[d]] = make_double2(rr, ri);
+  }
+  __syncthreads();
+}
+
 template <int UNR>
 struct RowMeta {
   int row[UNR], s[UNR], cnt[UNR];
@@ -143,7 +180,7 @@ __global__ void __launch_bounds__(kBlock)
            const typename Word<PREC>::T* __restrict__ l, double2* __restrict__ partials, int NH,
            int RW, int NRp, int ntiles, double alpha, double beta, int has_tail,
            const int* __restrict__ tile_lo, const int* __restrict__ treal,
-           const int* __restrict__ seg0, double gamma) {
+           const int* __restrict__ seg0, double gamma, PendingCombine pc) {
   using WT = Word<PREC>;
   using T = typename WT::T;
   constexpr int CPL = WT::CPL;
@@ -251,6 +288,12 @@ __global__ void __launch_bounds__(kBlock)
     }
     __syncthreads();
   };
+  // the previous step's moment columns (uniform per workgroup)
+  if constexpr (!SEG && ND > 0) {
+    const int npairs = pc.cb.nd * pc.NR;
+    for (int q = blockIdx.y * gridDim.x + blockIdx.x; q < npairs; q += gridDim.x * gridDim.y)
+      combine_pair(pc.part, pc.G, pc.cb.nd, pc.NRp, pc.NR, pc.mu, pc.cb, q / pc.NR, q % pc.NR, red);
+  }
   int seg = 0, cur = 0;
   if constexpr (SEG) {
     seg = seg0[b];
@@ -399,18 +442,19 @@ __global__ void __launch_bounds__(kBlock)
 
 using StepLauncher = void (*)(dim3, hipStream_t, const int*, const int*, const void*, const void*,
                               const void*, void*, const void*, double2*, int, int, int, int,
-                              double, double, int, const int*, const int*, const int*, double);
+                              double, double, int, const int*, const int*, const int*, double,
+                              const PendingCombine&);
 
 template <int PREC, int LPR, int UNR, int KB, bool REALH, int MODE, bool SEG>
 static void launch_step_t(dim3 grid, hipStream_t st, const int* rp, const int* ci, const void* v,
                           const void* x, const void* z, void* out, const void* l, double2* part,
                           int NH, int RW, int NRp, int ntiles, double alpha, double beta,
                           int has_tail, const int* tile_lo, const int* treal, const int* seg0,
-                          double gamma) {
+                          double gamma, const PendingCombine& pc) {
   using T = typename Word<PREC>::T;
   hipLaunchKernelGGL((k_step<PREC, LPR, UNR, KB, REALH, MODE, SEG>), grid, dim3(kBlock), 0, st, rp,
                      ci, v, (const T*)x, (const T*)z, (T*)out, (const T*)l, part, NH, RW, NRp,
-                     ntiles, alpha, beta, has_tail, tile_lo, treal, seg0, gamma);
+                     ntiles, alpha, beta, has_tail, tile_lo, treal, seg0, gamma, pc);
 }
 
 template <int PREC, int LPR, int UNR, int KB, bool REALH>
