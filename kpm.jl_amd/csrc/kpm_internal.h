@@ -74,13 +74,14 @@ struct kpm_ctx {
   kpm::Buffer slots_l;    // kpm_2d left slot array
   kpm::Buffer slots_r;    // kpm_2d right slot array
   kpm::Buffer ring[5];    // kpm_2d / kpm_3d ring and starter blocks
+  kpm::Buffer pinned;     // page-locked host staging for operator uploads
 };
 
 struct kpm_csr {
   kpm_ctx* ctx = nullptr;
   // position in this CSR of the k-th stored entry of the CSC arrays it was
   // built from (the adjoint returns gradients in the caller's CSC order)
-  std::vector<int64_t> csc_to_csr;
+  std::vector<int> csc_to_csr;
   int64_t n = 0;
   int64_t nnz = 0;
   int is_real = 0;
@@ -106,6 +107,12 @@ int ensure(kpm_ctx* ctx, Buffer& b, size_t bytes);
 // host CSR (0-based int64 rowptr/colind, interleaved complex values) -> device CSR
 int upload_csr(kpm_ctx* ctx, int64_t n, const std::vector<int64_t>& rp, const std::vector<int64_t>& ci,
                const std::vector<double>& v_cplx, kpm_csr** out);
+// CSC arrays -> device CSR through page-locked staging in one scatter pass
+// (conj: build adjoint(A) instead of A); perm, when non-null, receives the
+// CSR position of every CSC entry
+int csr_from_csc_staged(kpm_ctx* ctx, int64_t n, const int64_t* colptr, const int64_t* rowval,
+                        const double* nzval, int base, bool adjoint, std::vector<int>* perm,
+                        kpm_csr** out);
 // validate Julia CSC arrays (square n x n, index base 0 or 1)
 int check_csc(int64_t n, const int64_t* colptr, const int64_t* rowval, int base);
 // A non-owning block view over a cached workspace buffer.

@@ -119,6 +119,108 @@ int upload_csr(kpm_ctx* ctx, int64_t n, const std::vector<int64_t>& rp,
   return KPM_OK;
 }
 
+static int ensure_pinned(kpm_ctx* ctx, size_t bytes) {
+  Buffer& b = ctx->pinned;
+  if (b.bytes >= bytes && b.ptr) return KPM_OK;
+  if (b.ptr) {
+    KPM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    KPM_HIP_TRY(hipHostFree(b.ptr));
+    b.ptr = nullptr;
+    b.bytes = 0;
+  }
+  hipError_t e = hipHostMalloc(&b.ptr, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    b.ptr = nullptr;
+    return fail(KPM_ENOMEM, "hipHostMalloc of " + std::to_string(bytes) + " bytes failed: " +
+                                hipGetErrorString(e));
+  }
+  b.bytes = bytes;
+  return KPM_OK;
+}
+
+int csr_from_csc_staged(kpm_ctx* ctx, int64_t n, const int64_t* colptr, const int64_t* rowval,
+                        const double* nzval, int base, bool adjoint, std::vector<int>* perm,
+                        kpm_csr** out) {
+  const int64_t nnz = colptr[n] - base;
+  if (n >= INT32_MAX || nnz >= INT32_MAX)
+    return fail(KPM_EINVAL, "operator too large for int32 device indices (n or nnz >= 2^31)");
+  bool is_real = true;
+  for (int64_t k = 0; k < nnz && is_real; ++k) is_real = nzval[2 * k + 1] == 0.0;
+  const int64_t nz1 = std::max<int64_t>(nnz, 1);
+  const size_t brp = (size_t)(n + 1) * sizeof(int), bci = (size_t)nz1 * sizeof(int);
+  const size_t bv = (size_t)nz1 * (is_real ? sizeof(double) : 2 * sizeof(double));
+  const size_t off_ci = (brp + 255) / 256 * 256, off_v = off_ci + (bci + 255) / 256 * 256;
+  KPM_TRY(ensure_pinned(ctx, off_v + bv));
+  char* base_p = (char*)ctx->pinned.ptr;
+  int* rp = (int*)base_p;
+  int* ci = (int*)(base_p + off_ci);
+  double* vr = (double*)(base_p + off_v);
+  const double sgn = adjoint ? -1.0 : 1.0;
+  if (adjoint) {
+    // row j of adjoint(A) is column j of A, conjugated: no scatter
+    for (int64_t j = 0; j <= n; ++j) rp[j] = (int)(colptr[j] - base);
+    for (int64_t k = 0; k < nnz; ++k) {
+      ci[k] = (int)(rowval[k] - base);
+      if (is_real) {
+        vr[k] = nzval[2 * k];
+      } else {
+        vr[2 * k] = nzval[2 * k];
+        vr[2 * k + 1] = sgn * nzval[2 * k + 1];
+      }
+    }
+  } else {
+    // CSR(A): count entries per row, then scatter columns in increasing order
+    std::fill(rp, rp + n + 1, 0);
+    for (int64_t k = 0; k < nnz; ++k) rp[rowval[k] - base + 1]++;
+    for (int64_t i = 0; i < n; ++i) rp[i + 1] += rp[i];
+    std::vector<int> next(rp, rp + n);
+    if (perm) perm->resize((size_t)nnz);
+    for (int64_t j = 0; j < n; ++j)
+      for (int64_t k = colptr[j] - base; k < colptr[j + 1] - base; ++k) {
+        const int dst = next[rowval[k] - base]++;
+        ci[dst] = (int)j;
+        if (is_real) {
+          vr[dst] = nzval[2 * k];
+        } else {
+          vr[2 * dst] = nzval[2 * k];
+          vr[2 * dst + 1] = nzval[2 * k + 1];
+        }
+        if (perm) (*perm)[k] = dst;
+      }
+  }
+  int maxrow = 0;
+  for (int64_t i = 0; i < n; ++i) maxrow = std::max(maxrow, rp[i + 1] - rp[i]);
+  kpm_csr* A = new kpm_csr();
+  A->ctx = ctx;
+  A->n = n;
+  A->nnz = nnz;
+  A->is_real = is_real ? 1 : 0;
+  A->max_row_nnz = maxrow;
+  hipError_t e = hipMalloc(&A->rowptr, brp);
+  if (e == hipSuccess) e = hipMalloc(&A->colind, bci);
+  if (e == hipSuccess) e = hipMalloc(&A->vals, bv);
+  if (e != hipSuccess) {
+    if (A->rowptr) (void)hipFree(A->rowptr);
+    if (A->colind) (void)hipFree(A->colind);
+    if (A->vals) (void)hipFree(A->vals);
+    delete A;
+    return fail(KPM_ENOMEM, std::string("hipMalloc for CSR failed: ") + hipGetErrorString(e));
+  }
+  e = hipMemcpyAsync(A->rowptr, rp, brp, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(A->colind, ci, bci, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(A->vals, vr, bv, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // staging is reused by the next call
+  if (e != hipSuccess) {
+    (void)hipFree(A->rowptr);
+    (void)hipFree(A->colind);
+    (void)hipFree(A->vals);
+    delete A;
+    return fail(KPM_EHIP, std::string("CSR upload failed: ") + hipGetErrorString(e));
+  }
+  *out = A;
+  return KPM_OK;
+}
+
 int check_csc(int64_t n, const int64_t* colptr, const int64_t* rowval, int base) {
   if (n <= 0) return fail(KPM_EINVAL, "operator dimension must be positive");
   if (base != 0 && base != 1) return fail(KPM_EINVAL, "index_base must be 0 or 1");
@@ -189,6 +291,7 @@ int kpm_ctx_destroy(kpm_ctx* ctx) {
   free_buffer(ctx->slots_l);
   free_buffer(ctx->slots_r);
   for (auto& b : ctx->ring) free_buffer(b);
+  if (ctx->pinned.ptr) (void)hipHostFree(ctx->pinned.ptr);
   for (auto& ev : ctx->events)
     if (ev) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(ctx->stream);
@@ -224,26 +327,9 @@ int kpm_csr_from_csc(kpm_ctx* ctx, int64_t n, const int64_t* colptr, const int64
                      const double* nzval, int index_base, kpm_csr** out) {
   if (!ctx || !out) return fail(KPM_EINVAL, "null argument");
   KPM_TRY(check_csc(n, colptr, rowval, index_base));
-  const int64_t nnz = colptr[n] - index_base;
-  // CSR(A) = transpose of the CSC arrays: count entries per row, then scatter
-  // columns in increasing order so each row's column indices are sorted.
-  std::vector<int64_t> rp(n + 1, 0), ci(std::max<int64_t>(nnz, 1));
-  std::vector<double> v(2 * std::max<int64_t>(nnz, 1));
-  for (int64_t k = 0; k < nnz; ++k) rp[rowval[k] - index_base + 1]++;
-  for (int64_t i = 0; i < n; ++i) rp[i + 1] += rp[i];
-  std::vector<int64_t> next(rp.begin(), rp.end() - 1);
-  std::vector<int64_t> perm(nnz);
-  for (int64_t j = 0; j < n; ++j)
-    for (int64_t k = colptr[j] - index_base; k < colptr[j + 1] - index_base; ++k) {
-      const int64_t r = rowval[k] - index_base;
-      const int64_t dst = next[r]++;
-      ci[dst] = j;
-      v[2 * dst] = nzval[2 * k];
-      v[2 * dst + 1] = nzval[2 * k + 1];
-      perm[k] = dst;
-    }
   KPM_HIP_TRY(hipSetDevice(ctx->device));
-  KPM_TRY(upload_csr(ctx, n, rp, ci, v, out));
+  std::vector<int> perm;
+  KPM_TRY(csr_from_csc_staged(ctx, n, colptr, rowval, nzval, index_base, false, &perm, out));
   (*out)->csc_to_csr = std::move(perm);
   return KPM_OK;
 }
@@ -252,18 +338,8 @@ int kpm_csr_adjoint_from_csc(kpm_ctx* ctx, int64_t n, const int64_t* colptr, con
                              const double* nzval, int index_base, kpm_csr** out) {
   if (!ctx || !out) return fail(KPM_EINVAL, "null argument");
   KPM_TRY(check_csc(n, colptr, rowval, index_base));
-  const int64_t nnz = colptr[n] - index_base;
-  // row j of adjoint(A) is column j of A, conjugated
-  std::vector<int64_t> rp(n + 1), ci(std::max<int64_t>(nnz, 1));
-  std::vector<double> v(2 * std::max<int64_t>(nnz, 1));
-  for (int64_t j = 0; j <= n; ++j) rp[j] = colptr[j] - index_base;
-  for (int64_t k = 0; k < nnz; ++k) {
-    ci[k] = rowval[k] - index_base;
-    v[2 * k] = nzval[2 * k];
-    v[2 * k + 1] = -nzval[2 * k + 1];
-  }
   KPM_HIP_TRY(hipSetDevice(ctx->device));
-  return upload_csr(ctx, n, rp, ci, v, out);
+  return csr_from_csc_staged(ctx, n, colptr, rowval, nzval, index_base, true, nullptr, out);
 }
 
 int kpm_csr_from_csr(kpm_ctx* ctx, int64_t n, const int64_t* rowptr, const int64_t* colind,
