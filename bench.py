@@ -433,7 +433,7 @@ def bench_conductivity(args) -> int:
     rl = {}
     for gemm, code, peak in (("f64", 0, 78.6), ("f32", 1, 157.3)):
         ms = ctypes.c_float(0.0)
-        K._lib.check(ctx.lib.kpm_bench_contract(ctx.handle, NC, NC, Kc, 0, code, 3, ctypes.byref(ms)),
+        K._lib.check(ctx.lib.kpm_bench_contract(ctx.handle, NC, NC, Kc, 0, 0, code, 3, ctypes.byref(ms)),
                      "bench_contract")
         t = ms.value / 3 * 1e-3
         rl[gemm] = {"bound": "mfma", "achieved": round(flops / t / 1e12, 2), "peak": peak,

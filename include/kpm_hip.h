@@ -274,9 +274,11 @@ int kpm_bench_copy(kpm_ctx* ctx, kpm_block* dst, const kpm_block* src, int64_t r
 /* Measurement helper (no reference counterpart): total ms of `reps`
  * back-to-back kpm_2d contractions of nL x nR slots of K complex values
  * (K a multiple of 16; the _kpm_2d_blocked! GEMM, src/moment.jl:1055-1061),
- * after one untimed launch, on scratch slots the call allocates and frees. */
-int kpm_bench_contract(kpm_ctx* ctx, int64_t nL, int64_t nR, int64_t K, int vec_prec, int gemm_mode,
-                       int64_t reps, float* ms);
+ * after one untimed launch, on scratch slots the call allocates and frees.
+ * pitch: complex values between consecutive slots (>= K, a multiple of 8);
+ * 0 selects kpm_2d's own pitch, K plus 256 bytes. */
+int kpm_bench_contract(kpm_ctx* ctx, int64_t nL, int64_t nR, int64_t K, int64_t pitch, int vec_prec,
+                       int gemm_mode, int64_t reps, float* ms);
 /* HIP-event timer on the context stream: record into slot 0..15, read the
  * elapsed milliseconds between two recorded slots. */
 int kpm_event_record(kpm_ctx* ctx, int slot);

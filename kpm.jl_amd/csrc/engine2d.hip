@@ -50,9 +50,12 @@ static int run_2d(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, const kpm_c
   const int prec = p.prec;
   const int64_t NRp = padded_nr(NR, prec);
   const int64_t used = NH * NRp;                 // complex values per block
-  const int64_t Kp = (used + 15) / 16 * 16;      // slot stride in complex values
+  const int64_t Kp = (used + 15) / 16 * 16;      // contracted length in complex values
   const int64_t eb = elem_bytes(prec);
-  const int64_t stride = Kp * eb;
+  // Slot pitch: Kp plus 256 bytes, so the slots of one contraction tile do not
+  // sit a power of two apart (which maps them all to one L2 set and channel).
+  const int64_t pitch = Kp + 256 / eb;
+  const int64_t stride = pitch * eb;
   const int64_t KL = p.K_left, KR = p.K_right;
   KPM_TRY(ensure(ctx, ctx->slots_l, (size_t)(stride * KL)));
   KPM_TRY(ensure(ctx, ctx->slots_r, (size_t)(stride * KR)));
@@ -100,7 +103,8 @@ static int run_2d(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, const kpm_c
         KPM_TRY(spmm(ctx, Jb, &rr[ri], nullptr, &R[j - 1], 1.0, 0.0));
       }
       KPM_TRY(launch_contract(ctx, ctx->slots_l.ptr, (int)rep_size, ctx->slots_r.ptr, (int)n_count,
-                              Kp, prec, p.gemm_mode, scale, mu_dev, NC, n1 - 1, m1 - 1, p.parity));
+                              Kp, pitch, prec, p.gemm_mode, scale, mu_dev, NC, n1 - 1, m1 - 1,
+                              p.parity));
     }
     if (rep < reps) {  // chebyshev_iter_wrap(H, L, K_left)
       KPM_TRY(spmm(ctx, H, &L[KL - 1], &L[KL - 2], &L[0], 2.0, -1.0));

@@ -73,7 +73,8 @@ constexpr int kStep = 32;  // reals of K per stage (16 complex values per slot r
 template <typename VT, typename MT>
 __global__ void __launch_bounds__(256)
     k_contract(const VT* __restrict__ L, int nL, const VT* __restrict__ R, int nR, int64_t K2,
-               int64_t kchunk, double2* __restrict__ part, int ldn, int ldm) {
+               int64_t P2, int64_t kchunk, int tm, int ntiles, int splits,
+               double2* __restrict__ part, int ldn, int ldm) {
   using MF = Mfma<MT>;
   using acc_t = typename MF::acc_t;
   constexpr int PADE = 16 / (int)sizeof(MT);       // pad elements per LDS row
@@ -88,12 +89,26 @@ __global__ void __launch_bounds__(256)
   const int wave = tid >> 6;
   const int g = lane >> 4;
   const int i = lane & 15;
-  const int tile_n = blockIdx.y * 64, tile_m = blockIdx.x * 64;
+  // Workgroup b runs on XCD b % 8.  With splits a multiple of 8, XCD x takes
+  // the K-chunks z = x, x + 8, ... and, for each, all tiles of the table in
+  // consecutive dispatch slots, so the tiles resident together on one XCD
+  // read the same K-chunk of every slot and each slot segment is fetched
+  // into that XCD's L2 once rather than once per tile.
+  int tile, z;
+  if ((splits & 7) == 0) {
+    const int b = blockIdx.x, local = b >> 3;
+    tile = local % ntiles;
+    z = (local / ntiles) * 8 + (b & 7);
+  } else {
+    tile = blockIdx.x % ntiles;
+    z = blockIdx.x / ntiles;
+  }
+  const int tile_n = (tile / tm) * 64, tile_m = (tile % tm) * 64;
   const int n0 = (wave >> 1) * 32;
   const int m0 = (wave & 1) * 32;
-  const int64_t kbeg = (int64_t)blockIdx.z * kchunk;
+  const int64_t kbeg = (int64_t)z * kchunk;
   const int64_t kend = min(kbeg + kchunk, K2);
-  const int nstage = (int)((kend - kbeg) / kStep);
+  const int nstage = kend > kbeg ? (int)((kend - kbeg) / kStep) : 0;
 
   // this thread's global words: word w -> (operand, row, word-in-row)
   const VT* src[NW];
@@ -105,7 +120,7 @@ __global__ void __launch_bounds__(256)
     const int row = rem / WPR;
     const int wi = rem % WPR;
     const int grow = op == 0 ? min(tile_n + row, nR - 1) : min(tile_m + row, nL - 1);
-    src[q] = (op == 0 ? R : L) + (size_t)grow * K2 + kbeg + wi * EPW;
+    src[q] = (op == 0 ? R : L) + (size_t)grow * P2 + kbeg + wi * EPW;
   }
   auto gload = [&](int stage, float4* regs) {
 #pragma unroll
@@ -196,7 +211,7 @@ __global__ void __launch_bounds__(256)
     __syncthreads();
   }
   if (kPromote) flush();
-  double2* out = part + (size_t)blockIdx.z * ldn * ldm;
+  double2* out = part + (size_t)z * ldn * ldm;
 #pragma unroll
   for (int tn = 0; tn < 2; ++tn)
 #pragma unroll
@@ -238,40 +253,45 @@ __global__ void k_contract_reduce(const double2* __restrict__ part, int splits, 
 }
 
 int launch_contract(kpm_ctx* ctx, const void* Lslots, int nL, const void* Rslots, int nR,
-                    int64_t K, int prec, int gemm_mode, double scale, void* mu_dev, int64_t NC,
-                    int64_t n0, int64_t m0, int parity) {
-  const int64_t K2 = 2 * K;  // reals per slot (slot stride, a multiple of kStep)
+                    int64_t K, int64_t pitch, int prec, int gemm_mode, double scale, void* mu_dev,
+                    int64_t NC, int64_t n0, int64_t m0, int parity) {
+  const int64_t K2 = 2 * K;      // reals contracted per slot (a multiple of kStep)
+  const int64_t P2 = 2 * pitch;  // reals between consecutive slots
   if (K2 % kStep != 0) return fail(KPM_EINVAL, "contraction slot length must be a multiple of 16");
+  if (pitch < K || pitch % 8 != 0) return fail(KPM_EINVAL, "slot pitch must be >= K and a multiple of 8");
   const int tn = (nR + 63) / 64;
   const int tm = (nL + 63) / 64;
+  const int ntiles = tn * tm;
   const int64_t steps = K2 / kStep;
-  int64_t splits = std::max<int64_t>(1, (int64_t)(4 * std::max(ctx->num_cus, 1)) / (tn * tm));
+  int64_t splits = std::max<int64_t>(1, (int64_t)(4 * std::max(ctx->num_cus, 1)) / ntiles);
   splits = std::min<int64_t>(splits, std::max<int64_t>(1, steps / 16));
   splits = std::min<int64_t>(splits, 1024);
+  if (splits >= 8) splits = (splits + 7) / 8 * 8;  // whole K-chunk groups per XCD
   const int64_t steps_per = (steps + splits - 1) / splits;
-  const int64_t kchunk = steps_per * kStep;
-  splits = (K2 + kchunk - 1) / kchunk;
+  const int64_t kchunk = steps_per * kStep;  // trailing chunks may be empty (zero partials)
+  if ((int64_t)ntiles * splits > INT32_MAX) return fail(KPM_EINVAL, "contraction grid too large");
   const int ldn = nR, ldm = nL;
   KPM_TRY(ensure(ctx, ctx->gemm_part, (size_t)splits * ldn * ldm * sizeof(double2)));
-  dim3 grid(tm, tn, (unsigned)splits);
+  const dim3 grid((unsigned)(ntiles * splits));
+  const int sp = (int)splits;
   if (prec == KPM_VEC_F64) {
     if (gemm_mode == KPM_GEMM_F32)
       hipLaunchKernelGGL((k_contract<double, float>), grid, dim3(256), 0, ctx->stream,
-                         (const double*)Lslots, nL, (const double*)Rslots, nR, K2, kchunk,
-                         (double2*)ctx->gemm_part.ptr, ldn, ldm);
+                         (const double*)Lslots, nL, (const double*)Rslots, nR, K2, P2, kchunk, tm,
+                         ntiles, sp, (double2*)ctx->gemm_part.ptr, ldn, ldm);
     else
       hipLaunchKernelGGL((k_contract<double, double>), grid, dim3(256), 0, ctx->stream,
-                         (const double*)Lslots, nL, (const double*)Rslots, nR, K2, kchunk,
-                         (double2*)ctx->gemm_part.ptr, ldn, ldm);
+                         (const double*)Lslots, nL, (const double*)Rslots, nR, K2, P2, kchunk, tm,
+                         ntiles, sp, (double2*)ctx->gemm_part.ptr, ldn, ldm);
   } else {
     if (gemm_mode == KPM_GEMM_F32)
       hipLaunchKernelGGL((k_contract<float, float>), grid, dim3(256), 0, ctx->stream,
-                         (const float*)Lslots, nL, (const float*)Rslots, nR, K2, kchunk,
-                         (double2*)ctx->gemm_part.ptr, ldn, ldm);
+                         (const float*)Lslots, nL, (const float*)Rslots, nR, K2, P2, kchunk, tm,
+                         ntiles, sp, (double2*)ctx->gemm_part.ptr, ldn, ldm);
     else
       hipLaunchKernelGGL((k_contract<float, double>), grid, dim3(256), 0, ctx->stream,
-                         (const float*)Lslots, nL, (const float*)Rslots, nR, K2, kchunk,
-                         (double2*)ctx->gemm_part.ptr, ldn, ldm);
+                         (const float*)Lslots, nL, (const float*)Rslots, nR, K2, P2, kchunk, tm,
+                         ntiles, sp, (double2*)ctx->gemm_part.ptr, ldn, ldm);
   }
   KPM_HIP_TRY(hipGetLastError());
   const int64_t total = (int64_t)nR * nL;
@@ -288,30 +308,32 @@ using namespace kpm;
 
 extern "C" {
 
-int kpm_bench_contract(kpm_ctx* ctx, int64_t nL, int64_t nR, int64_t K, int vec_prec, int gemm_mode,
-                       int64_t reps, float* ms) {
+int kpm_bench_contract(kpm_ctx* ctx, int64_t nL, int64_t nR, int64_t K, int64_t pitch, int vec_prec,
+                       int gemm_mode, int64_t reps, float* ms) {
   if (!ctx || !ms || nL < 1 || nR < 1 || reps < 1 || K < 16 || K % 16 != 0)
     return fail(KPM_EINVAL, "bad arguments (K must be a positive multiple of 16)");
+  if (pitch == 0) pitch = K + 256 / elem_bytes(vec_prec == KPM_VEC_F32 ? KPM_VEC_F32 : KPM_VEC_F64);
+  if (pitch < K || pitch % 8 != 0) return fail(KPM_EINVAL, "pitch must be >= K and a multiple of 8");
   if (vec_prec != KPM_VEC_F64 && vec_prec != KPM_VEC_F32) return fail(KPM_EINVAL, "bad vec_prec");
   if (gemm_mode != KPM_GEMM_F64 && gemm_mode != KPM_GEMM_F32) return fail(KPM_EINVAL, "bad gemm_mode");
   KPM_HIP_TRY(hipSetDevice(ctx->device));
   const size_t eb = (size_t)elem_bytes(vec_prec);
   const int64_t NC = std::max(nL, nR);
   Buffer lb, rb, mb;
-  int rc = ensure(ctx, lb, (size_t)(nL * K) * eb);
-  if (rc == KPM_OK) rc = ensure(ctx, rb, (size_t)(nR * K) * eb);
+  int rc = ensure(ctx, lb, (size_t)(nL * pitch) * eb);
+  if (rc == KPM_OK) rc = ensure(ctx, rb, (size_t)(nR * pitch) * eb);
   if (rc == KPM_OK) rc = ensure(ctx, mb, (size_t)(NC * NC) * sizeof(double2));
   if (rc == KPM_OK && (hipMemsetAsync(lb.ptr, 0, lb.bytes, ctx->stream) != hipSuccess ||
                        hipMemsetAsync(rb.ptr, 0, rb.bytes, ctx->stream) != hipSuccess ||
                        hipMemsetAsync(mb.ptr, 0, mb.bytes, ctx->stream) != hipSuccess))
     rc = fail(KPM_EHIP, "workspace clear failed");
-  if (rc == KPM_OK) rc = launch_contract(ctx, lb.ptr, (int)nL, rb.ptr, (int)nR, K, vec_prec, gemm_mode,
-                                         1.0, mb.ptr, NC, 0, 0, KPM_PARITY_NONE);  // warm-up
+  if (rc == KPM_OK) rc = launch_contract(ctx, lb.ptr, (int)nL, rb.ptr, (int)nR, K, pitch, vec_prec,
+                                         gemm_mode, 1.0, mb.ptr, NC, 0, 0, KPM_PARITY_NONE);  // warm-up
   if (rc == KPM_OK && hipEventRecord(ctx->events[14], ctx->stream) != hipSuccess)
     rc = fail(KPM_EHIP, "event record failed");
   for (int64_t r = 0; r < reps && rc == KPM_OK; ++r)
-    rc = launch_contract(ctx, lb.ptr, (int)nL, rb.ptr, (int)nR, K, vec_prec, gemm_mode, 1.0, mb.ptr, NC,
-                         0, 0, KPM_PARITY_NONE);
+    rc = launch_contract(ctx, lb.ptr, (int)nL, rb.ptr, (int)nR, K, pitch, vec_prec, gemm_mode, 1.0,
+                         mb.ptr, NC, 0, 0, KPM_PARITY_NONE);
   if (rc == KPM_OK && (hipEventRecord(ctx->events[15], ctx->stream) != hipSuccess ||
                        hipEventSynchronize(ctx->events[15]) != hipSuccess ||
                        hipEventElapsedTime(ms, ctx->events[14], ctx->events[15]) != hipSuccess))

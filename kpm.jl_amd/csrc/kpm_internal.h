@@ -200,9 +200,10 @@ int launch_copy16(kpm_ctx* ctx, const void* src, void* dst, int64_t nwords);
 // table mu (NC x NC column-major device double2) at (n0 + n, m0 + m) with the
 // parity mask (0 none, 1 keep odd n+m, 2 keep even; 1-based n, m as in the
 // reference).
+// K: complex values contracted per slot; pitch: complex values between slots
 int launch_contract(kpm_ctx* ctx, const void* Lslots, int nL, const void* Rslots, int nR,
-                    int64_t K, int prec, int gemm_mode, double scale, void* mu_dev, int64_t NC,
-                    int64_t n0, int64_t m0, int parity);
+                    int64_t K, int64_t pitch, int prec, int gemm_mode, double scale, void* mu_dev,
+                    int64_t NC, int64_t n0, int64_t m0, int parity);
 
 // tuner table (tune.hip): best StepConfig for a shape, or defaults; record one
 StepConfig tuned_config(kpm_ctx* ctx, const kpm_csr* A, int64_t NR, int prec, int mode);

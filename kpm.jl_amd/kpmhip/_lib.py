@@ -102,7 +102,7 @@ _SIGS = {
                                      C.POINTER(C.c_float)]),
     "kpm_bench_step": (C.c_int, [_p, _p, _p, _p, _i64, C.POINTER(C.c_float)]),
     "kpm_bench_copy": (C.c_int, [_p, _p, _p, _i64, C.POINTER(C.c_float)]),
-    "kpm_bench_contract": (C.c_int, [_p, _i64, _i64, _i64, C.c_int, C.c_int, _i64,
+    "kpm_bench_contract": (C.c_int, [_p, _i64, _i64, _i64, _i64, C.c_int, C.c_int, _i64,
                                      C.POINTER(C.c_float)]),
     "kpm_event_record": (C.c_int, [_p, C.c_int]),
     "kpm_event_elapsed": (C.c_int, [_p, C.c_int, C.c_int, C.POINTER(C.c_float)]),
