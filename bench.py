@@ -47,13 +47,15 @@ def main() -> int:
     ap.add_argument("--skip-fp64", action="store_true", help="omit the FP64 comparison leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--kernel-launches", type=int, default=100)
-    ap.add_argument("--workload", choices=["graphene", "ensemble", "conductivity", "core", "adjoint"],
+    ap.add_argument("--workload", choices=["graphene", "ensemble", "conductivity", "core", "adjoint",
+                                           "cpge"],
                     default="graphene",
                     help="graphene: the headline single-lattice DOS; ensemble: batched disorder "
                          "realisations (cubic Anderson, one fused launch per step); "
                          "conductivity: Haldane kpm_2d table; core: the reference's own "
                          "benchmark suite (benchmarks/core_bench.jl) beside its published times; "
-                         "adjoint: DOS-loss gradient on a 512^2 disordered square lattice")
+                         "adjoint: DOS-loss gradient on a 512^2 disordered square lattice; "
+                         "cpge: three-current response from a Haldane kpm_3d table")
     ap.add_argument("--nreal", type=int, default=256)
     args = ap.parse_args()
 
@@ -70,6 +72,8 @@ def main() -> int:
         return bench_core(args)
     if args.workload == "adjoint":
         return bench_adjoint(args)
+    if args.workload == "cpge":
+        return bench_cpge(args)
 
     import numpy as np
 
@@ -451,6 +455,108 @@ def bench_conductivity(args) -> int:
            "roofline": rl["f64"], "roofline_fp32_gemm": rl["f32"],
            "fp64_gemm": out["f64"], "fp32_gemm": out["f32"],
            "sigma_xy_fp64_minus_fp32": out["f64"]["sigma_xy"] - out["f32"]["sigma_xy"]}
+    print(json.dumps(res))
+    return 0
+
+
+def bench_cpge(args) -> int:
+    """Three-current (CPGE) response of the Haldane model (honeycomb 64 x 64
+    cells, NH = 8192, t1 = 1, t2 = 0.1 e^{i pi/2}, M = 0, periodic).
+
+    The kpm_3d table mu[n,m,p] = Tr[Jx T_m Jy T_n Jy T_p]/D is computed on the
+    device at NC = 128, NR = 8. Then cpge runs with omega1 = 0.2,
+    omega2 = 0.1, E_f = 0, beta = inf, lambda = 0.01 (rescaled units).
+
+    One step is one cpge call on the host table: kernel weighting, table
+    upload, and the adaptive quadrature, with every refinement round's nodes
+    contracted in one FP64-MFMA launch. The roofline object is the node
+    contraction (k_trilinear) at the mean batch of those rounds, timed with
+    HIP events over back-to-back launches. The CPU baseline times the
+    oracle's numpy restatement of the same contraction on a bounded sample
+    of nodes and scales it to the call's node count."""
+    import ctypes
+
+    import numpy as np
+
+    import kpmhip as K
+    from kpmhip import models
+    from kpmhip.cpge import Table3, cpge_node_coefficients
+    from kpmhip.rng import Xoshiro256pp
+
+    if not K.activate(0):
+        raise SystemExit("no HIP device visible")
+    ctx = K.get_context()
+    Lc, NC, NR = 64, 128, 8
+    H, Jx, Jy, area = models.haldane_model(Lc, Lc, t=1.0, t2=0.1)
+    NH = H.shape[0]
+    a = 3.8
+    Hn = (H / a).tocsc()
+    psi = K.random_phase_vectors(Xoshiro256pp(42), NH, NR)
+    t0 = time.perf_counter()
+    mu3 = K.kpm_3d(Hn, Jx, Jy, Jy, NC, NR, psi_in=psi)
+    t3d = time.perf_counter() - t0
+    print(f"[bench] kpm_3d NC={NC} NR={NR} NH={NH}: {t3d:.2f} s", file=sys.stderr, flush=True)
+    w1, w2, lam = 0.2, 0.1, 0.01
+    counts = []
+    orig = Table3.forms
+
+    def counting(self, A, B, Cc):
+        counts.append(A.shape[0])
+        return orig(self, A, B, Cc)
+
+    Table3.forms = counting
+    try:
+        for _ in range(args.warmup):
+            K.cpge(mu3, NC, w1, w2, E_f=0.0, lambda_=lam)
+        counts.clear()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            val = K.cpge(mu3, NC, w1, w2, E_f=0.0, lambda_=lam)
+        wall = (time.perf_counter() - t0) / args.steps
+    finally:
+        Table3.forms = orig
+    forms = sum(counts) // args.steps
+    launches = len(counts) // args.steps
+    nodes = forms // 3
+    Qm = max(3, int(round(forms / max(launches, 1))))
+    print(f"[bench] cpge: {wall * 1e3:.1f} ms per call, {nodes} nodes in {launches} launches "
+          f"(mean batch {Qm} forms), value {val}", file=sys.stderr, flush=True)
+    mt = K.mu3D_apply_kernel_and_h(mu3, NC, K.JacksonKernel)
+    reps = 20
+    with Table3(mt, NC) as tb:
+        ms = ctypes.c_float(0.0)
+        K._lib.check(ctx.lib.kpm_bench_trilinear(tb.handle, Qm, reps, ctypes.byref(ms)), "bench_trilinear")
+    tk = ms.value / reps * 1e-3
+    flops = 8.0 * Qm * NC ** 3
+    roof = {"bound": "mfma", "achieved": round(flops / tk / 1e12, 2), "peak": 78.6, "unit": "TFLOP/s",
+            "frac": round(flops / tk / 1e12 / 78.6, 4), "traffic": None,
+            "kernel": "k_trilinear (+ k_trilinear_reduce)", "forms_per_launch": Qm,
+            "avg_launch_ms": round(tk * 1e3, 4)}
+    print(f"[bench] node contraction Q={Qm}: {tk * 1e3:.3f} ms, {flops / tk / 1e12:.1f} TF/s",
+          file=sys.stderr, flush=True)
+    cpu = None
+    if not args.skip_cpu:
+        import oracle as O
+        from threadpoolctl import threadpool_info
+        sample = 16
+        th = np.linspace(0.3, 2.8, sample)
+        A_, B_, C_ = cpge_node_coefficients(th, NC, w1, w2, lam, 0.0)
+        t0 = time.perf_counter()
+        O.cpge_forms(mt, A_, B_, C_)
+        tc = (time.perf_counter() - t0) / sample
+        threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
+        cpu = {"value": round(tc * nodes * 1e3, 1), "unit": "ms per cpge call (node contractions only)",
+               "cores": threads, "kind": "port",
+               "sample": f"{sample} nodes of the oracle's numpy _cpge_contract! restatement "
+                         f"({tc * 1e3:.2f} ms/node), scaled to {nodes} nodes"}
+    res = {"metric": "cpge call time (Haldane kpm_3d table)", "value": round(wall * 1e3, 2), "unit": "ms",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3, 2),
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+           "data": "synthetic",
+           "config": {"workload": f"Haldane {Lc}x{Lc} cpge from kpm_3d", "NH": NH, "NC": NC, "NR": NR,
+                      "omega1": w1, "omega2": w2, "lambda": lam, "nodes": nodes, "launches": launches},
+           "kpm_3d_s": round(t3d, 3), "nodes_per_s": round(nodes / wall, 1), "cpge": [val.real, val.imag],
+           "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(res))
     return 0
 

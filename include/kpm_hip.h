@@ -210,6 +210,27 @@ int kpm_cheb_lin_trans(kpm_ctx* ctx, int64_t Nx, const double* x, int64_t Nn, co
 int kpm_kubo_bastin_nodes(kpm_ctx* ctx, int64_t NC, const double* mu_tilde, int64_t K,
                           const double* x, const double* phi, double* acc);
 
+/* A kernel-weighted kpm_3d table resident on the device for the node
+ * contractions of cpge / d_cpge (src/applications/cpge.jl:66-120, where the
+ * reference holds mu_matrix = reshape(mu_tilde, NC, NC^2) for every node).
+ * mu_tilde: NC^3 complex, n fastest, then m, then p (Julia column-major
+ * mu[n,m,p]); copied during the call.  Destroy a table before its context. */
+typedef struct kpm_table3 kpm_table3;
+int kpm_table3_create(kpm_ctx* ctx, int64_t NC, const double* mu_tilde, kpm_table3** out);
+int kpm_table3_destroy(kpm_table3* t);
+/* out[q] = sum_{n,m,p} mu_tilde[n,m,p] a[q,n] b[q,m] c[q,p] for q < Q: the
+ * _cpge_contract! of src/applications/cpge.jl:59-64 (collapsed = mu^T slot1,
+ * right = reshape(collapsed) slot3, sum(slot2 .* right); plain transposes)
+ * with a = slot1, b = slot2, c = slot3.  a, b, c: Q x NC complex, row q
+ * contiguous; out: Q complex.  Q <= 2^20. */
+int kpm_trilinear_forms(kpm_table3* t, int64_t Q, const double* a, const double* b, const double* c,
+                        double* out);
+/* Measurement helper (no reference counterpart): total ms of `reps`
+ * kpm_trilinear_forms launches (contraction + fixed-order reduce, no
+ * transfers) of Q forms with device-generated unit-modulus coefficients,
+ * after one untimed launch. */
+int kpm_bench_trilinear(kpm_table3* t, int64_t Q, int64_t reps, float* ms);
+
 /* ---- reverse-mode gradient ---------------------------------------------------
  * No counterpart in the reference.  For the equal-vector moments
  * mu_n = (1/NR) sum_r <psi_r|T_n(H)|psi_r> (n < NC; the moments of

@@ -1,0 +1,298 @@
+// cpge.hip -- trilinear forms of the kpm_3d moment table on the MFMA cores.
+//
+// The CPGE reconstruction evaluates, at every quadrature node theta, three
+// trilinear forms of the kernel-weighted table mu~[n,m,p] (NC^3 complex, n
+// fastest, then m, then p):
+//   F = sum_{n,m,p} mu~[n,m,p] a[n] b[m] c[p]
+// (_cpge_contract!, src/applications/cpge.jl:59-64: collapsed = mu~^T a,
+// right = reshape(collapsed) c, F = sum(b .* right); plain transposes, no
+// conjugation; the node function that builds a, b, c is :66-120).
+//
+// A call evaluates Q forms (three per node, a batch of nodes) in one pass
+// over the table.  With columns j = m + NC p of the NC x NC^2 matrix mu~,
+//   X[q, j] = sum_n a_q[n] mu~[n, j]
+// is a (Q x NC) x (NC x NC^2) complex GEMM, run as two real GEMMs over the
+// interleaved reals k' = 2n + {0, 1} of a column:
+//   Re X = sum_k' a'[q, k'] mu'[k', j]     a' = (ar, -ai) per complex
+//   Im X = sum_k' a''[q, k'] mu'[k', j]    a'' = (ai, ar)
+// on v_mfma_f64_16x16x4f64 (a' and a'' are formed in registers from the
+// pair partner).  The epilogue multiplies X[q, j] by b_q[m] c_q[p] and sums
+// over the workgroup's columns in a fixed order (16-lane butterfly, then the
+// four waves); a second kernel adds the per-workgroup partials in column
+// order.  Results do not depend on timing or the launch shape.
+//
+// The table stays resident on the device (kpm_table3) across the many
+// calls of one adaptive quadrature.
+#include "kpm_internal.h"
+
+#include <algorithm>
+
+namespace kpm {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int kTQ = 32;  // forms per workgroup: two 16-row MFMA blocks
+constexpr int kTJ = 64;  // table columns per workgroup: 16 per wave
+constexpr int kStage = 32;  // reals of a column per stage (8 per lane group)
+
+// Each lane (g = lane / 16, i = lane % 16) owns column j0 + i of its wave's
+// 16 columns and, in each 32-real stage, reals k0 + 8g .. k0 + 8g + 7 of that
+// column and of its two form rows q0 + i, q0 + 16 + i; MFMA t of the stage
+// contracts element t of every lane group (a fixed permutation of k', the
+// same for both operands).
+__global__ void __launch_bounds__(256)
+    k_trilinear(const double* __restrict__ mu, int NC, const double* __restrict__ A,
+                const double2* __restrict__ B, const double2* __restrict__ C, int Q, int ntx,
+                int nqy, double2* __restrict__ part) {
+  __shared__ double2 red[4][kTQ];
+  // Workgroup b runs on XCD b % 8.  XCD x takes the column tiles x, x + 8, ...
+  // and, for each, all form blocks in consecutive dispatch slots, so a tile
+  // is fetched into that XCD's L2 once and reused by every form block; the
+  // coefficient rows (Q x NC) stay L2-resident.  The grid holds ntx rounded
+  // up to a multiple of 8 tiles; the padding tiles exit at once.
+  const int b = blockIdx.x, local = b >> 3;
+  const int tile = (local / nqy) * 8 + (b & 7), qblk = local % nqy;
+  if (tile >= ntx) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, i = lane & 15;
+  const int64_t NC2 = (int64_t)NC * NC;
+  const int64_t j = (int64_t)tile * kTJ + wave * 16 + i;
+  const bool jok = j < NC2;
+  const int64_t jc = jok ? j : NC2 - 1;
+  const int q0 = qblk * kTQ;
+  const int K2 = 2 * NC;
+  const double* mcol = mu + (int64_t)K2 * jc;
+  const double* arow[2];
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb) {
+    const int q = q0 + rb * 16 + i;
+    arow[rb] = A + (int64_t)K2 * (q < Q ? q : Q - 1);
+  }
+  // reals beyond 2NC (the last stage) and rows beyond Q read as zero; the
+  // clamped addresses stay inside the column / row
+  auto load = [&](int st, double* bb, double (*aa)[8]) {
+    const int kb = st * kStage + 8 * g;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int k = kb + 2 * t;
+      const bool ok = k < K2;
+      const int kc = ok ? k : 0;
+      const double2 v = *reinterpret_cast<const double2*>(mcol + kc);
+      bb[2 * t] = ok ? v.x : 0.0;
+      bb[2 * t + 1] = ok ? v.y : 0.0;
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) {
+        const double2 w = *reinterpret_cast<const double2*>(arow[rb] + kc);
+        aa[rb][2 * t] = ok ? w.x : 0.0;
+        aa[rb][2 * t + 1] = ok ? w.y : 0.0;
+      }
+    }
+  };
+
+  d4 are[2], aim[2];
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb) are[rb] = aim[rb] = d4{0.0, 0.0, 0.0, 0.0};
+  const int nst = (K2 + kStage - 1) / kStage;
+  double bc[8], ac[2][8];
+  load(0, bc, ac);
+  for (int st = 0; st < nst; ++st) {
+    double bn[8], an[2][8];
+    load(min(st + 1, nst - 1), bn, an);  // next stage in flight during the MFMAs
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) {
+        const double ar = (t & 1) ? -ac[rb][t] : ac[rb][t];
+        const double ai = (t & 1) ? ac[rb][t - 1] : ac[rb][t + 1];
+        are[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bc[t], are[rb], 0, 0, 0);
+        aim[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bc[t], aim[rb], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      bc[t] = bn[t];
+      ac[0][t] = an[0][t];
+      ac[1][t] = an[1][t];
+    }
+  }
+
+  // epilogue: accumulator element v of lane (g, i) is X[row g + 4v][column i]
+  const int m = (int)(jc % NC), p = (int)(jc / NC);
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = rb * 16 + g + 4 * v;
+      const int q = q0 + r;
+      double sx = 0.0, sy = 0.0;
+      if (jok && q < Q) {
+        const double2 bq = B[(int64_t)q * NC + m], cq = C[(int64_t)q * NC + p];
+        const double wr = bq.x * cq.x - bq.y * cq.y, wi = bq.x * cq.y + bq.y * cq.x;
+        const double xr = are[rb][v], xi = aim[rb][v];
+        sx = xr * wr - xi * wi;
+        sy = xr * wi + xi * wr;
+      }
+#pragma unroll
+      for (int off = 8; off >= 1; off >>= 1) {
+        sx += __shfl_xor(sx, off, 16);
+        sy += __shfl_xor(sy, off, 16);
+      }
+      if (i == 0) red[wave][r] = make_double2(sx, sy);
+    }
+  __syncthreads();
+  if (threadIdx.x < kTQ) {
+    const int q = q0 + threadIdx.x;
+    if (q < Q) {
+      double sx = 0.0, sy = 0.0;
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        sx += red[w][threadIdx.x].x;
+        sy += red[w][threadIdx.x].y;
+      }
+      part[(int64_t)tile * Q + q] = make_double2(sx, sy);
+    }
+  }
+}
+
+// out[q] = sum over column tiles, in tile order
+__global__ void k_trilinear_reduce(const double2* __restrict__ part, int ntiles, int Q,
+                                   double2* __restrict__ out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= Q) return;
+  double sx = 0.0, sy = 0.0;
+  for (int t = 0; t < ntiles; ++t) {
+    const double2 v = part[(int64_t)t * Q + q];
+    sx += v.x;
+    sy += v.y;
+  }
+  out[q] = make_double2(sx, sy);
+}
+
+// deterministic unit-modulus coefficient rows for the timing helper
+__global__ void k_fill_phase(double2* __restrict__ v, int64_t n, double step) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s, c;
+  sincos(step * (double)(i % 4099), &s, &c);
+  v[i] = make_double2(c, s);
+}
+
+}  // namespace kpm
+
+struct kpm_table3 {
+  kpm_ctx* ctx = nullptr;
+  int64_t NC = 0;
+  double2* mu = nullptr;  // NC^3, n fastest
+  kpm::Buffer work;       // coefficient rows, partials and results of a call
+};
+
+using namespace kpm;
+
+extern "C" {
+
+int kpm_table3_create(kpm_ctx* ctx, int64_t NC, const double* mu_tilde, kpm_table3** out) {
+  if (!ctx || !mu_tilde || !out) return fail(KPM_EINVAL, "null argument");
+  *out = nullptr;
+  if (NC < 1 || NC > 2048) return fail(KPM_EINVAL, "NC must lie in [1, 2048]");
+  KPM_HIP_TRY(hipSetDevice(ctx->device));
+  const size_t bytes = (size_t)(NC * NC * NC) * sizeof(double2);
+  kpm_table3* t = new kpm_table3;
+  t->ctx = ctx;
+  t->NC = NC;
+  if (hipMalloc(&t->mu, bytes) != hipSuccess) {
+    delete t;
+    (void)hipGetLastError();
+    return fail(KPM_ENOMEM, "device allocation of the moment table failed");
+  }
+  if (hipMemcpyAsync(t->mu, mu_tilde, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    (void)hipFree(t->mu);
+    delete t;
+    return fail(KPM_EHIP, "moment table upload failed");
+  }
+  *out = t;
+  return KPM_OK;
+}
+
+int kpm_table3_destroy(kpm_table3* t) {
+  if (!t) return KPM_OK;
+  (void)hipSetDevice(t->ctx->device);
+  (void)hipStreamSynchronize(t->ctx->stream);
+  if (t->mu) (void)hipFree(t->mu);
+  if (t->work.ptr) (void)hipFree(t->work.ptr);
+  delete t;
+  return KPM_OK;
+}
+
+int kpm_trilinear_forms(kpm_table3* t, int64_t Q, const double* a, const double* b, const double* c,
+                        double* out) {
+  if (!t || (Q > 0 && (!a || !b || !c || !out))) return fail(KPM_EINVAL, "null argument");
+  if (Q < 0 || Q > (1 << 20)) return fail(KPM_EINVAL, "Q must lie in [0, 2^20]");
+  if (Q == 0) return KPM_OK;
+  kpm_ctx* ctx = t->ctx;
+  const int64_t NC = t->NC, NC2 = NC * NC;
+  KPM_HIP_TRY(hipSetDevice(ctx->device));
+  const int64_t ntx = (NC2 + kTJ - 1) / kTJ, nqy = (Q + kTQ - 1) / kTQ;
+  if ((ntx + 7) / 8 * 8 * nqy > INT32_MAX) return fail(KPM_EINVAL, "too many forms in one call");
+  const size_t bcoef = (size_t)(Q * NC) * sizeof(double2), bpart = (size_t)(ntx * Q) * sizeof(double2),
+               bout = (size_t)Q * sizeof(double2);
+  KPM_TRY(ensure(ctx, t->work, 3 * bcoef + bpart + bout));
+  char* base = (char*)t->work.ptr;
+  double* da = (double*)base;
+  double2* db = (double2*)(base + bcoef);
+  double2* dc = (double2*)(base + 2 * bcoef);
+  double2* dpart = (double2*)(base + 3 * bcoef);
+  double2* dout = (double2*)(base + 3 * bcoef + bpart);
+  KPM_HIP_TRY(hipMemcpyAsync(da, a, bcoef, hipMemcpyHostToDevice, ctx->stream));
+  KPM_HIP_TRY(hipMemcpyAsync(db, b, bcoef, hipMemcpyHostToDevice, ctx->stream));
+  KPM_HIP_TRY(hipMemcpyAsync(dc, c, bcoef, hipMemcpyHostToDevice, ctx->stream));
+  const unsigned grid = (unsigned)((ntx + 7) / 8 * 8 * nqy);
+  hipLaunchKernelGGL(k_trilinear, dim3(grid), dim3(256), 0, ctx->stream, (const double*)t->mu, (int)NC,
+                     (const double*)da, (const double2*)db, (const double2*)dc, (int)Q, (int)ntx,
+                     (int)nqy, dpart);
+  KPM_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_trilinear_reduce, dim3((unsigned)((Q + 255) / 256)), dim3(256), 0, ctx->stream,
+                     (const double2*)dpart, (int)ntx, (int)Q, dout);
+  KPM_HIP_TRY(hipGetLastError());
+  KPM_HIP_TRY(hipMemcpyAsync(out, dout, bout, hipMemcpyDeviceToHost, ctx->stream));
+  KPM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return KPM_OK;
+}
+
+int kpm_bench_trilinear(kpm_table3* t, int64_t Q, int64_t reps, float* ms) {
+  if (!t || !ms || Q < 1 || Q > (1 << 20) || reps < 1) return fail(KPM_EINVAL, "bad arguments");
+  kpm_ctx* ctx = t->ctx;
+  const int64_t NC = t->NC, NC2 = NC * NC;
+  KPM_HIP_TRY(hipSetDevice(ctx->device));
+  const int64_t ntx = (NC2 + kTJ - 1) / kTJ, nqy = (Q + kTQ - 1) / kTQ;
+  if ((ntx + 7) / 8 * 8 * nqy > INT32_MAX) return fail(KPM_EINVAL, "too many forms in one call");
+  const size_t bcoef = (size_t)(Q * NC) * sizeof(double2), bpart = (size_t)(ntx * Q) * sizeof(double2),
+               bout = (size_t)Q * sizeof(double2);
+  KPM_TRY(ensure(ctx, t->work, 3 * bcoef + bpart + bout));
+  char* base = (char*)t->work.ptr;
+  double2* dc = (double2*)base;
+  double2* dpart = (double2*)(base + 3 * bcoef);
+  double2* dout = (double2*)(base + 3 * bcoef + bpart);
+  const int64_t n = 3 * Q * NC;
+  hipLaunchKernelGGL(k_fill_phase, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, dc, n,
+                     0.618);
+  KPM_HIP_TRY(hipGetLastError());
+  auto launch = [&]() {
+    hipLaunchKernelGGL(k_trilinear, dim3((unsigned)((ntx + 7) / 8 * 8 * nqy)), dim3(256), 0,
+                       ctx->stream, (const double*)t->mu, (int)NC, (const double*)dc,
+                       (const double2*)(dc + Q * NC), (const double2*)(dc + 2 * Q * NC), (int)Q,
+                       (int)ntx, (int)nqy, dpart);
+    hipLaunchKernelGGL(k_trilinear_reduce, dim3((unsigned)((Q + 255) / 256)), dim3(256), 0,
+                       ctx->stream, (const double2*)dpart, (int)ntx, (int)Q, dout);
+  };
+  launch();  // warm-up
+  KPM_HIP_TRY(hipEventRecord(ctx->events[14], ctx->stream));
+  for (int64_t r = 0; r < reps; ++r) launch();
+  KPM_HIP_TRY(hipEventRecord(ctx->events[15], ctx->stream));
+  KPM_HIP_TRY(hipGetLastError());
+  KPM_HIP_TRY(hipEventSynchronize(ctx->events[15]));
+  KPM_HIP_TRY(hipEventElapsedTime(ms, ctx->events[14], ctx->events[15]));
+  return KPM_OK;
+}
+
+}  // extern "C"

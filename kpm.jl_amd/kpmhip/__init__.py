@@ -9,7 +9,8 @@ interface: the moment engine (src/moment.jl), its primitives
 end (src/frontend.jl), and the matrix-function action with its consumers
 (src/utils/chebyshev_action.jl: evolve, fermi_projector, chern_marker).
 """
-from ._lib import ArgumentError, KPMHipError, KPMHipMissing, KPMUnstableError, load_library
+from ._lib import (ArgumentError, DomainError, KPMHipError, KPMHipMissing, KPMUnstableError,
+                   QuadratureError, load_library)
 from .action import (chebyshev_action, chebyshev_action_, chern_marker, chern_marker_average,
                      chern_marker_region, evolution_coefficients, evolution_order,
                      evolution_tail, evolve, evolve_, fermi_coefficients, fermi_projector)
@@ -21,6 +22,7 @@ from .applications import dos as dos_raw
 from .applications import dos0 as dos0_raw
 from .applications import kubo_bastin_cond as kubo_bastin_cond_raw
 from .batch import BatchOperator, kpm_batch_1d
+from .cpge import Table3, cpge, d_cpge
 from .device import (ACTIVE_DEVICE, CPUDevice, DeviceBlock, DeviceCSR, HIPDevice, activate,
                      device_free_memory, get_context, maybe_on_device_zeros, maybe_to_host,
                      to_device, whichcore)

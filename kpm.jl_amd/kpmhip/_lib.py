@@ -41,6 +41,14 @@ class KPMUnstableError(RuntimeError):
     """Julia's ErrorException from the recurrence stability guard."""
 
 
+class DomainError(ValueError):
+    """Julia's DomainError (an argument outside a function's domain)."""
+
+
+class QuadratureError(RuntimeError):
+    """Julia's ErrorException from the spectral quadrature error contract."""
+
+
 class KPMHipMissing(RuntimeError):
     """libkpm_hip.so is not built or no GPU is visible."""
 
@@ -89,6 +97,10 @@ _SIGS = {
     "kpm_spectral_edges": (C.c_int, [_p, _p, _i64, C.c_double, C.c_uint64, _pd, _pd, _pi64]),
     "kpm_cheb_lin_trans": (C.c_int, [_p, _i64, _pd, _i64, _pi64, _pd, _pd]),
     "kpm_kubo_bastin_nodes": (C.c_int, [_p, _i64, _pd, _i64, _pd, _pd, _pd]),
+    "kpm_table3_create": (C.c_int, [_p, _i64, _pd, C.POINTER(C.c_void_p)]),
+    "kpm_table3_destroy": (C.c_int, [_p]),
+    "kpm_trilinear_forms": (C.c_int, [_p, _i64, _pd, _pd, _pd, _pd]),
+    "kpm_bench_trilinear": (C.c_int, [_p, _i64, _i64, C.POINTER(C.c_float)]),
     "kpm_bond_gradient": (C.c_int, [_i64, _pi64, _pi64, _pd, _pd, C.c_double, _pi64, _pi64, _pd,
                                     _pi64]),
     "kpm_1d_adjoint": (C.c_int, [_p, _p, _i64, _i64, _i64, _pd, _pd, _pd, C.c_int, _i64]),

@@ -13,6 +13,8 @@ CSC ComplexF64, 0-based).  Each restates a model builder of the reference:
                        coordinates and cell area, for local Chern markers)
 - ``ring_model`` / ``flux_ring_model``  test/ed_reference.jl:530-570
 - ``square_disordered`` the periodic square lattice with on-site eps_i ~ U[-1, 1]
+- ``helix_cluster``    test/cpge_test.jl:10-35 (six-site chiral helix with three
+                       bond-current operators, for the three-current response)
                        and Hermitian-tied bond hoppings t_ij ~ 1 + U[-0.1, 0.1]
                        (the trainable-parameter lattice of the gradient workload)
 
@@ -267,3 +269,24 @@ def square_disordered(L: int, *, seed: int = 42, eps: float = 1.0, dt: float = 0
     cols = np.concatenate([right, site, up, site, site])
     vals = np.concatenate([-tr, -tr, -tu, -tu, e])
     return _csc(rows, cols, vals, N)
+
+
+def helix_cluster(*, t: float = 1.0, t2: float = 0.45, phi: float = 0.7, w: float = 0.3):
+    """Six sites on a helix (angle 2 pi j/3, height 0.4 j), hopping -t,
+    second-neighbour t2 e^{i phi}, staggered on-site w (-1)^j (1-based j);
+    J_alpha[i, j] = H[i, j] (r_i - r_j)_alpha (test/cpge_test.jl:10-35).
+    Returns (H, [Jx, Jy, Jz], positions)."""
+    N = 6
+    pos = np.array([[math.cos(2 * math.pi * j / 3), math.sin(2 * math.pi * j / 3), 0.4 * j]
+                    for j in range(N)])
+    H = np.zeros((N, N), dtype=np.complex128)
+    for j in range(N - 1):
+        H[j + 1, j] += -t
+        H[j, j + 1] += -t
+    for j in range(N - 2):
+        H[j + 2, j] += t2 * np.exp(1j * phi)
+        H[j, j + 2] += t2 * np.exp(-1j * phi)
+    for j in range(N):
+        H[j, j] = w * (-1) ** (j + 1)
+    J = [sp.csc_matrix(H * (pos[:, a][:, None] - pos[:, a][None, :])) for a in range(3)]
+    return sp.csc_matrix(H), J, pos

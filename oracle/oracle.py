@@ -483,3 +483,200 @@ def kubo_gamma_direct(mt, x):
                  (x + 1j * n * s) * np.exp(-1j * n * th) * np.cos(m * th))
             tot += mt[n, m] * g
     return tot
+
+
+# ---- three-current (CPGE) response (src/applications/cpge.jl) ---------------
+# cpge_forms restates the node contraction _cpge_contract!
+# (src/applications/cpge.jl:59-64).  The remaining helpers restate the
+# reference's independent test references (test/cpge_test.jl:37-316): the
+# closed-form resolvent coefficients, the eigenbasis integrand and an
+# adaptive integral over theta that shares no code with the product's
+# quadrature (scipy's QUADPACK instead of the product's Gauss-Kronrod
+# restatement).
+
+def cpge_forms(mu_tilde, a, b, c):
+    """For each row q: collapsed = mu~^T a_q over the NC x NC^2 view,
+    right = reshape(collapsed, NC, NC) c_q, F_q = sum(b_q .* right)."""
+    mu_tilde = np.asarray(mu_tilde)
+    NC = mu_tilde.shape[0]
+    M = np.asfortranarray(mu_tilde).reshape(NC, NC * NC, order="F")
+    out = np.empty(a.shape[0], dtype=np.complex128)
+    for q in range(a.shape[0]):
+        collapsed = M.T @ a[q]
+        right = collapsed.reshape(NC, NC, order="F") @ c[q]
+        out[q] = np.sum(b[q] * right)
+    return out
+
+
+def _cheb_vec(x, NC):
+    T = np.zeros(NC, dtype=np.complex128 if np.iscomplexobj(x) else np.float64)
+    if NC == 0:
+        return T
+    T[0] = 1
+    if NC > 1:
+        T[1] = x
+    for n in range(2, NC):
+        T[n] = 2 * x * T[n - 1] - T[n - 2]
+    return T
+
+
+def cpge_ref_delta(x, NC, times_sin=False):
+    """T_n(x) / (pi sqrt(1 - x^2)) (test/cpge_test.jl:67); with times_sin,
+    that value times sin(theta) = sqrt(1 - x^2), i.e. T_n(x) / pi."""
+    if times_sin:
+        return _cheb_vec(x, NC) / np.pi
+    return _cheb_vec(x, NC) / (np.pi * np.sqrt(1 - x * x))
+
+
+def cpge_ref_green(z, NC, lam, branch, delta=0.0):
+    """Closed-form resolvent coefficients (test/cpge_test.jl:69-96)."""
+    n = np.arange(NC)
+    if delta > 0 and 1 - delta < abs(z) < 1 + delta:
+        return np.zeros(NC, dtype=np.complex128)
+    if lam == 0:
+        if abs(z) < 1:
+            phi = np.arccos(z)
+            pre = -1j / np.sin(phi) if branch == "R" else 1j / np.sin(phi)
+            sign = -1 if branch == "R" else 1
+            return pre * np.exp(sign * 1j * n * phi)
+        if z > 1:
+            eta = np.arccosh(z)
+            return (np.exp(-n * eta) / np.sqrt(z * z - 1)).astype(np.complex128)
+        if z < -1:
+            eta = np.arccosh(-z)
+            return (-((-1.0) ** n) * np.exp(-n * eta) / np.sqrt(z * z - 1)).astype(np.complex128)
+        raise ValueError("z = +/-1 is singular")
+    w = complex(z, lam) if branch == "R" else complex(z, -lam)
+    if branch == "R":
+        return -1j * np.exp(-1j * n * np.arccos(w)) / np.sqrt(1 - w * w)
+    return 1j * np.exp(1j * n * np.arccos(w)) / np.sqrt(1 - w * w)
+
+
+def cpge_ref_fermi(x, xF, beta):
+    if np.isinf(beta):
+        return (float(x < xF) + float(x <= xF)) / 2
+    y = beta * (x - xF)
+    return np.exp(-y) / (1 + np.exp(-y)) if y > 0 else 1 / (1 + np.exp(y))
+
+
+def cpge_ref_integral(fun, shifts, xF, beta, lam, *, delta=0.0, rtol=2e-10, atol=2e-12):
+    """Adaptive integral of sin(theta) f(cos theta) fun(cos theta) over the
+    occupied part of (0, pi), split at the shifted edges (and the delta
+    annulus), with the u^2 endpoint map at singular edges when lambda =
+    delta = 0 (test/cpge_test.jl:112-198); fun may be array-valued."""
+    from scipy.integrate import quad_vec
+
+    lo = np.arccos(np.clip(xF, -1, 1)) if np.isinf(beta) else 0.0
+    points, singular = [lo, np.pi], []
+    for shift in shifts:
+        for edge in (-1.0, 1.0):
+            xs = edge - shift
+            if -1 < xs < 1:
+                th = np.arccos(xs)
+                if lo < th < np.pi:
+                    points.append(th)
+                    if lam == 0 and delta == 0:
+                        singular.append(th)
+    if delta > 0:
+        for shift in shifts:
+            for radius in (1 - delta, 1 + delta):
+                if radius < 0:
+                    continue
+                for edge in (-radius, radius):
+                    xs = edge - shift
+                    if -1 < xs < 1:
+                        th = np.arccos(xs)
+                        if lo < th < np.pi:
+                            points.append(th)
+    if not np.isinf(beta) and beta > 50 and -1 < xF < 1:
+        points.append(np.arccos(xF))
+
+    def uniq(p, tol=1e-13):
+        out = []
+        for v in sorted(p):
+            if not out or abs(v - out[-1]) > tol:
+                out.append(v)
+        return out
+
+    points, singular = uniq(points), uniq(singular)
+
+    def isedge(th):
+        return any(abs(th - s) < 1e-12 for s in singular)
+
+    # fun(x, s) returns sin(theta) * bracket(x) with s = sin(theta): every
+    # bracket term has exactly one delta factor, whose 1/sqrt(1 - x^2) cancels
+    # sin(theta) analytically (no 0 * inf where cos(theta) rounds to +/-1)
+    def node(th):
+        return cpge_ref_fermi(np.cos(th), xF, beta) * np.asarray(fun(np.cos(th), np.sin(th)))
+
+    def quad(f, a, b):
+        return quad_vec(f, a, b, epsabs=atol, epsrel=rtol, limit=2000)[0]
+
+    def piece(a, b, le, re):
+        if le and re:
+            mid = (a + b) / 2
+            return piece(a, mid, True, False) + piece(mid, b, False, True)
+        w = b - a
+        if le:
+            return quad(lambda u: node(a + w * u * u) * (2 * w * u), 0.0, 1.0)
+        if re:
+            return quad(lambda u: node(b - w * u * u) * (2 * w * u), 0.0, 1.0)
+        return quad(node, a, b)
+
+    total = 0
+    for k in range(len(points) - 1):
+        total = total + piece(points[k], points[k + 1], isedge(points[k]), isedge(points[k + 1]))
+    return total
+
+
+def cpge_eigen_setup(Hn, Ja, Jb, Jg, NC):
+    E, V = np.linalg.eigh(np.asarray(Hn.toarray() if sp.issparse(Hn) else Hn))
+    dense = [np.asarray(J.toarray() if sp.issparse(J) else J) for J in (Ja, Jb, Jg)]
+    A, B, C = (V.conj().T @ J @ V for J in dense)
+    T = np.stack([_cheb_vec(e, NC) for e in E], axis=1)  # NC x D
+    kh = jackson(np.arange(NC), NC) * hn(np.arange(NC))
+    return E, A, B, C, T, kh
+
+
+def cpge_eigen_integrand(setup, NC, omega1, omega2, x, lam, delta=0.0, times_sin=False):
+    """The eigenbasis bracket RR + RA + AA at energy x
+    (test/cpge_test.jl:200-227): Tr[Ja T_n Jb T_m Jg T_p] puts the (n, m, p)
+    coefficient sums at eigenvalues (E_b, E_c, E_a).  times_sin: the bracket
+    times sin(theta) (x = cos(theta)), exact at the interval ends."""
+    E, A, B, C, T, kh = setup
+    Om = omega1 + omega2
+    dK = (kh * cpge_ref_delta(x, NC, times_sin)) @ T
+    g = lambda z, br: (kh * cpge_ref_green(z, NC, lam, br, delta)) @ T
+    rO, r2, r1 = g(x + Om, "R"), g(x + omega2, "R"), g(x + omega1, "R")
+    a2, a1, aO = g(x - omega2, "A"), g(x - omega1, "A"), g(x - Om, "A")
+    ABC = np.einsum("ab,bc,ca->abc", A, B, C)
+    acc = (np.einsum("abc,b,c,a->", ABC, rO, r2, dK) + np.einsum("abc,b,c,a->", ABC, r1, dK, a2) +
+           np.einsum("abc,b,c,a->", ABC, dK, a1, aO))
+    return acc / len(E)
+
+
+def cpge_eigen_reference(Hn, Ja, Jb, Jg, NC, omega1, omega2, *, xF, beta, lam, delta=0.0,
+                         rtol=2e-10, atol=2e-12):
+    """i * the integral of the eigenbasis bracket (test/cpge_test.jl:229-285)."""
+    setup = cpge_eigen_setup(Hn, Ja, Jb, Jg, NC)
+    shifts = (omega1 + omega2, omega2, omega1, -omega2, -omega1, -omega1 - omega2)
+    f = lambda x, s: cpge_eigen_integrand(setup, NC, omega1, omega2, x, lam, delta, times_sin=True)
+    return 1j * cpge_ref_integral(f, shifts, xF, beta, lam, delta=delta, rtol=rtol, atol=atol)
+
+
+def cpge_lambda_table(NC, omega1, omega2, *, xF, beta, lam):
+    """Integral of the coefficient tensor rO[n] r2[m] D[p] + r1[n] D[m] a2[p]
+    + D[n] a1[m] aO[p] (test/cpge_test.jl:287-309)."""
+    Om = omega1 + omega2
+    shifts = (Om, omega2, omega1, -omega2, -omega1, -Om)
+
+    def f(x, s):
+        D = cpge_ref_delta(x, NC, times_sin=True)
+        rO, r2 = cpge_ref_green(x + Om, NC, lam, "R"), cpge_ref_green(x + omega2, NC, lam, "R")
+        r1 = cpge_ref_green(x + omega1, NC, lam, "R")
+        a2, a1 = cpge_ref_green(x - omega2, NC, lam, "A"), cpge_ref_green(x - omega1, NC, lam, "A")
+        aO = cpge_ref_green(x - Om, NC, lam, "A")
+        return (np.einsum("n,m,p->nmp", rO, r2, D) + np.einsum("n,m,p->nmp", r1, D, a2) +
+                np.einsum("n,m,p->nmp", D, a1, aO))
+
+    return cpge_ref_integral(f, shifts, xF, beta, lam)
