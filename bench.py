@@ -498,13 +498,13 @@ def bench_cpge(args) -> int:
     print(f"[bench] kpm_3d NC={NC} NR={NR} NH={NH}: {t3d:.2f} s", file=sys.stderr, flush=True)
     w1, w2, lam = 0.2, 0.1, 0.01
     counts = []
-    orig = Table3.forms
+    orig = Table3.forms_rec
 
     def counting(self, A, B, Cc):
         counts.append(A.shape[0])
         return orig(self, A, B, Cc)
 
-    Table3.forms = counting
+    Table3.forms_rec = counting
     try:
         for _ in range(args.warmup):
             K.cpge(mu3, NC, w1, w2, E_f=0.0, lambda_=lam)
@@ -514,16 +514,16 @@ def bench_cpge(args) -> int:
             val = K.cpge(mu3, NC, w1, w2, E_f=0.0, lambda_=lam)
         wall = (time.perf_counter() - t0) / args.steps
     finally:
-        Table3.forms = orig
+        Table3.forms_rec = orig
     forms = sum(counts) // args.steps
     launches = len(counts) // args.steps
     nodes = forms // 3
     Qm = max(3, int(round(forms / max(launches, 1))))
     print(f"[bench] cpge: {wall * 1e3:.1f} ms per call, {nodes} nodes in {launches} launches "
           f"(mean batch {Qm} forms), value {val}", file=sys.stderr, flush=True)
-    mt = K.mu3D_apply_kernel_and_h(mu3, NC, K.JacksonKernel)
     reps = 20
-    with Table3(mt, NC) as tb:
+    kv = K.JacksonKernel(np.arange(NC), NC) * K.hn(np.arange(NC))
+    with Table3(mu3, NC, kvec=kv) as tb:
         ms = ctypes.c_float(0.0)
         K._lib.check(ctx.lib.kpm_bench_trilinear(tb.handle, Qm, reps, ctypes.byref(ms)), "bench_trilinear")
     tk = ms.value / reps * 1e-3
@@ -536,11 +536,14 @@ def bench_cpge(args) -> int:
           file=sys.stderr, flush=True)
     cpu = None
     if not args.skip_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
         from threadpoolctl import threadpool_info
+        mt = K.mu3D_apply_kernel_and_h(mu3, NC, K.JacksonKernel)
         sample = 16
         th = np.linspace(0.3, 2.8, sample)
         A_, B_, C_ = cpge_node_coefficients(th, NC, w1, w2, lam, 0.0)
+        O.cpge_forms(mt, A_[:3], B_[:3], C_[:3])  # warm the BLAS thread pool
         t0 = time.perf_counter()
         O.cpge_forms(mt, A_, B_, C_)
         tc = (time.perf_counter() - t0) / sample

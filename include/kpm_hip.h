@@ -225,6 +225,24 @@ int kpm_table3_destroy(kpm_table3* t);
  * contiguous; out: Q complex.  Q <= 2^20. */
 int kpm_trilinear_forms(kpm_table3* t, int64_t Q, const double* a, const double* b, const double* c,
                         double* out);
+/* The same forms with coefficient rows generated on the device from one
+ * 5-double descriptor per row (desc_x: Q x 5, row-major):
+ *   (0, c, -, -, -)       delta row: d[0] = 1/pi, d[1] = c/pi,
+ *                         d[n] = 2c d[n-1] - d[n-2]  (chebyshev_delta_theta!,
+ *                         src/applications/spectral_coefficients.jl:22-33)
+ *   (1, g0r, g0i, qr, qi) resolvent row: g[0] = g0, g[n] = g[n-1] q
+ *                         (green_coefficients!, spectral_coefficients.jl:84-86)
+ *   (2, -, -, -, -)       zero row (the delta-annulus exclusion,
+ *                         src/applications/cpge.jl:30-36)
+ * so a quadrature round ships O(Q) scalars instead of O(Q NC) values. */
+int kpm_trilinear_forms_rec(kpm_table3* t, int64_t Q, const double* desc_a, const double* desc_b,
+                            const double* desc_c, double* out);
+/* kpm_table3_create from the raw table mu (NC^3, n fastest) and the per-axis
+ * weight k[n] = kernel(n, NC) h_n: the device applies
+ * mu~ = ((mu k[n]) k[m]) k[p], the axis order of muND_apply_kernel_and_h
+ * (src/kernels/kernels.jl:92-103; cpge.jl:122-125). */
+int kpm_table3_create_weighted(kpm_ctx* ctx, int64_t NC, const double* mu, const double* kvec,
+                               kpm_table3** out);
 /* Measurement helper (no reference counterpart): total ms of `reps`
  * kpm_trilinear_forms launches (contraction + fixed-order reduce, no
  * transfers) of Q forms with device-generated unit-modulus coefficients,

@@ -168,6 +168,70 @@ __global__ void k_trilinear_reduce(const double2* __restrict__ part, int ntiles,
   out[q] = make_double2(sx, sy);
 }
 
+// mu~[n,m,p] = ((mu[n,m,p] k[n]) k[m]) k[p]: the damping kernel and h_n
+// along each axis, applied axis by axis in the order of
+// muND_apply_kernel_and_h (src/kernels/kernels.jl:92-103)
+__global__ void k_weight3(double2* __restrict__ mu, const double* __restrict__ k, int NC) {
+  const int64_t total = (int64_t)NC * NC * NC;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int n = (int)(idx % NC);
+    const int m = (int)((idx / NC) % NC);
+    const int p = (int)(idx / ((int64_t)NC * NC));
+    double2 v = mu[idx];
+    v.x *= k[n];
+    v.y *= k[n];
+    v.x *= k[m];
+    v.y *= k[m];
+    v.x *= k[p];
+    v.y *= k[p];
+    mu[idx] = v;
+  }
+}
+
+// Coefficient rows from per-row recurrence descriptors (5 doubles each):
+//   kind 0, c:        delta[0] = 1/pi, delta[1] = c/pi,
+//                     delta[n] = 2c delta[n-1] - delta[n-2]
+//                     (chebyshev_delta_theta!, spectral_coefficients.jl:22-33)
+//   kind 1, g0, q:    g[0] = g0, g[n] = g[n-1] q
+//                     (green_coefficients!, spectral_coefficients.jl:84-86)
+//   kind 2:           zeros (the delta-annulus exclusion, cpge.jl:30-36)
+// One thread per row, n in order, so each row follows the reference's
+// sequential recurrence.
+__global__ void k_expand_rows(const double* __restrict__ desc, int64_t rows, int NC,
+                              double2* __restrict__ out) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const double* d = desc + 5 * r;
+  double2* o = out + r * NC;
+  const int kind = (int)d[0];
+  if (kind == 0) {
+    const double c = d[1];
+    const double inv_pi = 0.31830988618379067154;
+    double dm2 = inv_pi, dm1 = c / 3.14159265358979323846;
+    o[0] = make_double2(dm2, 0.0);
+    if (NC > 1) o[1] = make_double2(dm1, 0.0);
+    for (int n = 2; n < NC; ++n) {
+      const double dn = 2.0 * c * dm1 - dm2;
+      o[n] = make_double2(dn, 0.0);
+      dm2 = dm1;
+      dm1 = dn;
+    }
+  } else if (kind == 1) {
+    double gr = d[1], gi = d[2];
+    const double qr = d[3], qi = d[4];
+    o[0] = make_double2(gr, gi);
+    for (int n = 1; n < NC; ++n) {
+      const double nr = gr * qr - gi * qi, ni = gr * qi + gi * qr;
+      gr = nr;
+      gi = ni;
+      o[n] = make_double2(gr, gi);
+    }
+  } else {
+    for (int n = 0; n < NC; ++n) o[n] = make_double2(0.0, 0.0);
+  }
+}
+
 // deterministic unit-modulus coefficient rows for the timing helper
 __global__ void k_fill_phase(double2* __restrict__ v, int64_t n, double step) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -187,6 +251,28 @@ struct kpm_table3 {
 };
 
 using namespace kpm;
+
+namespace {
+
+// device coefficient rows da (real view), db, dc -> host out (Q complex)
+int run_forms(kpm_table3* t, int64_t Q, const double* da, const double2* db, const double2* dc,
+              double2* dpart, double2* dout, double* out) {
+  kpm_ctx* ctx = t->ctx;
+  const int64_t NC = t->NC, NC2 = NC * NC;
+  const int64_t ntx = (NC2 + kTJ - 1) / kTJ, nqy = (Q + kTQ - 1) / kTQ;
+  const unsigned grid = (unsigned)((ntx + 7) / 8 * 8 * nqy);
+  hipLaunchKernelGGL(k_trilinear, dim3(grid), dim3(256), 0, ctx->stream, (const double*)t->mu, (int)NC,
+                     da, db, dc, (int)Q, (int)ntx, (int)nqy, dpart);
+  KPM_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_trilinear_reduce, dim3((unsigned)((Q + 255) / 256)), dim3(256), 0, ctx->stream,
+                     (const double2*)dpart, (int)ntx, (int)Q, dout);
+  KPM_HIP_TRY(hipGetLastError());
+  KPM_HIP_TRY(hipMemcpyAsync(out, dout, (size_t)Q * sizeof(double2), hipMemcpyDeviceToHost, ctx->stream));
+  KPM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return KPM_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -246,17 +332,71 @@ int kpm_trilinear_forms(kpm_table3* t, int64_t Q, const double* a, const double*
   KPM_HIP_TRY(hipMemcpyAsync(da, a, bcoef, hipMemcpyHostToDevice, ctx->stream));
   KPM_HIP_TRY(hipMemcpyAsync(db, b, bcoef, hipMemcpyHostToDevice, ctx->stream));
   KPM_HIP_TRY(hipMemcpyAsync(dc, c, bcoef, hipMemcpyHostToDevice, ctx->stream));
-  const unsigned grid = (unsigned)((ntx + 7) / 8 * 8 * nqy);
-  hipLaunchKernelGGL(k_trilinear, dim3(grid), dim3(256), 0, ctx->stream, (const double*)t->mu, (int)NC,
-                     (const double*)da, (const double2*)db, (const double2*)dc, (int)Q, (int)ntx,
-                     (int)nqy, dpart);
+  return run_forms(t, Q, da, db, dc, dpart, dout, out);
+}
+
+int kpm_trilinear_forms_rec(kpm_table3* t, int64_t Q, const double* desc_a, const double* desc_b,
+                            const double* desc_c, double* out) {
+  if (!t || (Q > 0 && (!desc_a || !desc_b || !desc_c || !out))) return fail(KPM_EINVAL, "null argument");
+  if (Q < 0 || Q > (1 << 20)) return fail(KPM_EINVAL, "Q must lie in [0, 2^20]");
+  if (Q == 0) return KPM_OK;
+  for (const double* d : {desc_a, desc_b, desc_c})
+    for (int64_t r = 0; r < Q; ++r) {
+      const double k = d[5 * r];
+      if (!(k == 0.0 || k == 1.0 || k == 2.0)) return fail(KPM_EINVAL, "row kind must be 0, 1 or 2");
+    }
+  kpm_ctx* ctx = t->ctx;
+  const int64_t NC = t->NC, NC2 = NC * NC;
+  KPM_HIP_TRY(hipSetDevice(ctx->device));
+  const int64_t ntx = (NC2 + kTJ - 1) / kTJ, nqy = (Q + kTQ - 1) / kTQ;
+  if ((ntx + 7) / 8 * 8 * nqy > INT32_MAX) return fail(KPM_EINVAL, "too many forms in one call");
+  const size_t bcoef = (size_t)(Q * NC) * sizeof(double2), bpart = (size_t)(ntx * Q) * sizeof(double2),
+               bout = (size_t)Q * sizeof(double2), bdesc = (size_t)(15 * Q) * sizeof(double);
+  KPM_TRY(ensure(ctx, t->work, 3 * bcoef + bpart + bout + bdesc));
+  char* base = (char*)t->work.ptr;
+  double2* rows = (double2*)base;  // a | b | c, each Q x NC
+  double2* dpart = (double2*)(base + 3 * bcoef);
+  double2* dout = (double2*)(base + 3 * bcoef + bpart);
+  double* ddesc = (double*)(base + 3 * bcoef + bpart + bout);
+  const size_t b1 = (size_t)(5 * Q) * sizeof(double);
+  KPM_HIP_TRY(hipMemcpyAsync(ddesc, desc_a, b1, hipMemcpyHostToDevice, ctx->stream));
+  KPM_HIP_TRY(hipMemcpyAsync(ddesc + 5 * Q, desc_b, b1, hipMemcpyHostToDevice, ctx->stream));
+  KPM_HIP_TRY(hipMemcpyAsync(ddesc + 10 * Q, desc_c, b1, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_expand_rows, dim3((unsigned)((3 * Q + 63) / 64)), dim3(64), 0, ctx->stream,
+                     (const double*)ddesc, 3 * Q, (int)NC, rows);
   KPM_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_trilinear_reduce, dim3((unsigned)((Q + 255) / 256)), dim3(256), 0, ctx->stream,
-                     (const double2*)dpart, (int)ntx, (int)Q, dout);
-  KPM_HIP_TRY(hipGetLastError());
-  KPM_HIP_TRY(hipMemcpyAsync(out, dout, bout, hipMemcpyDeviceToHost, ctx->stream));
-  KPM_HIP_TRY(hipStreamSynchronize(ctx->stream));
-  return KPM_OK;
+  return run_forms(t, Q, (const double*)rows, rows + Q * NC, rows + 2 * Q * NC, dpart, dout, out);
+}
+
+int kpm_table3_create_weighted(kpm_ctx* ctx, int64_t NC, const double* mu, const double* kvec,
+                               kpm_table3** out) {
+  if (!kvec) return fail(KPM_EINVAL, "null argument");
+  KPM_TRY(kpm_table3_create(ctx, NC, mu, out));
+  kpm_table3* t = *out;
+  double* dk = nullptr;
+  if (hipMalloc(&dk, (size_t)NC * sizeof(double)) != hipSuccess) {
+    kpm_table3_destroy(t);
+    *out = nullptr;
+    (void)hipGetLastError();
+    return fail(KPM_ENOMEM, "device allocation failed");
+  }
+  const int64_t total = NC * NC * NC;
+  const unsigned g = (unsigned)std::min<int64_t>((total + 255) / 256, 65536);
+  int rc = KPM_OK;
+  if (hipMemcpyAsync(dk, kvec, (size_t)NC * sizeof(double), hipMemcpyHostToDevice, ctx->stream) !=
+      hipSuccess)
+    rc = fail(KPM_EHIP, "kernel vector upload failed");
+  if (rc == KPM_OK) {
+    hipLaunchKernelGGL(k_weight3, dim3(g), dim3(256), 0, ctx->stream, t->mu, (const double*)dk, (int)NC);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess)
+      rc = fail(KPM_EHIP, "kernel weighting failed");
+  }
+  (void)hipFree(dk);
+  if (rc != KPM_OK) {
+    kpm_table3_destroy(t);
+    *out = nullptr;
+  }
+  return rc;
 }
 
 int kpm_bench_trilinear(kpm_table3* t, int64_t Q, int64_t reps, float* ms) {

@@ -101,6 +101,8 @@ _SIGS = {
     "kpm_table3_destroy": (C.c_int, [_p]),
     "kpm_trilinear_forms": (C.c_int, [_p, _i64, _pd, _pd, _pd, _pd]),
     "kpm_bench_trilinear": (C.c_int, [_p, _i64, _i64, C.POINTER(C.c_float)]),
+    "kpm_trilinear_forms_rec": (C.c_int, [_p, _i64, _pd, _pd, _pd, _pd]),
+    "kpm_table3_create_weighted": (C.c_int, [_p, _i64, _pd, _pd, C.POINTER(C.c_void_p)]),
     "kpm_bond_gradient": (C.c_int, [_i64, _pi64, _pi64, _pd, _pd, C.c_double, _pi64, _pi64, _pd,
                                     _pi64]),
     "kpm_1d_adjoint": (C.c_int, [_p, _p, _i64, _i64, _i64, _pd, _pd, _pd, C.c_int, _i64]),

@@ -2,11 +2,14 @@
 (src/applications/cpge.jl).
 
 The hot part is the node function: at each quadrature node it evaluates
-three trilinear forms of the kernel-weighted NC^3 table. The table is
-uploaded once per call to a device-resident kpm_table3. Every refinement
-round of the adaptive integral then evaluates all its nodes in one
-kpm_trilinear_forms launch, an FP64 MFMA pass over the table. Coefficient
-vectors, the Fermi weight and the quadrature bookkeeping are host FP64.
+three trilinear forms of the kernel-weighted NC^3 table. The raw table is
+uploaded once per call to a device-resident kpm_table3, which applies the
+kernel weights on the device. Every refinement round of the adaptive
+integral then evaluates all its nodes in one kpm_trilinear_forms_rec call:
+the device expands each coefficient row from its recurrence seed (cos theta,
+or the resolvent's g0 and q), then runs one FP64-MFMA pass over the table.
+The host keeps the per-node seeds, the Fermi weight and the quadrature
+bookkeeping (FP64).
 """
 from __future__ import annotations
 
@@ -17,8 +20,8 @@ import numpy as np
 
 from ._lib import ArgumentError, DomainError, check, dptr
 from .device import get_context
-from .kernels import JacksonKernel, mu3D_apply_kernel_and_h
-from .spectral import (chebyshev_delta_theta, deduplicate_sorted, green_coefficients,
+from .kernels import JacksonKernel, hn
+from .spectral import (chebyshev_delta_theta, deduplicate_sorted, green_coefficients, green_seed,
                        spectral_integral)
 
 # Paper coefficients (u, v, w) multiply package tensor slots (2, 1, 3):
@@ -27,17 +30,45 @@ CPGE_SLOTS = (2, 1, 3)
 
 
 class Table3:
-    """A kernel-weighted NC^3 table resident on the device (kpm_table3).
-    mu_tilde[n, m, p] is stored n fastest (Julia column-major)."""
+    """A kernel-weighted NC^3 table resident on the device (kpm_table3),
+    stored n fastest (Julia column-major mu[n, m, p]). With ``kvec`` the
+    table is the raw moments and the device applies the per-axis weights
+    k[n] = kernel(n, NC) h_n; without, it is uploaded as given."""
 
-    def __init__(self, mu_tilde: np.ndarray, NC: int):
+    def __init__(self, mu: np.ndarray, NC: int, kvec=None):
         self.ctx = get_context()
         self.NC = int(NC)
-        host = np.asfortranarray(mu_tilde[:NC, :NC, :NC], dtype=np.complex128)
+        host = np.asfortranarray(np.asarray(mu)[:NC, :NC, :NC], dtype=np.complex128)
         h = C.c_void_p()
-        check(self.ctx.lib.kpm_table3_create(self.ctx.handle, self.NC, dptr(host), C.byref(h)),
-              "kpm_table3_create")
+        if kvec is None:
+            check(self.ctx.lib.kpm_table3_create(self.ctx.handle, self.NC, dptr(host), C.byref(h)),
+                  "kpm_table3_create")
+        else:
+            kv = np.ascontiguousarray(kvec, dtype=np.float64)
+            if kv.shape != (self.NC,):
+                raise ArgumentError("kernel vector must have NC entries")
+            check(self.ctx.lib.kpm_table3_create_weighted(self.ctx.handle, self.NC, dptr(host), dptr(kv),
+                                                          C.byref(h)), "kpm_table3_create_weighted")
         self.handle = h
+
+    def forms_rec(self, da: np.ndarray, db: np.ndarray, dc: np.ndarray) -> np.ndarray:
+        """Forms with device-generated coefficient rows from (Q, 5)
+        descriptors (kind, p0r, p0i, p1r, p1i): kind 0 delta row (p0r =
+        cos theta), 1 resolvent row (g0 = p0, q = p1), 2 zero row."""
+        da, db, dc = (np.ascontiguousarray(x, dtype=np.float64) for x in (da, db, dc))
+        Q = da.shape[0]
+        if da.shape != (Q, 5) or db.shape != da.shape or dc.shape != da.shape:
+            raise ArgumentError("descriptors must be Q x 5")
+        out = np.zeros(Q, dtype=np.complex128)
+        step = 1 << 20
+        for q0 in range(0, Q, step):
+            q1 = min(Q, q0 + step)
+            part = np.zeros(q1 - q0, dtype=np.complex128)
+            check(self.ctx.lib.kpm_trilinear_forms_rec(self.handle, q1 - q0, dptr(da[q0:q1]),
+                                                       dptr(db[q0:q1]), dptr(dc[q0:q1]), dptr(part)),
+                  "kpm_trilinear_forms_rec")
+            out[q0:q1] = part
+        return out
 
     def forms(self, a: np.ndarray, b: np.ndarray, c: np.ndarray) -> np.ndarray:
         """out[q] = sum mu~[n,m,p] a[q,n] b[q,m] c[q,p] (plain products)."""
@@ -146,19 +177,61 @@ def cpge_node_coefficients(theta, NC, omega1, omega2, lam, delta):
     return (v.reshape(3 * N, NC), u.reshape(3 * N, NC), w.reshape(3 * N, NC))
 
 
+def _green_desc(z, lam, delta, branch):
+    """(N, 5) resolvent-row descriptors: (1, g0, q), or (2, ...) on the
+    delta annulus."""
+    z = np.asarray(z, dtype=np.float64)
+    d = np.zeros((z.size, 5))
+    d[:, 0] = 2.0
+    keep = ~((delta > 0) & (1 - delta < np.abs(z)) & (np.abs(z) < 1 + delta))
+    if np.any(keep):
+        g0, q = green_seed(z[keep], lam, branch)
+        d[keep, 0] = 1.0
+        d[keep, 1], d[keep, 2] = g0.real, g0.imag
+        d[keep, 3], d[keep, 4] = q.real, q.imag
+    return d
+
+
+def cpge_node_descriptors(theta, omega1, omega2, lam, delta):
+    """Row descriptors of the forms RR, RA, AA at each node (the recurrence
+    seeds of cpge_node_coefficients): (a, b, c), each (3 N, 5), node-major."""
+    theta = np.atleast_1d(np.asarray(theta, dtype=np.float64))
+    N = theta.size
+    Omega = omega1 + omega2
+    x = np.cos(theta)
+    D = np.zeros((N, 5))
+    D[:, 1] = x
+    gRO = _green_desc(x + Omega, lam, delta, "R")
+    gR2 = _green_desc(x + omega2, lam, delta, "R")
+    gR1 = _green_desc(x + omega1, lam, delta, "R")
+    gAm2 = _green_desc(x - omega2, lam, delta, "A")
+    gAm1 = _green_desc(x - omega1, lam, delta, "A")
+    gAmO = _green_desc(x - Omega, lam, delta, "A")
+    u = np.stack([gRO, gR1, D], axis=1)
+    v = np.stack([gR2, D, gAm1], axis=1)
+    w = np.stack([D, gAm2, gAmO], axis=1)
+    return (v.reshape(3 * N, 5), u.reshape(3 * N, 5), w.reshape(3 * N, 5))
+
+
 def _node_function(table: Table3, NC, omega1, omega2, lam, delta):
     def F(theta):
         theta = np.atleast_1d(np.asarray(theta, dtype=np.float64))
-        a, b, c = cpge_node_coefficients(theta, NC, omega1, omega2, lam, delta)
-        forms = table.forms(a, b, c).reshape(theta.size, 3)
+        a, b, c = cpge_node_descriptors(theta, omega1, omega2, lam, delta)
+        forms = table.forms_rec(a, b, c).reshape(theta.size, 3)
         return (forms[:, 0] + forms[:, 1] + forms[:, 2])[:, None]  # rr + ra + aa
 
     return F
 
 
-def _kernel_moments(mu3, NC, kernel):
-    """_cpge_kernel_moments (cpge.jl:122-125)."""
-    return mu3D_apply_kernel_and_h(np.asarray(mu3)[:NC, :NC, :NC], NC, kernel)
+def _kernel_vector(kernel, NC):
+    """k[n] = kernel(n, NC) h_n, the per-axis weight of
+    mu3D_apply_kernel_and_h (cpge.jl:122-125)."""
+    ns = np.arange(NC)
+    return np.asarray(kernel(ns, NC), dtype=np.float64) * hn(ns)
+
+
+def _weighted_table(mu3, NC, kernel):
+    return Table3(np.asarray(mu3), NC, kvec=_kernel_vector(kernel, NC))
 
 
 def cpge(mu3, NC: int, omega1_tilde: float, omega2_tilde: float, *, E_f: float = 0.0,
@@ -171,15 +244,17 @@ def cpge(mu3, NC: int, omega1_tilde: float, omega2_tilde: float, *, E_f: float =
     NC = int(NC)
     omega1_tilde, omega2_tilde = float(omega1_tilde), float(omega2_tilde)
     _check_cpge_inputs(mu3, NC, omega1_tilde, omega2_tilde, lambda_, delta)
-    mt = _kernel_moments(mu3, NC, kernel)
     Omega = omega1_tilde + omega2_tilde
     shifts = (Omega, omega2_tilde, omega1_tilde, -omega2_tilde, -omega1_tilde, -Omega)
-    with Table3(mt, NC) as table:
+    kv = _kernel_vector(kernel, NC)
+    # structural zero of mu~ (all moments zero, or every weight zero)
+    exact_zero = not (np.any(np.asarray(mu3)[:NC, :NC, :NC]) and np.any(kv))
+    with _weighted_table(mu3, NC, kernel) as table:
         F = _node_function(table, NC, omega1_tilde, omega2_tilde, lambda_, delta)
         integral, _ = spectral_integral(F, 1, NC, shifts, float(E_f), float(beta), float(lambda_),
                                         rtol=quad_rtol, atol=quad_atol, maxevals=maxevals,
                                         breakpoints=cpge_exclusion_breakpoints(shifts, delta),
-                                        exact_zero=[not np.any(mt)])
+                                        exact_zero=[exact_zero])
     return complex(1j * integral[0])
 
 
@@ -208,8 +283,7 @@ def d_cpge(mu3, NC: int, omega1_tilde: float, omega2_tilde: float, x=None, *, la
         grid = ((np.arange(N_int) + 0.5) / N_int) * (e_range[1] - e_range[0]) + e_range[0]
         if not np.all((grid > -1) & (grid < 1)):
             raise DomainError(f"{list(e_range)}: resolved grid nodes must lie strictly inside (-1,1)")
-    mt = _kernel_moments(mu3, NC, kernel)
-    with Table3(mt, NC) as table:
+    with _weighted_table(mu3, NC, kernel) as table:
         F = _node_function(table, NC, omega1_tilde, omega2_tilde, lambda_, delta)
         vals = F(np.arccos(grid))[:, 0] / np.sqrt(1 - grid ** 2)
     if x is not None:

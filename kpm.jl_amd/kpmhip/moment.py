@@ -397,11 +397,21 @@ def kpm_2d(H, Ja, Jb, NC: int, NR: int, NH: int | None = None, *, psi_in=None, p
 
 
 def kpm_3d_(H, Ja, Jb, Jg, NC: int, NR: int, NH: int, mu: np.ndarray, psi_in_l, psi_in_r, *,
-            arr_size: int = 3, right_block: int | None = None, verbose: int = 0,
+            arr_size="auto", right_block="auto", workspace_bytes="auto", verbose: int = 0,
             precision="f64", gemm="f64"):
     """kpm_3d! (src/moment.jl:1106-1227): μ[n,m,p] = (1/NR) Σ_r
-    <T_{m-1} ψl | Jβ T_{n-1} Jγ T_{p-1} Jα ψr>."""
-    right_block = min(NC, 16) if right_block is None else int(right_block)
+    <T_{m-1} ψl | Jβ T_{n-1} Jγ T_{p-1} Jα ψr>.
+
+    Each of the NC kpm_2d passes uses the engine's slot plan
+    (engine_blocking): with ':auto' sizes every left and right slot stays
+    resident when the workspace allows, so each pass is one MFMA
+    contraction. The reference's default (arr_size = 3, right_block =
+    min(NC, 16)) recomputes the right chain NC/3 times per pass; explicit
+    sizes are kept as given, and any blocking gives the same table to
+    rounding (test/kpm2d_blocking_test.jl:97-117)."""
+    plan = engine_blocking(NH, NR, NC, precision=precision, workspace_bytes=workspace_bytes,
+                           arr_size=arr_size, right_block=right_block)
+    arr_size, right_block = plan["arr_size"], plan["right_block"]
     psl = _probe(psi_in_l, NH, NR, "psi_in_l")
     psr = _probe(psi_in_r, NH, NR, "psi_in_r")
     if arr_size < 2:
@@ -417,9 +427,9 @@ def kpm_3d_(H, Ja, Jb, Jg, NC: int, NR: int, NH: int, mu: np.ndarray, psi_in_l, 
     mu[...] = out
 
 
-def kpm_3d(H, Ja, Jb, Jg, NC: int, NR: int, NH: int | None = None, *, arr_size: int = 3,
-           right_block: int | None = None, verbose: int = 0, psi_in_l=None, psi_in_r=None,
-           psi_in=None, precision="f64", gemm="f64"):
+def kpm_3d(H, Ja, Jb, Jg, NC: int, NR: int, NH: int | None = None, *, arr_size="auto",
+           right_block="auto", workspace_bytes="auto", verbose: int = 0, psi_in_l=None,
+           psi_in_r=None, psi_in=None, precision="f64", gemm="f64"):
     """kpm_3d (src/moment.jl:1229-1280): the NC x NC x NC table."""
     import warnings
     NC, NR = int(NC), int(NR)
@@ -439,5 +449,6 @@ def kpm_3d(H, Ja, Jb, Jg, NC: int, NR: int, NH: int | None = None, *, arr_size: 
         psi_in_l = random_probes(NH, NR)
         psi_in_r = psi_in_l
     kpm_3d_(H, Ja, Jb, Jg, NC, NR, NH, mu, psi_in_l, psi_in_r, arr_size=arr_size,
-            right_block=right_block, verbose=verbose, precision=precision, gemm=gemm)
+            right_block=right_block, workspace_bytes=workspace_bytes, verbose=verbose,
+            precision=precision, gemm=gemm)
     return mu

@@ -81,11 +81,10 @@ def chebyshev_delta_theta(theta, NC: int) -> np.ndarray:
     return out
 
 
-def green_coefficients(z, NC: int, lam: float, branch: str) -> np.ndarray:
-    """Chebyshev coefficients of the retarded ("R") or advanced ("A")
-    resolvent at real energies z (green_coefficients!,
-    spectral_coefficients.jl:43-87): g[n] = g[0] q^n by repeated
-    multiplication; z (N,) -> (N, NC) complex."""
+def green_seed(z, lam: float, branch: str):
+    """(g0, q) of the retarded ("R") or advanced ("A") resolvent coefficients
+    at real energies z (green_coefficients!, spectral_coefficients.jl:43-82):
+    g[n] = g[n-1] q with g[0] = g0."""
     if not lam >= 0:
         raise ArgumentError(f"lambda must be nonnegative (got {lam})")
     if branch not in ("R", "A"):
@@ -118,6 +117,15 @@ def green_coefficients(z, NC: int, lam: float, branch: str) -> np.ndarray:
         phi = np.arccos(w)
         g0[:] = 1j / np.sqrt(1 - w * w)
         q[:] = np.exp(1j * phi)
+    return g0, q
+
+
+def green_coefficients(z, NC: int, lam: float, branch: str) -> np.ndarray:
+    """Chebyshev coefficients of the resolvent at real energies z
+    (green_coefficients!, spectral_coefficients.jl:43-87): g[n] = g[0] q^n
+    by repeated multiplication; z (N,) -> (N, NC) complex."""
+    g0, q = green_seed(z, lam, branch)
+    z = np.atleast_1d(np.asarray(z, dtype=np.float64))
     if NC == 0:
         return np.empty((z.size, 0), dtype=np.complex128)
     seq = np.empty((z.size, NC), dtype=np.complex128)

@@ -53,6 +53,29 @@ def test_trilinear_forms_vs_oracle(NC, Q):
     assert np.array_equal(got, again)  # fixed-order reductions: run to run identical
 
 
+@pytest.mark.parametrize("NC", [3, 12, 64])
+def test_device_rows_and_weighting_match_host(NC):
+    """Device row expansion (kpm_trilinear_forms_rec) and device kernel
+    weighting (kpm_table3_create_weighted) against the host coefficient rows
+    and host-weighted table through the same contraction."""
+    from kpmhip.cpge import _kernel_vector, cpge_node_coefficients, cpge_node_descriptors
+    rng = np.random.default_rng(NC)
+    mu = rng.standard_normal((NC, NC, NC)) + 1j * rng.standard_normal((NC, NC, NC))
+    kv = _kernel_vector(K.JacksonKernel, NC)
+    mt = K.mu3D_apply_kernel_and_h(mu, NC, K.JacksonKernel)
+    theta = np.linspace(0.05, 3.1, 23)
+    for lam, delta in ((0.0, 0.0), (0.02, 0.0), (0.0, 0.05)):
+        a, b, c = cpge_node_coefficients(theta, NC, 0.3, -0.45, lam, delta)
+        da, db, dc = cpge_node_descriptors(theta, 0.3, -0.45, lam, delta)
+        with Table3(mt, NC) as th, Table3(mu, NC, kvec=kv) as td:
+            host = th.forms(a, b, c)
+            weighted = td.forms(a, b, c)
+            dev = td.forms_rec(da, db, dc)
+        assert np.array_equal(host, weighted)  # same weighting order, bit for bit
+        scale = np.abs(host).max()
+        assert np.abs(dev - host).max() <= 1e-12 * scale
+
+
 def test_kpm_3d_helix_vs_dense():
     NC = 8
     Hn, J = helix()

@@ -253,6 +253,9 @@ def test_kpm3d_vs_oracle():
     got = K.kpm_3d(Hn, Jx, Jy, Jx, NC, NR, psi_in_l=pl, psi_in_r=pr, arr_size=3, right_block=3)
     ref = oracle.kpm_3d(Hn, Jx, Jy, Jx, NC, pl, pr, 3, 3)
     np.testing.assert_allclose(got, ref, **TOL)
+    # the engine's own slot plan (all slots resident) gives the same table
+    auto = K.kpm_3d(Hn, Jx, Jy, Jx, NC, NR, psi_in_l=pl, psi_in_r=pr)
+    np.testing.assert_allclose(auto, ref, **TOL)
 
 
 # ---- FP32 recurrence -------------------------------------------------------------------------
