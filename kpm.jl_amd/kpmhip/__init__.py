@@ -22,6 +22,10 @@ from .applications import dos as dos_raw
 from .applications import dos0 as dos0_raw
 from .applications import kubo_bastin_cond as kubo_bastin_cond_raw
 from .batch import BatchOperator, kpm_batch_1d
+from .bdg import (BdGOperator, ScaledOperator, bdg_assemble, chebyshev_stability_probe,
+                  gershgorin_bound, spectral_radius)
+from .stiffness import (diamagnetic_term, nambu_current_q, nambu_diamagnetic, superfluid_stiffness,
+                        two_energy_response)
 from .cpge import Table3, cpge, d_cpge
 from .device import (ACTIVE_DEVICE, CPUDevice, DeviceBlock, DeviceCSR, HIPDevice, activate,
                      device_free_memory, get_context, maybe_on_device_zeros, maybe_to_host,

@@ -199,6 +199,8 @@ def to_device(A, ctx: Context | None = None) -> DeviceCSR:
     a DeviceCSR passes through (ext/KPMCUDAExt.jl:26-34)."""
     if isinstance(A, DeviceCSR):
         return A
+    if hasattr(A, "device_operator"):  # BdG and scaled operators assemble themselves
+        return A.device_operator(ctx)
     return DeviceCSR(A, ctx)
 
 

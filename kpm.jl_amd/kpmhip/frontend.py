@@ -94,9 +94,19 @@ def npairs(m: GreenMoments) -> int:
     return int(m.mu.shape[1])
 
 
-def rescale(H, *, center: bool = False, eps: float = 0.1, fixed_a: float = 0.0,
-            method: str = "lanczos"):
-    """src/frontend.jl:149-161 (``method``: the normalizeH edge estimator)"""
+def rescale(H, *, center: bool = False, eps: float | None = None, fixed_a: float = 0.0,
+            method: str = "lanczos", **bdg_kwargs):
+    """src/frontend.jl:149-161 (``method``: the normalizeH edge estimator).
+    A BdGOperator takes the radial BdG rescaling instead
+    (src/frontend.jl:163-203: eps default 0.2, ``bound``, ``radius`` and the
+    power-iteration keywords)."""
+    from .bdg import BdGOperator, rescale_bdg
+
+    if isinstance(H, BdGOperator):
+        return rescale_bdg(H, eps=0.2 if eps is None else eps, **bdg_kwargs)
+    if bdg_kwargs:
+        raise ArgumentError(f"rescale: unexpected keywords {sorted(bdg_kwargs)}")
+    eps = 0.1 if eps is None else eps
     if center:
         a, b, Hn = normalizeH(H, eps=eps, fixed_a=fixed_a, center=True, method=method)
     else:

@@ -13,6 +13,10 @@ CSC ComplexF64, 0-based).  Each restates a model builder of the reference:
                        coordinates and cell area, for local Chern markers)
 - ``ring_model`` / ``flux_ring_model``  test/ed_reference.jl:530-570
 - ``square_disordered`` the periodic square lattice with on-site eps_i ~ U[-1, 1]
+- ``square_model``     test/ed_reference.jl:653-690 (periodic square with site
+                       coordinates and the minimum-image displacement)
+- ``ring_geometry``    the coordinates / displacement of ring_model and
+                       flux_ring_model (test/ed_reference.jl:535-573)
 - ``helix_cluster``    test/cpge_test.jl:10-35 (six-site chiral helix with three
                        bond-current operators, for the three-current response)
                        and Hermitian-tied bond hoppings t_ij ~ 1 + U[-0.1, 0.1]
@@ -290,3 +294,53 @@ def helix_cluster(*, t: float = 1.0, t2: float = 0.45, phi: float = 0.7, w: floa
         H[j, j] = w * (-1) ** (j + 1)
     J = [sp.csc_matrix(H * (pos[:, a][:, None] - pos[:, a][None, :])) for a in range(3)]
     return sp.csc_matrix(H), J, pos
+
+
+def square_model(Lx: int, Ly: int, *, t: float = 1.0):
+    """Periodic Lx x Ly square lattice, hopping -t, site i = ix + Lx iy at
+    (ix, iy); returns (h, pos, disp) with disp(i, j) the minimum-image
+    r_i - r_j (test/ed_reference.jl:653-690)."""
+    if not (Lx > 0 and Ly > 0):
+        raise ValueError("square_model: Lx and Ly must be positive")
+    N = Lx * Ly
+    ix, iy = np.meshgrid(np.arange(Lx), np.arange(Ly), indexing="ij")
+    ix, iy = ix.T.ravel(), iy.T.ravel()  # site order i = ix + Lx iy
+    i = ix + Lx * iy
+    pos = np.zeros((N, 2))
+    pos[i, 0], pos[i, 1] = ix, iy
+    H = sp.lil_matrix((N, N))
+    for dx, dy in ((1, 0), (-1, 0), (0, 1), (0, -1)):
+        j = (ix + dx) % Lx + Lx * ((iy + dy) % Ly)
+        H[i, j] = -float(t)
+    h = sp.csc_matrix(H)
+
+    def disp(a, b):
+        dx = pos[a, 0] - pos[b, 0]
+        dy = pos[a, 1] - pos[b, 1]
+        if dx > Lx / 2:
+            dx -= Lx
+        if dx < -Lx / 2:
+            dx += Lx
+        if dy > Ly / 2:
+            dy -= Ly
+        if dy < -Ly / 2:
+            dy += Ly
+        return np.array([dx, dy])
+
+    return h, pos, disp
+
+
+def ring_geometry(N: int):
+    """Coordinates (k, 0) and the minimum-image displacement of the N-site
+    ring (test/ed_reference.jl:535-550)."""
+    pos = np.stack([np.arange(N, dtype=np.float64), np.zeros(N)], axis=1)
+
+    def disp(a, b):
+        dx = pos[a, 0] - pos[b, 0]
+        if dx > N / 2:
+            dx -= N
+        if dx < -N / 2:
+            dx += N
+        return np.array([dx, 0.0])
+
+    return pos, disp

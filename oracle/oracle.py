@@ -680,3 +680,90 @@ def cpge_lambda_table(NC, omega1, omega2, *, xF, beta, lam):
                 np.einsum("n,m,p->nmp", D, a1, aO))
 
     return cpge_ref_integral(f, shifts, xF, beta, lam)
+
+
+# ---- BdG and stiffness ED references (test/ed_reference.jl:576-735) ----------
+
+def bdg_matrix(h, mu, U, n, Delta, convention="intervalley"):
+    """Dense reduced Nambu matrix [xi D; D^* hole] with xi = h - mu - U n/2
+    (bdg_matrix / bdg_matrix_singlet, test/ed_reference.jl:576-598): hole =
+    -xi (intervalley) or -conj(xi) (singlet)."""
+    hd = h.toarray() if sp.issparse(h) else np.asarray(h)
+    N = hd.shape[0]
+    xi = hd - mu * np.eye(N) - np.diag(np.asarray(U) * np.asarray(n) / 2)
+    D = np.diag(np.asarray(Delta, dtype=np.complex128))
+    hole = -xi if convention == "intervalley" else -np.conj(xi)
+    return np.block([[xi, D], [np.conj(D), hole]]).astype(np.complex128)
+
+
+def ed_two_energy_response(H, Jl, Jr, *, beta, eta, omega=0.0, Ef=0.0):
+    """Dense Lehmann sum of the two-energy response
+    (test/ed_reference.jl:600-632)."""
+    E, V = np.linalg.eigh(np.asarray(H.toarray() if sp.issparse(H) else H))
+    Jl = V.conj().T @ np.asarray(Jl.toarray() if sp.issparse(Jl) else Jl) @ V
+    Jr = V.conj().T @ np.asarray(Jr.toarray() if sp.issparse(Jr) else Jr) @ V
+
+    def fermi(e):
+        if np.isinf(beta):
+            return (float(e < Ef) + float(e <= Ef)) / 2
+        return 1 / (np.exp(beta * (e - Ef)) + 1)
+
+    occ = np.array([fermi(e) for e in E])
+    acc = 0j
+    for p in range(len(E)):
+        for q in range(len(E)):
+            dE = E[p] - E[q]
+            if eta == 0:
+                if abs(dE) < 1e-8:
+                    fm = fermi((E[p] + E[q]) / 2)
+                    dd = -beta * fm * (1 - fm)
+                else:
+                    dd = (occ[p] - occ[q]) / dE
+            else:
+                dd = (occ[p] - occ[q]) / (omega + dE + 1j * eta)
+            acc += Jl[p, q] * Jr[q, p] * dd
+    return complex(acc)
+
+
+def ed_diamagnetic(H, Dhat, *, beta):
+    """sum_n f(E_n) <n|Dhat|n> at Fermi level zero
+    (test/ed_reference.jl:634-651)."""
+    E, V = np.linalg.eigh(np.asarray(H.toarray() if sp.issparse(H) else H))
+    Dd = V.conj().T @ np.asarray(Dhat.toarray() if sp.issparse(Dhat) else Dhat) @ V
+    f = 1 / (np.exp(beta * E) + 1)
+    return float(np.real(np.sum(f * np.diag(Dd))))
+
+
+def bdg_peierls_matrix(h, pos, disp, A, *, q=None, dir=1, convention="singlet"):
+    """Kinetic Nambu part of the Peierls-coupled BdG matrix with a real
+    modulation cos(q.m) (test/ed_reference.jl:692-733)."""
+    hc = sp.coo_matrix(h)
+    N = h.shape[0]
+    hp = np.zeros((N, N), dtype=np.complex128)
+    hh = np.zeros((N, N), dtype=np.complex128)
+    for i, j, v in zip(hc.row, hc.col, hc.data):
+        d = disp(i, j)
+        m = pos[j, :] + d / 2
+        u = 1.0 if q is None else np.cos(np.dot(q, m))
+        hp[i, j] = v * np.exp(1j * A * d[dir - 1] * u)
+        hole = np.conj(v) if convention == "singlet" else v
+        hh[i, j] = -hole * np.exp(-1j * A * d[dir - 1] * u)
+    Z = np.zeros((N, N), dtype=np.complex128)
+    return np.block([[hp, Z], [Z, hh]])
+
+
+def ed_bdg_free_energy(h, pos, disp, mu, U, n, Delta, A, *, q=None, dir=1, beta, convention="singlet"):
+    """F(A) = -(1/beta) sum_n log(1 + e^{-beta E_n(A)}) of the Peierls-coupled
+    kinetic matrix plus the local mu, Hartree and pairing terms (rigid Delta)
+    (ed_bdg_free_energy, test/ed_reference.jl:735-810)."""
+    N = h.shape[0]
+    Dm = np.diag(np.asarray(Delta, dtype=np.complex128)) if np.ndim(Delta) == 1 else \
+        np.asarray(Delta, dtype=np.complex128)
+    xi0 = -mu * np.eye(N) - np.diag(np.asarray(U) * np.asarray(n) / 2)
+    hole0 = -np.conj(xi0) if convention == "singlet" else -xi0
+    local = np.block([[xi0, Dm], [Dm.conj().T, hole0]])
+    H = bdg_peierls_matrix(h, pos, disp, A, q=q, dir=dir, convention=convention) + local
+    E = np.linalg.eigvalsh(H)
+    x = -beta * E
+    sp_ = np.where(x > 0, x + np.log1p(np.exp(-np.abs(x))), np.log1p(np.exp(np.minimum(x, 0))))
+    return float(-np.sum(sp_) / beta)

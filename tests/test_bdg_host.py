@@ -1,0 +1,203 @@
+"""CPU pins of the BdG / stiffness host layer (no GPU).
+
+- BdGOperator validation and bdg_assemble against the dense ED matrices
+  (test/ed_reference.jl:576-598), both hole conventions;
+- gershgorin_bound as a certified bound;
+- the Nambu vertices:
+  - the adjoint identity J(q)^H = -J(-q) and the q = 0 bond current
+    (test/stiffness_test.jl:246-266);
+  - finite differences of the Peierls-coupled matrix (:268-289);
+  - the diamagnetic vertex's hermiticity and conventions (:568-604);
+- two_energy_response against the dense Lehmann sum (:39-244), on tables
+  from the oracle's kpm_2d; the typed wrapper and argument errors.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import kpmhip as K
+from kpmhip import models
+import oracle as O
+
+
+def sq33():
+    return models.square_model(3, 3)
+
+
+QY = np.array([0.0, 2 * math.pi / 3])
+
+
+def test_bdg_assemble_vs_dense_both_conventions():
+    h, pos, disp = sq33()
+    dfield = np.full(9, 0.45 + 0.2j)
+    op = K.BdGOperator(h, mu=-0.7, U=2.0, n=np.full(9, 0.4), Delta=dfield)
+    A = K.bdg_assemble(op).toarray()
+    np.testing.assert_allclose(A, O.bdg_matrix(h, -0.7, np.full(9, 2.0), np.full(9, 0.4), dfield),
+                               atol=1e-15)
+    hf = models.flux_ring_model(4, phi=0.35)
+    opc = K.BdGOperator(hf, mu=0.2, U=np.array([1.0, 0.5, 0.0, 2.0]), n=np.array([0.1, 0.2, 0.3, 0.4]),
+                        Delta=np.array([0.3, 0.1j, -0.2, 0.05]), hole_convention="singlet")
+    ref = O.bdg_matrix(hf, 0.2, [1.0, 0.5, 0.0, 2.0], [0.1, 0.2, 0.3, 0.4], [0.3, 0.1j, -0.2, 0.05],
+                       convention="singlet")
+    np.testing.assert_allclose(K.bdg_assemble(opc).toarray(), ref, atol=1e-15)
+    assert np.allclose(ref, ref.conj().T)
+    # a general pairing matrix D: [xi D; D^H hole]
+    rng = np.random.default_rng(2)
+    Dm = sp.csc_matrix(rng.standard_normal((9, 9)) * (rng.random((9, 9)) < 0.3))
+    opD = K.BdGOperator(h, mu=0.1, U=0.0, D=Dm)
+    Ad = K.bdg_assemble(opD).toarray()
+    np.testing.assert_allclose(Ad[:9, 9:], Dm.toarray(), atol=0)
+    np.testing.assert_allclose(Ad[9:, :9], Dm.toarray().conj().T, atol=0)
+    with pytest.raises(K.ArgumentError):
+        opD.Delta
+
+
+def test_bdg_validation_errors():
+    h, _, _ = sq33()
+    hf = models.flux_ring_model(4)
+    with pytest.raises(K.ArgumentError):
+        K.BdGOperator(hf, mu=0.0, U=1.0)  # complex h, intervalley, not asserted
+    K.BdGOperator(hf, mu=0.0, U=1.0, assume_intervalley=True)
+    with pytest.raises(K.ArgumentError):
+        K.BdGOperator(hf, mu=0.0, U=1.0, hole_convention="conjugate", assume_intervalley=True)
+    with pytest.raises(K.ArgumentError):
+        K.BdGOperator(sp.csc_matrix(np.triu(np.ones((3, 3)))), mu=0.0, U=1.0)  # not Hermitian
+    with pytest.raises(K.ArgumentError):
+        K.BdGOperator(h, mu=0.0, U=np.ones(4))
+    with pytest.raises(K.ArgumentError):
+        K.BdGOperator(h, mu=0.0, U=1.0, Delta=np.zeros(9), D=sp.identity(9))
+    with pytest.raises(K.ArgumentError):
+        K.BdGOperator(h, mu=0.0, U=1.0, h_hole=h)
+    with pytest.raises(K.ArgumentError):
+        K.BdGOperator(hf, mu=0.0, U=1.0, hole_convention="conjugate", h_hole=hf)  # not conj(h)
+    with pytest.raises(K.ArgumentError):
+        K.BdGOperator(h, mu=0.0, U=1.0, hole_convention="nambu")
+
+
+def test_gershgorin_is_a_bound():
+    h, _, _ = sq33()
+    for Delta in (np.zeros(9), np.full(9, 0.45 + 0.2j), np.linspace(-1, 1, 9) + 0.3j):
+        op = K.BdGOperator(h, mu=-0.7, U=2.0, n=np.full(9, 0.4), Delta=Delta)
+        g = K.gershgorin_bound(op)
+        ev = np.linalg.eigvalsh(K.bdg_assemble(op).toarray())
+        assert g >= np.abs(ev).max() - 1e-12
+        # onsite pairing: 4 bonds of |t| = 1 + |0 - mu - U n/2| + |Delta_i|
+        assert abs(g - np.max(4 + abs(0.7 - 0.4) + np.abs(Delta))) < 1e-12
+
+
+def test_vertex_adjoint_identity_and_bond_current():
+    h, pos, disp = sq33()
+    for q in (np.zeros(2), QY, np.array([2 * math.pi / 3, 4 * math.pi / 3])):
+        Jq = K.nambu_current_q(h, pos, q, dir=1, disp=disp)
+        Jmq = K.nambu_current_q(h, pos, -q, dir=1, disp=disp)
+        np.testing.assert_allclose(Jq.conj().T.toarray(), -Jmq.toarray(), atol=1e-13)
+    J0 = K.nambu_current_q(h, pos, np.zeros(2), dir=1, disp=disp).toarray()
+    np.testing.assert_allclose(J0[:9, :9], J0[9:, 9:], atol=1e-13)
+    man = np.zeros((9, 9), dtype=complex)
+    hc = sp.coo_matrix(h)
+    for i, j, v in zip(hc.row, hc.col, hc.data):
+        man[i, j] = v * disp(i, j)[0]
+    np.testing.assert_allclose(J0[:9, :9], man, atol=1e-13)
+
+
+def test_vertex_finite_difference_vs_peierls():
+    h, pos, disp = sq33()
+    dl = 1e-6
+    J0 = K.nambu_current_q(h, pos, np.zeros(2), dir=1, disp=disp).toarray()
+    FD = (O.bdg_peierls_matrix(h, pos, disp, dl) - O.bdg_peierls_matrix(h, pos, disp, -dl)) / (2j * dl)
+    np.testing.assert_allclose(FD, J0, atol=1e-7)
+    FDm = (O.bdg_peierls_matrix(h, pos, disp, dl, q=QY) -
+           O.bdg_peierls_matrix(h, pos, disp, -dl, q=QY)) / (2j * dl)
+    Jq = K.nambu_current_q(h, pos, QY, dir=1, disp=disp)
+    Jmq = K.nambu_current_q(h, pos, -QY, dir=1, disp=disp)
+    np.testing.assert_allclose(FDm, ((Jq + Jmq) / 2).toarray(), atol=1e-7)
+    H0 = O.bdg_peierls_matrix(h, pos, disp, 0.0)
+    np.testing.assert_allclose(H0, O.bdg_matrix(h, 0.0, np.zeros(9), np.zeros(9), np.zeros(9)), atol=1e-13)
+    # the intervalley vertex differentiates the same-h hole block
+    hf = models.flux_ring_model(4, phi=0.35)
+    pf, df = models.ring_geometry(4)
+    FDi = (O.bdg_peierls_matrix(hf, pf, df, dl, convention="intervalley") -
+           O.bdg_peierls_matrix(hf, pf, df, -dl, convention="intervalley")) / (2j * dl)
+    Ji = K.nambu_current_q(hf, pf, np.zeros(2), dir=1, disp=df, hole_convention="intervalley")
+    np.testing.assert_allclose(FDi, Ji.toarray(), atol=1e-7)
+    Js = K.nambu_current_q(hf, pf, np.zeros(2), dir=1, disp=df, hole_convention="singlet")
+    assert np.abs((Js - Ji).toarray()).max() > 1e-3
+
+
+def test_diamagnetic_vertex():
+    h, pos, disp = sq33()
+    Dh = K.nambu_diamagnetic(h, pos, disp=disp)
+    np.testing.assert_allclose(Dh.toarray(), Dh.conj().T.toarray(), atol=1e-13)
+    assert (Dh != K.nambu_diamagnetic(h, pos, disp=disp, hole_convention="intervalley")).nnz == 0
+    # second Peierls derivative of the kinetic block
+    dl = 1e-4
+    FD2 = (O.bdg_peierls_matrix(h, pos, disp, dl) - 2 * O.bdg_peierls_matrix(h, pos, disp, 0.0) +
+           O.bdg_peierls_matrix(h, pos, disp, -dl)) / dl ** 2
+    np.testing.assert_allclose(FD2, Dh.toarray(), atol=1e-6)
+    hf = models.flux_ring_model(4, phi=0.35)
+    pf, df = models.ring_geometry(4)
+    a = K.nambu_diamagnetic(hf, pf, disp=df, hole_convention="singlet")
+    b = K.nambu_diamagnetic(hf, pf, disp=df, hole_convention="intervalley")
+    assert np.abs((a - b).toarray()).max() > 1e-3
+
+
+def stiff_system():
+    rng = np.random.default_rng(42)
+    D = 6
+    A = rng.standard_normal((D, D)) + 1j * rng.standard_normal((D, D))
+    H0 = (A + A.conj().T) / 2
+    JL = rng.standard_normal((D, D)) + 1j * rng.standard_normal((D, D))
+    JR = rng.standard_normal((D, D)) + 1j * rng.standard_normal((D, D))
+    ev = np.linalg.eigvalsh(H0)
+    b0 = 0.2
+    a0 = np.abs(ev - b0).max() / 0.9
+    HN = (H0 - b0 * np.eye(D)) / a0
+    return D, H0, JL, JR, a0, b0, HN
+
+
+def oracle_2d(HN, JL, JR, NC, D):
+    psi = np.eye(D, dtype=complex)
+    return O.kpm_2d(sp.csc_matrix(HN), sp.csc_matrix(JL), sp.csc_matrix(JR), NC, psi, psi, NC, NC)
+
+
+def test_two_energy_response_vs_lehmann():
+    """The reference pins NC = 256 on its own seeded instance; on this one
+    (a = 3.9, an eigenvalue 0.11 below E_f) the same tolerances hold from
+    NC = 1024, and the error falls monotonically (as 1/NC^2) towards the
+    Lehmann sum."""
+    D, H0, JL, JR, a0, b0, HN = stiff_system()
+    mus = {nc: oracle_2d(HN, JL, JR, nc, D) for nc in (64, 256, 1024)}
+    mu256 = mus[1024]
+    kw = dict(b=b0, beta=8.0, eta=0.25, omega=0.3, Ef=0.15, NH=D, volume=1.0)
+    ed = O.ed_two_energy_response(H0, JL, JR, beta=8.0, eta=0.25, omega=0.3, Ef=0.15)
+    err = {nc: abs(K.two_energy_response(mu, a0, **kw) - ed) for nc, mu in mus.items()}
+    assert err[1024] < err[256] < err[64]
+    assert err[256] / err[1024] > 8  # second-order convergence in 1/NC
+    assert err[1024] <= 0.015 * abs(ed)
+    # static eta = 0 divided difference; Hermitian vertices give a real result
+    JLh, JRh = (JL + JL.conj().T) / 2, (JR + JR.conj().T) / 2
+    ks = dict(b=b0, beta=8.0, eta=0.0, omega=0.0, Ef=0.15, NH=D, volume=1.0)
+    pg = K.two_energy_response(mu256, a0, **ks)
+    eg = O.ed_two_energy_response(H0, JL, JR, beta=8.0, eta=0.0, omega=0.0, Ef=0.15)
+    assert abs(pg - eg) <= 0.012 * abs(eg)
+    ph = K.two_energy_response(oracle_2d(HN, JLh, JRh, 1024, D), a0, **ks)
+    eh = O.ed_two_energy_response(H0, JLh, JRh, beta=8.0, eta=0.0, omega=0.0, Ef=0.15)
+    assert abs(ph.imag) < 1e-10 and abs(eh.imag) < 1e-10
+    assert abs(ph - eh) <= 0.05 * abs(eh)
+    with pytest.raises(K.ArgumentError):
+        K.two_energy_response(mu256, a0, b=b0, beta=8.0, eta=0.0, omega=0.3, NH=D, volume=1.0)
+    with pytest.raises(K.ArgumentError):
+        K.two_energy_response(mu256, a0, b=b0, beta=math.inf, eta=0.0, omega=0.0, NH=D, volume=1.0)
+    # typed wrapper (src/frontend.jl:1083-1105) and beta = inf
+    m = K.ConductivityMoments(mu256, a0, b0, D, D)
+    raw = K.two_energy_response(mu256, a0, **kw)
+    assert K.two_energy_response(m, beta=8.0, eta=0.25, omega=0.3, Ef=0.15, volume=1.0) == raw
+    with pytest.raises(K.ArgumentError):
+        K.two_energy_response(m, beta=8.0, eta=0.25, volume=1.0, b=b0)
+    with pytest.raises(K.ArgumentError):
+        K.two_energy_response(m, beta=8.0, eta=0.25, volume=1.0, NH=D)
+    pinf = K.two_energy_response(m, beta=math.inf, eta=0.25, omega=0.3, Ef=0.15, volume=1.0)
+    einf = O.ed_two_energy_response(H0, JL, JR, beta=math.inf, eta=0.25, omega=0.3, Ef=0.15)
+    assert np.isfinite(pinf) and abs(pinf - einf) <= 0.02 * abs(einf)
