@@ -418,7 +418,7 @@ def bench_conductivity(args) -> int:
     tuned = K.autotune(Hd, NR, "f64")
     print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
     out = {}
-    for gemm in ("f64", "f32"):
+    for gemm in ("f64", "f32", "bf16x3"):
         K.kpm_2d(Hd, Jxd, Jyd, NC, NR, psi_in=psi, gemm=gemm)  # warm-up
         ctx.synchronize()
         t0 = time.perf_counter()
@@ -435,14 +435,15 @@ def bench_conductivity(args) -> int:
     flops = 8.0 * NC * NC * NH * NR
     Kc = NH * K.moment._padded_nr(NR, 0)
     rl = {}
-    for gemm, code, peak in (("f64", 0, 78.6), ("f32", 1, 157.3)):
+    for gemm, code, peak in (("f64", 0, 78.6), ("f32", 1, 157.3), ("bf16x3", 2, 2500.0 / 3)):
         ms = ctypes.c_float(0.0)
         K._lib.check(ctx.lib.kpm_bench_contract(ctx.handle, NC, NC, Kc, 0, 0, code, 3, ctypes.byref(ms)),
                      "bench_contract")
         t = ms.value / 3 * 1e-3
         rl[gemm] = {"bound": "mfma", "achieved": round(flops / t / 1e12, 2), "peak": peak,
                     "unit": "TFLOP/s", "frac": round(flops / t / 1e12 / peak, 4), "traffic": None,
-                    "kernel": f"k_contract<double,{'double' if code == 0 else 'float'}>",
+                    "kernel": ("k_contract_bf16x3<double>" if code == 2 else
+                               f"k_contract<double,{'double' if code == 0 else 'float'}>"),
                     "avg_launch_ms": round(t * 1e3, 3)}
         print(f"[bench] contraction {gemm}: {t * 1e3:.2f} ms, {flops / t / 1e12:.1f} TF/s",
               file=sys.stderr, flush=True)
@@ -452,9 +453,10 @@ def bench_conductivity(args) -> int:
            "vs_baseline": None, "dtype": "c128", "data": "synthetic",
            "config": {"workload": f"Haldane {Lc}x{Lc} kpm_2d", "NH": NH, "NC": NC, "NR": NR,
                       "a": a, "contraction_flops": flops},
-           "roofline": rl["f64"], "roofline_fp32_gemm": rl["f32"],
-           "fp64_gemm": out["f64"], "fp32_gemm": out["f32"],
-           "sigma_xy_fp64_minus_fp32": out["f64"]["sigma_xy"] - out["f32"]["sigma_xy"]}
+           "roofline": rl["f64"], "roofline_fp32_gemm": rl["f32"], "roofline_bf16x3_gemm": rl["bf16x3"],
+           "fp64_gemm": out["f64"], "fp32_gemm": out["f32"], "bf16x3_gemm": out["bf16x3"],
+           "sigma_xy_fp64_minus_fp32": out["f64"]["sigma_xy"] - out["f32"]["sigma_xy"],
+           "sigma_xy_fp64_minus_bf16x3": out["f64"]["sigma_xy"] - out["bf16x3"]["sigma_xy"]}
     print(json.dumps(res))
     return 0
 

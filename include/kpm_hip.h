@@ -45,9 +45,12 @@ extern "C" {
 #define KPM_VEC_F64 0
 #define KPM_VEC_F32 1
 
-/* kpm_2d contraction arithmetic */
+/* kpm_2d contraction arithmetic: FP64 MFMA; FP32 MFMA (FP64 promotion every
+ * 256 reals of K); 3-pass split BF16 (hi*hi + hi*lo + lo*hi on BF16 MFMA,
+ * FP32 accumulation promoted to FP64 every 256 reals) */
 #define KPM_GEMM_F64 0
 #define KPM_GEMM_F32 1
+#define KPM_GEMM_BF16X3 2
 
 /* moment_parity of kpm_2d! (src/moment.jl:913-915,1067-1069) */
 #define KPM_PARITY_NONE 0

@@ -134,7 +134,8 @@ static int validate_2d(const kpm_csr* H, const kpm_csr* Ja, const kpm_csr* Jb, c
   if (p.K_right < 1) return fail(KPM_EINVAL, "JTnHJψr must have at least one slot");
   if (p.K_left > INT32_MAX || p.K_right > INT32_MAX) return fail(KPM_EINVAL, "block too large");
   if (p.prec != KPM_VEC_F64 && p.prec != KPM_VEC_F32) return fail(KPM_EINVAL, "bad vec_prec");
-  if (p.gemm_mode != KPM_GEMM_F64 && p.gemm_mode != KPM_GEMM_F32) return fail(KPM_EINVAL, "bad gemm_mode");
+  if (p.gemm_mode != KPM_GEMM_F64 && p.gemm_mode != KPM_GEMM_F32 && p.gemm_mode != KPM_GEMM_BF16X3)
+    return fail(KPM_EINVAL, "bad gemm_mode");
   return KPM_OK;
 }
 

@@ -227,6 +227,179 @@ __global__ void __launch_bounds__(256)
       }
 }
 
+// ---- 3-pass split-BF16 mode ------------------------------------------------
+// Each operand real x is split as hi = bf16(x), lo = bf16(x - hi) (round to
+// nearest even, from the FP32 value of x), and every product as
+//   x y ~ hi_x hi_y + hi_x lo_y + lo_x hi_y
+// (the lo lo term, ~2^-16 relative, is dropped) on v_mfma_f32_16x16x32_bf16:
+// one instruction per 16 x 16 tile consumes a whole 32-real stage (8 reals
+// per lane group), so a stage costs 6 MFMAs per tile pair (3 Re, 3 Im) where
+// the FP32 mode needs 16.  FP32 accumulators are promoted into FP64 every 8
+// stages, as in the FP32 mode.  The tile scheme (64 x 64 per workgroup, LDS
+// double buffering, XCD-aware K-chunk dispatch) is k_contract's; the LDS holds
+// the hi and lo planes as bf16 pairs.
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ uint32_t bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+
+// (hi, lo) bf16 bit patterns of x
+__device__ __forceinline__ void split_bf16(float x, uint32_t& hi, uint32_t& lo) {
+  hi = bf16_rne(x);
+  lo = bf16_rne(x - __uint_as_float(hi << 16));
+}
+
+// L'' = (-Li, Lr) on packed (Re, Im) bf16 pairs: swap the halves, negate the new low half
+__device__ __forceinline__ uint4 swap_neg(uint4 w) {
+  auto f = [](uint32_t x) { return ((x >> 16) ^ 0x8000u) | (x << 16); };
+  return make_uint4(f(w.x), f(w.y), f(w.z), f(w.w));
+}
+
+__device__ __forceinline__ f4 mfma_bf16(uint4 a, uint4 b, f4 c) {
+  bf8 av, bv;
+  __builtin_memcpy(&av, &a, 16);
+  __builtin_memcpy(&bv, &b, 16);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, c, 0, 0, 0);
+}
+
+template <typename VT>
+__global__ void __launch_bounds__(256)
+    k_contract_bf16x3(const VT* __restrict__ L, int nL, const VT* __restrict__ R, int nR, int64_t K2,
+                      int64_t P2, int64_t kchunk, int tm, int ntiles, int splits,
+                      double2* __restrict__ part, int ldn, int ldm) {
+  constexpr int LDB = kStep / 2 + 4;                // LDS row stride in 32-bit bf16 pairs (+16 B pad)
+  constexpr int WPR = kStep * (int)sizeof(VT) / 16; // 16-byte global words per row segment
+  constexpr int EPW = 16 / (int)sizeof(VT);         // VT elements per 16-byte word
+  constexpr int NW = 2 * 64 * WPR / 256;            // global words per thread per stage (A+B)
+  __shared__ uint32_t lhi[2][2][64 * LDB], llo[2][2][64 * LDB];
+
+  int tile, z;
+  if ((splits & 7) == 0) {
+    const int b = blockIdx.x, local = b >> 3;
+    tile = local % ntiles;
+    z = (local / ntiles) * 8 + (b & 7);
+  } else {
+    tile = blockIdx.x % ntiles;
+    z = blockIdx.x / ntiles;
+  }
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int g = lane >> 4, i = lane & 15;
+  const int tile_n = (tile / tm) * 64, tile_m = (tile % tm) * 64;
+  const int n0 = (wave >> 1) * 32, m0 = (wave & 1) * 32;
+  const int64_t kbeg = (int64_t)z * kchunk;
+  const int64_t kend = min(kbeg + kchunk, K2);
+  const int nstage = kend > kbeg ? (int)((kend - kbeg) / kStep) : 0;
+
+  const VT* src[NW];
+#pragma unroll
+  for (int q = 0; q < NW; ++q) {
+    const int w = tid + 256 * q;
+    const int op = w / (64 * WPR), rem = w % (64 * WPR);
+    const int row = rem / WPR, wi = rem % WPR;
+    const int grow = op == 0 ? min(tile_n + row, nR - 1) : min(tile_m + row, nL - 1);
+    src[q] = (op == 0 ? R : L) + (size_t)grow * P2 + kbeg + wi * EPW;
+  }
+  auto gload = [&](int stage, float4* regs) {
+#pragma unroll
+    for (int q = 0; q < NW; ++q) regs[q] = *reinterpret_cast<const float4*>(src[q] + (int64_t)stage * kStep);
+  };
+  auto lstore = [&](int buf, const float4* regs) {
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+      const int w = tid + 256 * q;
+      const int op = w / (64 * WPR), rem = w % (64 * WPR);
+      const int row = rem / WPR, wi = rem % WPR;
+      const VT* e = reinterpret_cast<const VT*>(&regs[q]);
+      uint32_t* ph = &lhi[buf][op][row * LDB + wi * EPW / 2];
+      uint32_t* pl = &llo[buf][op][row * LDB + wi * EPW / 2];
+#pragma unroll
+      for (int t = 0; t < EPW / 2; ++t) {
+        uint32_t h0, l0, h1, l1;
+        split_bf16((float)e[2 * t], h0, l0);
+        split_bf16((float)e[2 * t + 1], h1, l1);
+        ph[t] = h0 | (h1 << 16);
+        pl[t] = l0 | (l1 << 16);
+      }
+    }
+  };
+
+  f4 are[2][2], aim[2][2];
+  double fre[2][2][4], fim[2][2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      are[a][b] = aim[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int v = 0; v < 4; ++v) fre[a][b][v] = fim[a][b][v] = 0.0;
+    }
+  auto flush = [&]() {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          fre[a][b][v] += (double)are[a][b][v];
+          fim[a][b][v] += (double)aim[a][b][v];
+        }
+        are[a][b] = aim[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+      }
+  };
+  constexpr int kFlush = 8;
+
+  float4 regs[NW];
+  if (nstage > 0) {
+    gload(0, regs);
+    lstore(0, regs);
+  }
+  __syncthreads();
+  for (int st = 0; st < nstage; ++st) {
+    const int buf = st & 1;
+    if (st + 1 < nstage) gload(st + 1, regs);
+    uint4 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int ra = (n0 + 16 * t + i) * LDB + 4 * g, rb = (m0 + 16 * t + i) * LDB + 4 * g;
+      ah[t] = *reinterpret_cast<const uint4*>(&lhi[buf][0][ra]);
+      al[t] = *reinterpret_cast<const uint4*>(&llo[buf][0][ra]);
+      bh[t] = *reinterpret_cast<const uint4*>(&lhi[buf][1][rb]);
+      bl[t] = *reinterpret_cast<const uint4*>(&llo[buf][1][rb]);
+    }
+#pragma unroll
+    for (int tmi = 0; tmi < 2; ++tmi) {
+      const uint4 b2h = swap_neg(bh[tmi]), b2l = swap_neg(bl[tmi]);
+#pragma unroll
+      for (int tn = 0; tn < 2; ++tn) {
+        are[tn][tmi] = mfma_bf16(al[tn], bh[tmi], are[tn][tmi]);
+        are[tn][tmi] = mfma_bf16(ah[tn], bl[tmi], are[tn][tmi]);
+        are[tn][tmi] = mfma_bf16(ah[tn], bh[tmi], are[tn][tmi]);
+        aim[tn][tmi] = mfma_bf16(al[tn], b2h, aim[tn][tmi]);
+        aim[tn][tmi] = mfma_bf16(ah[tn], b2l, aim[tn][tmi]);
+        aim[tn][tmi] = mfma_bf16(ah[tn], b2h, aim[tn][tmi]);
+      }
+    }
+    if ((st % kFlush) == kFlush - 1) flush();
+    if (st + 1 < nstage) lstore(buf ^ 1, regs);
+    __syncthreads();
+  }
+  flush();
+  double2* out = part + (size_t)z * ldn * ldm;
+#pragma unroll
+  for (int tn = 0; tn < 2; ++tn)
+#pragma unroll
+    for (int tmi = 0; tmi < 2; ++tmi)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int n = tile_n + n0 + 16 * tn + 4 * g + v;  // f32 16x16 accumulator rows
+        const int m = tile_m + m0 + 16 * tmi + i;
+        if (n < nR && m < nL) out[(size_t)n * ldm + m] = make_double2(fre[tn][tmi][v], fim[tn][tmi][v]);
+      }
+}
+
 __global__ void k_contract_reduce(const double2* __restrict__ part, int splits, int nR, int nL,
                                   int ldn, int ldm, double scale, double2* __restrict__ mu,
                                   int64_t NC, int64_t n0, int64_t m0, int parity) {
@@ -274,7 +447,16 @@ int launch_contract(kpm_ctx* ctx, const void* Lslots, int nL, const void* Rslots
   KPM_TRY(ensure(ctx, ctx->gemm_part, (size_t)splits * ldn * ldm * sizeof(double2)));
   const dim3 grid((unsigned)(ntiles * splits));
   const int sp = (int)splits;
-  if (prec == KPM_VEC_F64) {
+  if (gemm_mode == KPM_GEMM_BF16X3) {
+    if (prec == KPM_VEC_F64)
+      hipLaunchKernelGGL((k_contract_bf16x3<double>), grid, dim3(256), 0, ctx->stream,
+                         (const double*)Lslots, nL, (const double*)Rslots, nR, K2, P2, kchunk, tm,
+                         ntiles, sp, (double2*)ctx->gemm_part.ptr, ldn, ldm);
+    else
+      hipLaunchKernelGGL((k_contract_bf16x3<float>), grid, dim3(256), 0, ctx->stream,
+                         (const float*)Lslots, nL, (const float*)Rslots, nR, K2, P2, kchunk, tm,
+                         ntiles, sp, (double2*)ctx->gemm_part.ptr, ldn, ldm);
+  } else if (prec == KPM_VEC_F64) {
     if (gemm_mode == KPM_GEMM_F32)
       hipLaunchKernelGGL((k_contract<double, float>), grid, dim3(256), 0, ctx->stream,
                          (const double*)Lslots, nL, (const double*)Rslots, nR, K2, P2, kchunk, tm,
@@ -315,7 +497,8 @@ int kpm_bench_contract(kpm_ctx* ctx, int64_t nL, int64_t nR, int64_t K, int64_t 
   if (pitch == 0) pitch = K + 256 / elem_bytes(vec_prec == KPM_VEC_F32 ? KPM_VEC_F32 : KPM_VEC_F64);
   if (pitch < K || pitch % 8 != 0) return fail(KPM_EINVAL, "pitch must be >= K and a multiple of 8");
   if (vec_prec != KPM_VEC_F64 && vec_prec != KPM_VEC_F32) return fail(KPM_EINVAL, "bad vec_prec");
-  if (gemm_mode != KPM_GEMM_F64 && gemm_mode != KPM_GEMM_F32) return fail(KPM_EINVAL, "bad gemm_mode");
+  if (gemm_mode != KPM_GEMM_F64 && gemm_mode != KPM_GEMM_F32 && gemm_mode != KPM_GEMM_BF16X3)
+    return fail(KPM_EINVAL, "bad gemm_mode");
   KPM_HIP_TRY(hipSetDevice(ctx->device));
   const size_t eb = (size_t)elem_bytes(vec_prec);
   const int64_t NC = std::max(nL, nR);

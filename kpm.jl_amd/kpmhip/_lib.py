@@ -26,6 +26,7 @@ VEC_F64 = 0
 VEC_F32 = 1
 GEMM_F64 = 0
 GEMM_F32 = 1
+GEMM_BF16X3 = 2
 PARITY = {"NONE": 0, "ODD": 1, "EVEN": 2}
 
 

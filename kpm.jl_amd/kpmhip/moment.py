@@ -328,6 +328,8 @@ def _padded_nr(NR: int, prec: int) -> int:
 
 
 _PARITY = {"NONE": 0, "ODD": 1, "EVEN": 2}
+# kpm_2d contraction arithmetic: FP64 MFMA, FP32 MFMA, or 3-pass split BF16
+_GEMM_MODES = {"f64": _lib.GEMM_F64, "f32": _lib.GEMM_F32, "bf16x3": _lib.GEMM_BF16X3}
 
 
 def _parity_code(moment_parity) -> int:
@@ -359,9 +361,9 @@ def kpm_2d_(H, Ja, Jb, NC: int, NR: int, NH: int, mu: np.ndarray, psi_in_l=None,
     ctx = get_context()
     Hd, Jad, Jbd = to_device(H, ctx), to_device(Ja, ctx), to_device(Jb, ctx)
     out = np.zeros((NC, NC), dtype=np.complex128, order="F")
-    gm = {"f64": _lib.GEMM_F64, "f32": _lib.GEMM_F32}.get(str(gemm).lower())
+    gm = _GEMM_MODES.get(str(gemm).lower())
     if gm is None:
-        raise ArgumentError(f"gemm must be 'f64' or 'f32' (got {gemm!r})")
+        raise ArgumentError(f"gemm must be 'f64', 'f32' or 'bf16x3' (got {gemm!r})")
     check(ctx.lib.kpm_2d(ctx.handle, Hd.handle, Jad.handle, Jbd.handle, NC, NR, NH,
                          dptr(psl), dptr(psr), K_left, K_right,
                          parity, 1 if mn_sym else 0, prec_code(precision), gm,
@@ -419,7 +421,7 @@ def kpm_3d_(H, Ja, Jb, Jg, NC: int, NR: int, NH: int, mu: np.ndarray, psi_in_l, 
     ctx = get_context()
     Hd, Jad, Jbd, Jgd = (to_device(X, ctx) for X in (H, Ja, Jb, Jg))
     out = np.zeros((NC, NC, NC), dtype=np.complex128, order="F")
-    gm = {"f64": _lib.GEMM_F64, "f32": _lib.GEMM_F32}[str(gemm).lower()]
+    gm = _GEMM_MODES[str(gemm).lower()]
     check(ctx.lib.kpm_3d(ctx.handle, Hd.handle, Jad.handle, Jbd.handle, Jgd.handle, NC, NR, NH,
                          dptr(psl), dptr(psr), int(arr_size),
                          right_block, prec_code(precision), gm, dptr(out)),

@@ -276,8 +276,27 @@ def test_fp32_kpm2d_gemm_modes():
     NR, NC = 4, 32
     pl = phases(3, H.shape[0], NR)
     ref = oracle.kpm_2d(Hn, Jx, Jy, NC, pl, pl, NC, NC)
-    for prec, gemm, tol in (("f64", "f32", 2e-5), ("f32", "f32", 2e-5), ("f32", "f64", 2e-5)):
+    for prec, gemm, tol in (("f64", "f32", 2e-5), ("f32", "f32", 2e-5), ("f32", "f64", 2e-5),
+                            ("f64", "bf16x3", 2e-4), ("f32", "bf16x3", 2e-4)):
         got = K.kpm_2d(Hn, Jx, Jy, NC, NR, psi_in=pl, precision=prec, gemm=gemm)
         err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
         print(f"kpm_2d prec={prec} gemm={gemm} rel err={err:.2e}", flush=True)
         assert err < tol
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_bf16x3_contraction_long_k(prec):
+    """The split-BF16 contraction over many K stages and split-K chunks
+    (Haldane 32 x 32, NR 16: 32,768 complex per slot) against the FP64 MFMA
+    contraction of the same slots: error at the 2^-16 level of the 3-pass
+    split (above FP32 rounding, so the BF16 path is the one that ran), and
+    ragged NC (70: tiles of 64 plus a 6-wide remainder)."""
+    H, Jx, Jy, area = models.haldane_model(32, 32, t2=0.2)
+    Hn = (H / 4.0).tocsc()
+    NR, NC = 16, 70
+    pl = phases(11, H.shape[0], NR)
+    ref = K.kpm_2d(Hn, Jx, Jy, NC, NR, psi_in=pl, precision=prec, gemm="f64")
+    got = K.kpm_2d(Hn, Jx, Jy, NC, NR, psi_in=pl, precision=prec, gemm="bf16x3")
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    print(f"bf16x3 vs f64 contraction ({prec} slots): rel err {err:.2e}", flush=True)
+    assert 1e-9 < err < 1e-4
