@@ -400,6 +400,156 @@ __global__ void __launch_bounds__(256)
       }
 }
 
+// Large-table form of the split-BF16 contraction: a 128 x 128 workgroup tile,
+// four waves of 64 x 64 (4 x 4 MFMA tiles), so every operand fragment read
+// from LDS feeds 4 tiles instead of 2 and the LDS traffic per MFMA halves.
+// The MFMA accumulators hold FP32 sums over one K-chunk of at most
+// kChunkBF16 reals (no FP64 registers, which is what frees the room for the
+// larger tile); the host sizes the split so that chunks stay within that
+// bound, and the FP32 chunk partials are added in FP64 in a fixed order.
+constexpr int64_t kChunkBF16 = 8192;
+
+template <typename VT>
+__global__ void __launch_bounds__(256)
+    k_contract_bf16x3_t128(const VT* __restrict__ L, int nL, const VT* __restrict__ R, int nR,
+                           int64_t K2, int64_t P2, int64_t kchunk, int tm, int ntiles, int splits,
+                           float2* __restrict__ part, int ldn, int ldm) {
+  constexpr int TT = 128;
+  constexpr int LDB = kStep / 2 + 4;
+  constexpr int WPR = kStep * (int)sizeof(VT) / 16;
+  constexpr int EPW = 16 / (int)sizeof(VT);
+  constexpr int NW = 2 * TT * WPR / 256;
+  __shared__ uint32_t lhi[2][2][TT * LDB], llo[2][2][TT * LDB];
+
+  int tile, z;
+  if ((splits & 7) == 0) {
+    const int b = blockIdx.x, local = b >> 3;
+    tile = local % ntiles;
+    z = (local / ntiles) * 8 + (b & 7);
+  } else {
+    tile = blockIdx.x % ntiles;
+    z = blockIdx.x / ntiles;
+  }
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int g = lane >> 4, i = lane & 15;
+  const int tile_n = (tile / tm) * TT, tile_m = (tile % tm) * TT;
+  const int n0 = (wave >> 1) * 64, m0 = (wave & 1) * 64;
+  const int64_t kbeg = (int64_t)z * kchunk;
+  const int64_t kend = min(kbeg + kchunk, K2);
+  const int nstage = kend > kbeg ? (int)((kend - kbeg) / kStep) : 0;
+
+  const VT* src[NW];
+#pragma unroll
+  for (int q = 0; q < NW; ++q) {
+    const int w = tid + 256 * q;
+    const int op = w / (TT * WPR), rem = w % (TT * WPR);
+    const int row = rem / WPR, wi = rem % WPR;
+    const int grow = op == 0 ? min(tile_n + row, nR - 1) : min(tile_m + row, nL - 1);
+    src[q] = (op == 0 ? R : L) + (size_t)grow * P2 + kbeg + wi * EPW;
+  }
+  auto gload = [&](int stage, float4* regs) {
+#pragma unroll
+    for (int q = 0; q < NW; ++q) regs[q] = *reinterpret_cast<const float4*>(src[q] + (int64_t)stage * kStep);
+  };
+  auto lstore = [&](int buf, const float4* regs) {
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+      const int w = tid + 256 * q;
+      const int op = w / (TT * WPR), rem = w % (TT * WPR);
+      const int row = rem / WPR, wi = rem % WPR;
+      const VT* e = reinterpret_cast<const VT*>(&regs[q]);
+      uint32_t* ph = &lhi[buf][op][row * LDB + wi * EPW / 2];
+      uint32_t* pl = &llo[buf][op][row * LDB + wi * EPW / 2];
+#pragma unroll
+      for (int t = 0; t < EPW / 2; ++t) {
+        uint32_t h0, l0, h1, l1;
+        split_bf16((float)e[2 * t], h0, l0);
+        split_bf16((float)e[2 * t + 1], h1, l1);
+        ph[t] = h0 | (h1 << 16);
+        pl[t] = l0 | (l1 << 16);
+      }
+    }
+  };
+
+  f4 are[4][4], aim[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) are[a][b] = aim[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+
+  float4 regs[NW];
+  if (nstage > 0) {
+    gload(0, regs);
+    lstore(0, regs);
+  }
+  __syncthreads();
+  for (int st = 0; st < nstage; ++st) {
+    const int buf = st & 1;
+    if (st + 1 < nstage) gload(st + 1, regs);
+    uint4 ah[4], al[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int ra = (n0 + 16 * t + i) * LDB + 4 * g;
+      ah[t] = *reinterpret_cast<const uint4*>(&lhi[buf][0][ra]);
+      al[t] = *reinterpret_cast<const uint4*>(&llo[buf][0][ra]);
+    }
+#pragma unroll
+    for (int tmi = 0; tmi < 4; ++tmi) {
+      const int rb = (m0 + 16 * tmi + i) * LDB + 4 * g;
+      const uint4 bh = *reinterpret_cast<const uint4*>(&lhi[buf][1][rb]);
+      const uint4 bl = *reinterpret_cast<const uint4*>(&llo[buf][1][rb]);
+      const uint4 b2h = swap_neg(bh), b2l = swap_neg(bl);
+#pragma unroll
+      for (int tn = 0; tn < 4; ++tn) {
+        are[tn][tmi] = mfma_bf16(al[tn], bh, are[tn][tmi]);
+        are[tn][tmi] = mfma_bf16(ah[tn], bl, are[tn][tmi]);
+        are[tn][tmi] = mfma_bf16(ah[tn], bh, are[tn][tmi]);
+        aim[tn][tmi] = mfma_bf16(al[tn], b2h, aim[tn][tmi]);
+        aim[tn][tmi] = mfma_bf16(ah[tn], b2l, aim[tn][tmi]);
+        aim[tn][tmi] = mfma_bf16(ah[tn], b2h, aim[tn][tmi]);
+      }
+    }
+    if (st + 1 < nstage) lstore(buf ^ 1, regs);
+    __syncthreads();
+  }
+  float2* out = part + (size_t)z * ldn * ldm;
+#pragma unroll
+  for (int tn = 0; tn < 4; ++tn)
+#pragma unroll
+    for (int tmi = 0; tmi < 4; ++tmi)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int n = tile_n + n0 + 16 * tn + 4 * g + v;
+        const int m = tile_m + m0 + 16 * tmi + i;
+        if (n < nR && m < nL) out[(size_t)n * ldm + m] = make_float2(are[tn][tmi][v], aim[tn][tmi][v]);
+      }
+}
+
+// FP32 chunk partials -> FP64 sums in chunk order, then scale / mask / accumulate
+__global__ void k_contract_reduce_f32(const float2* __restrict__ part, int splits, int nR, int nL,
+                                      int ldn, int ldm, double scale, double2* __restrict__ mu,
+                                      int64_t NC, int64_t n0, int64_t m0, int parity) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)nR * nL) return;
+  const int n = (int)(idx / nL);
+  const int m = (int)(idx % nL);
+  const int64_t s = n0 + n + m0 + m + 2;
+  const bool keep = parity == KPM_PARITY_NONE || (parity == KPM_PARITY_ODD && (s & 1)) ||
+                    (parity == KPM_PARITY_EVEN && !(s & 1));
+  if (!keep) return;
+  double sr = 0.0, si = 0.0;
+  for (int z = 0; z < splits; ++z) {
+    const float2 v = part[((size_t)z * ldn + n) * ldm + m];
+    sr += (double)v.x;
+    si += (double)v.y;
+  }
+  double2* dst = mu + (n0 + n) + NC * (m0 + m);
+  double2 cur = *dst;
+  cur.x += scale * sr;
+  cur.y += scale * si;
+  *dst = cur;
+}
+
 __global__ void k_contract_reduce(const double2* __restrict__ part, int splits, int nR, int nL,
                                   int ldn, int ldm, double scale, double2* __restrict__ mu,
                                   int64_t NC, int64_t n0, int64_t m0, int parity) {
@@ -425,6 +575,43 @@ __global__ void k_contract_reduce(const double2* __restrict__ part, int splits, 
   *dst = cur;
 }
 
+static int launch_contract_bf16_t128(kpm_ctx* ctx, const void* Lslots, int nL, const void* Rslots,
+                                     int nR, int64_t K2, int64_t P2, int prec, double scale,
+                                     void* mu_dev, int64_t NC, int64_t n0, int64_t m0, int parity) {
+  const int tn = (nR + 127) / 128, tm = (nL + 127) / 128;
+  const int ntiles = tn * tm;
+  const int64_t steps = K2 / kStep;
+  // chunks of at most kChunkBF16 reals (FP32 accumulation length), at least
+  // 4 workgroups per CU, and partials within 2 GiB
+  int64_t splits = std::max<int64_t>((K2 + kChunkBF16 - 1) / kChunkBF16,
+                                     (int64_t)(4 * std::max(ctx->num_cus, 1)) / ntiles);
+  const int64_t cap = std::max<int64_t>(1, ((int64_t)2 << 30) / ((int64_t)nR * nL * (int64_t)sizeof(float2)));
+  splits = std::min<int64_t>(std::min(splits, cap), std::max<int64_t>(1, steps));
+  if (splits >= 8) splits = std::min<int64_t>((splits + 7) / 8 * 8, std::max<int64_t>(8, cap / 8 * 8));
+  const int64_t steps_per = (steps + splits - 1) / splits;
+  const int64_t kchunk = steps_per * kStep;
+  if ((int64_t)ntiles * splits > INT32_MAX) return fail(KPM_EINVAL, "contraction grid too large");
+  const int ldn = nR, ldm = nL;
+  KPM_TRY(ensure(ctx, ctx->gemm_part, (size_t)splits * ldn * ldm * sizeof(float2)));
+  const dim3 grid((unsigned)(ntiles * splits));
+  const int sp = (int)splits;
+  if (prec == KPM_VEC_F64)
+    hipLaunchKernelGGL((k_contract_bf16x3_t128<double>), grid, dim3(256), 0, ctx->stream,
+                       (const double*)Lslots, nL, (const double*)Rslots, nR, K2, P2, kchunk, tm, ntiles, sp,
+                       (float2*)ctx->gemm_part.ptr, ldn, ldm);
+  else
+    hipLaunchKernelGGL((k_contract_bf16x3_t128<float>), grid, dim3(256), 0, ctx->stream,
+                       (const float*)Lslots, nL, (const float*)Rslots, nR, K2, P2, kchunk, tm, ntiles, sp,
+                       (float2*)ctx->gemm_part.ptr, ldn, ldm);
+  KPM_HIP_TRY(hipGetLastError());
+  const int64_t total = (int64_t)nR * nL;
+  hipLaunchKernelGGL(k_contract_reduce_f32, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     ctx->stream, (const float2*)ctx->gemm_part.ptr, sp, nR, nL, ldn, ldm, scale,
+                     (double2*)mu_dev, NC, n0, m0, parity);
+  KPM_HIP_TRY(hipGetLastError());
+  return KPM_OK;
+}
+
 int launch_contract(kpm_ctx* ctx, const void* Lslots, int nL, const void* Rslots, int nR,
                     int64_t K, int64_t pitch, int prec, int gemm_mode, double scale, void* mu_dev,
                     int64_t NC, int64_t n0, int64_t m0, int parity) {
@@ -447,6 +634,9 @@ int launch_contract(kpm_ctx* ctx, const void* Lslots, int nL, const void* Rslots
   KPM_TRY(ensure(ctx, ctx->gemm_part, (size_t)splits * ldn * ldm * sizeof(double2)));
   const dim3 grid((unsigned)(ntiles * splits));
   const int sp = (int)splits;
+  if (gemm_mode == KPM_GEMM_BF16X3 && nL >= 128 && nR >= 128)
+    return launch_contract_bf16_t128(ctx, Lslots, nL, Rslots, nR, K2, P2, prec, scale, mu_dev, NC, n0,
+                                     m0, parity);
   if (gemm_mode == KPM_GEMM_BF16X3) {
     if (prec == KPM_VEC_F64)
       hipLaunchKernelGGL((k_contract_bf16x3<double>), grid, dim3(256), 0, ctx->stream,

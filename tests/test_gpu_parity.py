@@ -285,15 +285,17 @@ def test_fp32_kpm2d_gemm_modes():
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-def test_bf16x3_contraction_long_k(prec):
+@pytest.mark.parametrize("NC", [70, 200])
+def test_bf16x3_contraction_long_k(prec, NC):
     """The split-BF16 contraction over many K stages and split-K chunks
     (Haldane 32 x 32, NR 16: 32,768 complex per slot) against the FP64 MFMA
-    contraction of the same slots: error at the 2^-16 level of the 3-pass
-    split (above FP32 rounding, so the BF16 path is the one that ran), and
-    ragged NC (70: tiles of 64 plus a 6-wide remainder)."""
+    contraction of the same slots: error at the level of the 3-pass split
+    (above FP32 rounding alone, so the BF16 path is the one that ran).
+    NC = 70 runs the 64 x 64 kernel (tiles of 64 plus a 6-wide remainder),
+    NC = 200 the 128 x 128 kernel with FP32 chunk sums (128 + 72)."""
     H, Jx, Jy, area = models.haldane_model(32, 32, t2=0.2)
     Hn = (H / 4.0).tocsc()
-    NR, NC = 16, 70
+    NR = 16
     pl = phases(11, H.shape[0], NR)
     ref = K.kpm_2d(Hn, Jx, Jy, NC, NR, psi_in=pl, precision=prec, gemm="f64")
     got = K.kpm_2d(Hn, Jx, Jy, NC, NR, psi_in=pl, precision=prec, gemm="bf16x3")
