@@ -48,14 +48,16 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--kernel-launches", type=int, default=100)
     ap.add_argument("--workload", choices=["graphene", "ensemble", "conductivity", "core", "adjoint",
-                                           "cpge"],
+                                           "cpge", "cond_ensemble"],
                     default="graphene",
                     help="graphene: the headline single-lattice DOS; ensemble: batched disorder "
                          "realisations (cubic Anderson, one fused launch per step); "
                          "conductivity: Haldane kpm_2d table; core: the reference's own "
                          "benchmark suite (benchmarks/core_bench.jl) beside its published times; "
                          "adjoint: DOS-loss gradient on a 512^2 disordered square lattice; "
-                         "cpge: three-current response from a Haldane kpm_3d table")
+                         "cpge: three-current response from a Haldane kpm_3d table; "
+                         "cond_ensemble: one antithetic-pair task of the Anderson conductivity "
+                         "ensemble pipeline")
     ap.add_argument("--nreal", type=int, default=256)
     args = ap.parse_args()
 
@@ -74,6 +76,8 @@ def main() -> int:
         return bench_adjoint(args)
     if args.workload == "cpge":
         return bench_cpge(args)
+    if args.workload == "cond_ensemble":
+        return bench_cond_ensemble(args)
 
     import numpy as np
 
@@ -562,6 +566,76 @@ def bench_cpge(args) -> int:
                       "omega1": w1, "omega2": w2, "lambda": lam, "nodes": nodes, "launches": launches},
            "kpm_3d_s": round(t3d, 3), "nodes_per_s": round(nodes / wall, 1), "cpge": [val.real, val.imag],
            "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(res))
+    return 0
+
+
+def bench_cond_ensemble(args) -> int:
+    """One task of the Anderson-Seebeck conductivity ensemble
+    (scripts/anderson_seebeck_ensemble.jl with the reference manifest's
+    values: W = 12, a = 12.25, NR = 8, L = 16, NC = 1024). A task is the
+    antithetic pair H(+eps), H(-eps) with Gamma-paired probes: two device
+    kpm_2d tables and the task record written to a scratch directory. One
+    step = one task (a fresh seed each step, so no record is skipped). The
+    CPU baseline times the oracle's C kpm_2d on the same task at NC = 256 (a
+    bounded sample; the contraction scales as NC^2, the recurrences as NC)."""
+    import shutil
+    import tempfile
+
+    import numpy as np
+
+    import kpmhip as K
+    from kpmhip import ensemble as E
+    from kpmhip.frontend import random_phase_vectors
+    from kpmhip.rng import Xoshiro256pp
+
+    if not K.activate(0):
+        raise SystemExit("no HIP device visible")
+    mf = E.validate_manifest({"model": {"W": 12.0, "t": 1.0, "a": 12.25, "b": 0.0},
+                              "kpm": {"NR": 8, "arr_size": 16},
+                              "ensemble": {"seed_base": 99532492, "sizes": [16], "NC": [1024],
+                                           "pairs_per_size": 64}})
+    tmp = tempfile.mkdtemp(prefix="kpm_ens_")
+    try:
+        quiet = lambda *a: None  # noqa: E731
+        for t in range(1, args.warmup + 1):
+            E.run_task(mf, t, tmp, log=quiet)
+        t0 = time.perf_counter()
+        for t in range(args.warmup + 1, args.warmup + args.steps + 1):
+            path = E.run_task(mf, t, tmp, log=quiet)
+        wall = (time.perf_counter() - t0) / args.steps
+        rec = E.load_task(path)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    L, NC, NR = 16, 1024, 8
+    NH = L ** 3
+    print(f"[bench] cond ensemble task (L={L}, NC={NC}, NR={NR}): {wall:.3f} s, pair residual "
+          f"{rec['meta']['pair_residual']:.2e}", file=sys.stderr, flush=True)
+    cpu = None
+    if not args.skip_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        _, _, _, _, seed = E.task_point(mf, 1)
+        rng = Xoshiro256pp(seed)
+        eps = 12.0 * (rng.rand(NH) - 0.5)
+        psi = random_phase_vectors(rng, NH, NR)
+        Hp, Jx, _ = E.cubic_hamiltonian_and_current(L, eps)
+        threads = min(16, os.cpu_count() or 1)
+        oracle.set_threads(threads)
+        NCs = 256
+        t0 = time.perf_counter()
+        oracle.kpm_2d((Hp / 12.25).tocsc(), Jx, Jx, NCs, psi, psi, NCs, NCs)
+        tc = time.perf_counter() - t0
+        cpu = {"value": round(2 * tc, 3), "unit": f"s per task at NC={NCs} (two tables)", "cores": threads,
+               "kind": "port", "sample": f"one oracle kpm_2d (C, {threads} threads) at NC={NCs}, doubled "
+                                         f"for the pair; the GPU task runs NC={NC}"}
+    res = {"metric": "conductivity-ensemble task time (Anderson L=16 antithetic pair)",
+           "value": round(wall, 4), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(wall * 1e3, 2), "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": "c128", "data": "synthetic",
+           "config": {"workload": "Anderson L=16 cubic, W=12, a=12.25, NC=1024, NR=8, two kpm_2d tables",
+                      "NH": NH, "NC": NC, "NR": NR},
+           "pair_residual": rec["meta"]["pair_residual"], "cpu_baseline": cpu}
     print(json.dumps(res))
     return 0
 

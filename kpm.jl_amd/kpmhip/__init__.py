@@ -27,6 +27,7 @@ from .bdg import (BdGOperator, ScaledOperator, bdg_assemble, chebyshev_stability
 from .stiffness import (diamagnetic_term, nambu_current_q, nambu_diamagnetic, superfluid_stiffness,
                         two_energy_response)
 from .cpge import Table3, cpge, d_cpge
+from . import ensemble
 from .device import (ACTIVE_DEVICE, CPUDevice, DeviceBlock, DeviceCSR, HIPDevice, activate,
                      device_free_memory, get_context, maybe_on_device_zeros, maybe_to_host,
                      to_device, whichcore)
