@@ -258,6 +258,33 @@ def test_kpm3d_vs_oracle():
     np.testing.assert_allclose(auto, ref, **TOL)
 
 
+@pytest.mark.parametrize("prec,NR", [("f64", 16), ("f32", 32), ("f64", 64), ("f64", 13)])
+def test_kpm2d_wide_blocks_vs_oracle(prec, NR):
+    """kpm_2d at wider probe blocks (NRp = 16, 32, 64 complex per row; NR = 13
+    pads to 16) in every contraction mode, against the oracle and against an
+    explicit blocking (arr_size = NC - 1, so the left ring wraps), plus the
+    EVEN parity mask."""
+    H, Jx, Jy, area = models.haldane_model(8, 8, t2=0.2)
+    Hn = (H / 4.0).tocsc()
+    NC = 24
+    pl, pr = phases(21, H.shape[0], NR), phases(22, H.shape[0], NR)
+    print(f"wide-block kpm_2d prec={prec} NR={NR}", flush=True)
+    got = K.kpm_2d(Hn, Jx, Jy, NC, NR, psi_in_l=pl, psi_in_r=pr, precision=prec)
+    sep = K.kpm_2d(Hn, Jx, Jy, NC, NR, psi_in_l=pl, psi_in_r=pr, precision=prec, arr_size=NC - 1,
+                   right_block=NC)
+    ref = oracle.kpm_2d(Hn, Jx, Jy, NC, pl, pr, NC, NC)
+    tol = TOL if prec == "f64" else dict(rtol=0, atol=2e-5 * np.abs(ref).max())
+    np.testing.assert_allclose(got, ref, **tol)
+    np.testing.assert_allclose(got, sep, rtol=1e-12, atol=1e-13 * np.abs(ref).max())
+    for gemm in ("f32", "bf16x3"):
+        g2 = K.kpm_2d(Hn, Jx, Jy, NC, NR, psi_in_l=pl, psi_in_r=pr, precision=prec, gemm=gemm)
+        assert np.abs(g2 - ref).max() <= 2e-4 * np.abs(ref).max()
+    par = K.kpm_2d(Hn, Jx, Jy, NC, NR, psi_in_l=pl, psi_in_r=pr, precision=prec, moment_parity="EVEN")
+    n, m = np.indices(par.shape)
+    assert np.all(par[(n + m) % 2 == 1] == 0)
+    np.testing.assert_allclose(par[(n + m) % 2 == 0], got[(n + m) % 2 == 0], rtol=1e-12, atol=1e-14)
+
+
 # ---- FP32 recurrence -------------------------------------------------------------------------
 
 def test_fp32_vectors_error_bound():
