@@ -201,3 +201,27 @@ def test_two_energy_response_vs_lehmann():
     pinf = K.two_energy_response(m, beta=math.inf, eta=0.25, omega=0.3, Ef=0.15, volume=1.0)
     einf = O.ed_two_energy_response(H0, JL, JR, beta=math.inf, eta=0.25, omega=0.3, Ef=0.15)
     assert np.isfinite(pinf) and abs(pinf - einf) <= 0.02 * abs(einf)
+
+
+def test_bdg_assemble_block_combinations():
+    """Dense h, complex intervalley h (asserted) and the constructor at
+    production size (test/bdg_test.jl:98-164)."""
+    h = models.ring_model(4)
+    D = np.full(4, 0.3 + 0.4j)
+    ref = O.bdg_matrix(h, -0.5, np.full(4, 2.0), np.full(4, 0.5), D)
+    opd = K.BdGOperator(h.toarray(), mu=-0.5, U=2.0, n=np.full(4, 0.5), Delta=D)
+    np.testing.assert_allclose(K.bdg_assemble(opd).toarray(), ref, atol=1e-12)
+    hc = np.array([[0, 1j], [-1j, 0]])
+    Dc = np.array([0.3 + 0.1j, 0.2 - 0.4j])
+    with pytest.raises(K.ArgumentError):
+        K.BdGOperator(hc, mu=0.2, U=0.0, n=np.zeros(2), Delta=Dc)
+    opc = K.BdGOperator(hc, mu=0.2, U=0.0, n=np.zeros(2), Delta=Dc, assume_intervalley=True)
+    np.testing.assert_allclose(K.bdg_assemble(opc).toarray(),
+                               O.bdg_matrix(hc, 0.2, np.zeros(2), np.zeros(2), Dc), atol=1e-12)
+    big = K.BdGOperator(sp.identity(50_000, dtype=np.complex128, format="csc"), mu=0.0, U=0.0)
+    assert big.N == 50_000 and big.shape == (100_000, 100_000)
+    # particle-hole symmetry of the spectrum and global gap-phase invariance
+    ev = np.linalg.eigvalsh(ref)
+    np.testing.assert_allclose(np.sort(ev), -np.sort(ev)[::-1], atol=1e-10)
+    evp = np.linalg.eigvalsh(O.bdg_matrix(h, -0.5, np.full(4, 2.0), np.full(4, 0.5), np.exp(0.77j) * D))
+    np.testing.assert_allclose(ev, evp, atol=1e-10)

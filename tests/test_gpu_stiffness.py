@@ -182,3 +182,40 @@ def test_non_assemblable_operator_is_rejected():
     op.h = np.asarray(H.toarray())  # dense blocks cannot go to the device
     with pytest.raises(K.ArgumentError):
         K.kpm_2d(op, H, H, 4, 1, 18)
+
+
+def test_scaled_operator_radius_and_rescale_contract():
+    """test/bdg_test.jl:383-449 on the 4-site ring."""
+    h = models.ring_model(4)
+    D = np.full(4, 0.3 + 0.4j)
+    op = K.BdGOperator(h, mu=-0.5, U=2.0, n=np.full(4, 0.5), Delta=D)
+    Hd = O.bdg_matrix(h, -0.5, np.full(4, 2.0), np.full(4, 0.5), D)
+    exact = np.abs(np.linalg.eigvalsh(Hd)).max()
+    rad, v = K.spectral_radius(op)
+    assert abs(rad - exact) <= 2e-3 * exact and rad <= exact + 1e-8 and v.shape == (8,)
+    rh = K.rescale(op)
+    assert rh.b == 0.0 and np.all(np.abs(np.linalg.eigvalsh(Hd / rh.a)) < 1)
+    # the device form of (op - b I)/a through a real-coefficient SpMM
+    S2 = ScaledOperator(op, 2.0, 0.3)
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal((8, 3)) + 1j * rng.standard_normal((8, 3))
+    y = rng.standard_normal((8, 3)) + 1j * rng.standard_normal((8, 3))
+    X, Y = K.DeviceBlock.from_host(np.asfortranarray(x)), K.DeviceBlock.from_host(np.asfortranarray(y))
+    K.spmm(S2, X, Y, alpha=1.3, beta=0.7)
+    np.testing.assert_allclose(Y.download(), 1.3 * ((Hd - 0.3 * np.eye(8)) / 2.0) @ x + 0.7 * y,
+                               atol=1e-12)
+    m = K.dos_moments(rh, NC=32, NR=2, rng=Xoshiro256pp(1))
+    assert np.all(np.isfinite(m.mu)) and abs(m.mu[0] - 1) < 1e-10
+    assert K.gershgorin_bound(op) >= exact
+    # an isolated edge: one of the two bounds must cover it
+    Ne, true_r = 20_000, 2.0
+    he = sp.diags(np.concatenate([[true_r], np.ones(Ne - 1)]), format="csc")
+    ope = K.BdGOperator(he, mu=0.0, U=0.0)
+    need = true_r / (1 - 0.2 / 2)
+    assert K.rescale(ope).a >= need or K.rescale(ope, bound="gershgorin").a >= need
+    op0 = K.BdGOperator(sp.csc_matrix((2, 2)), mu=0.0, U=0.0)
+    with pytest.raises(K.ArgumentError):
+        K.rescale(op0)
+    for bad in (0.0, 2.0):
+        with pytest.raises(K.ArgumentError):
+            K.rescale(op, eps=bad)
