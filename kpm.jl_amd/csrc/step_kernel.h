@@ -40,6 +40,8 @@
 //   on store and the dots use the stored (rounded) values.
 #pragma once
 
+#include <type_traits>
+
 #include "kpm_internal.h"
 
 namespace kpm {
@@ -78,6 +80,30 @@ struct Word<KPM_VEC_F32> {
 };
 
 constexpr int kBlock = 256;  // 4 waves
+
+// compile-time loop: f(std::integral_constant<int, I>) for I in [B, E)
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// Lane J of every 16-lane DPP row, broadcast to the whole row (DPP
+// row_newbcast, gfx90a and later: one VALU move, no LDS traffic).  Every lane
+// of the wave must be active.
+template <int J>
+__device__ __forceinline__ int row_bcast(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x150 + J, 0xF, 0xF, false);
+}
+template <int J>
+__device__ __forceinline__ double row_bcast(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = row_bcast<J>((int)(b & 0xffffffffLL));
+  const int hi = row_bcast<J>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
 // Streamed operands (z = v_{n-1}, read once; out = v_{n+1}, not reread until
 // the next pass) use non-temporal loads/stores so they do not evict the
@@ -165,6 +191,14 @@ template <int UNR, int KB>
 struct ColMeta {
   int c[UNR][KB];
 };
+// One nonzero per lane: lane j of every 16-lane DPP row holds column index
+// and value of nonzero j of its row (one coalesced load per array instead of
+// KB same-address loads per lane); row_bcast hands slot j to the whole row.
+template <int UNR>
+struct LaneMeta {
+  int c[UNR];
+  double hr[UNR], hi[UNR];
+};
 
 // Segmented schedule (SEG = true, batched realisations): workgroup b owns the
 // contiguous tiles [tile_lo[b], tile_lo[b+1]); treal[t] is the realisation of
@@ -188,6 +222,9 @@ __global__ void __launch_bounds__(kBlock)
   constexpr int TR = 4 * RPW * UNR;
   constexpr int ND = MODE == MODE_DBL ? 2 : (MODE == MODE_LR ? 1 : 0);
   constexpr int NDA = ND > 0 ? ND : 1;
+  // rows that span whole DPP rows take their nonzeros one per lane
+  constexpr bool kLaneMeta = LPR >= 16 && KB <= 16;
+  using CMeta = std::conditional_t<kLaneMeta, LaneMeta<UNR>, ColMeta<UNR, KB>>;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -236,16 +273,37 @@ __global__ void __launch_bounds__(kBlock)
       R.cnt[u] = v ? e0 - s0 : 0;
     }
   };
-  // column indices of the first KB nonzeros (invalid slots: index 0)
-  auto load_cols = [&](const RowMeta<UNR>& R, ColMeta<UNR, KB>& Cm) {
+  // column indices of the first KB nonzeros (invalid slots: index 0); with
+  // kLaneMeta also their values (invalid slots: weight 0)
+  auto load_cols = [&](const RowMeta<UNR>& R, CMeta& Cm) {
+    if constexpr (kLaneMeta) {
+      const int j = lane & 15;
 #pragma unroll
-    for (int u = 0; u < UNR; ++u)
-#pragma unroll
-      for (int jj = 0; jj < KB; ++jj) {
-        const bool v = jj < R.cnt[u];
-        const int c = colind[v ? R.s[u] + jj : 0];
-        Cm.c[u][jj] = v ? c : 0;
+      for (int u = 0; u < UNR; ++u) {
+        const bool v = j < R.cnt[u];
+        const int k = v ? R.s[u] + j : 0;
+        const int c = colind[k];
+        Cm.c[u] = v ? c : 0;
+        if constexpr (REALH) {
+          const double h = rvals[k];
+          Cm.hr[u] = v ? h : 0.0;
+          Cm.hi[u] = 0.0;
+        } else {
+          const double2 h = cvals[k];
+          Cm.hr[u] = v ? h.x : 0.0;
+          Cm.hi[u] = v ? h.y : 0.0;
+        }
       }
+    } else {
+#pragma unroll
+      for (int u = 0; u < UNR; ++u)
+#pragma unroll
+        for (int jj = 0; jj < KB; ++jj) {
+          const bool v = jj < R.cnt[u];
+          const int c = colind[v ? R.s[u] + jj : 0];
+          Cm.c[u][jj] = v ? c : 0;
+        }
+    }
   };
   auto mac = [&](double hr, double hi, const T& xw, double* yr, double* yi) {
     double xr[CPL], xi[CPL];
@@ -302,7 +360,7 @@ __global__ void __launch_bounds__(kBlock)
 
   int t = tstart;
   RowMeta<UNR> R0, R1;
-  ColMeta<UNR, KB> C0;
+  CMeta C0;
   load_rows(t, R0);
   load_cols(R0, C0);
   load_rows(t + Bx, R1);
@@ -311,23 +369,35 @@ __global__ void __launch_bounds__(kBlock)
     // 1. this tile's values, gathers and row-local operands, all in flight
     double hr[UNR][KB], hi[UNR][KB];
     T xv[UNR][KB];
+    if constexpr (kLaneMeta) {  // every lane is active here (uniform loop)
 #pragma unroll
-    for (int u = 0; u < UNR; ++u)
+      for (int u = 0; u < UNR; ++u)
+        static_for<0, KB>([&](auto J) {
+          constexpr int jj = decltype(J)::value;
+          const int c = row_bcast<jj>(C0.c[u]);
+          hr[u][jj] = row_bcast<jj>(C0.hr[u]);
+          hi[u][jj] = REALH ? 0.0 : row_bcast<jj>(C0.hi[u]);
+          xv[u][jj] = x[(size_t)c * RW + wcol];
+        });
+    } else {
 #pragma unroll
-      for (int jj = 0; jj < KB; ++jj) {
-        const bool v = jj < R0.cnt[u];
-        const int k = v ? R0.s[u] + jj : 0;
-        if constexpr (REALH) {
-          const double h = rvals[k];
-          hr[u][jj] = v ? h : 0.0;
-          hi[u][jj] = 0.0;
-        } else {
-          const double2 h = cvals[k];
-          hr[u][jj] = v ? h.x : 0.0;
-          hi[u][jj] = v ? h.y : 0.0;
+      for (int u = 0; u < UNR; ++u)
+#pragma unroll
+        for (int jj = 0; jj < KB; ++jj) {
+          const bool v = jj < R0.cnt[u];
+          const int k = v ? R0.s[u] + jj : 0;
+          if constexpr (REALH) {
+            const double h = rvals[k];
+            hr[u][jj] = v ? h : 0.0;
+            hi[u][jj] = 0.0;
+          } else {
+            const double2 h = cvals[k];
+            hr[u][jj] = v ? h.x : 0.0;
+            hi[u][jj] = v ? h.y : 0.0;
+          }
+          xv[u][jj] = x[(size_t)C0.c[u][jj] * RW + wcol];
         }
-        xv[u][jj] = x[(size_t)C0.c[u][jj] * RW + wcol];
-      }
+    }
     T zv[UNR], sv[UNR];
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
@@ -338,7 +408,7 @@ __global__ void __launch_bounds__(kBlock)
       if constexpr (MODE == MODE_LR || MODE == MODE_SRC) sv[u] = l[off];
     }
     // 2. prefetch: column indices of the next tile, row pointers of the one after
-    ColMeta<UNR, KB> C1;
+    CMeta C1;
     load_cols(R1, C1);
     RowMeta<UNR> R2;
     load_rows(t + 2 * Bx, R2);
