@@ -90,18 +90,29 @@ __device__ __forceinline__ void static_for(F&& f) {
   }
 }
 
-// Lane J of every 16-lane DPP row, broadcast to the whole row (DPP
-// row_newbcast, gfx90a and later: one VALU move, no LDS traffic).  Every lane
-// of the wave must be active.
-template <int J>
-__device__ __forceinline__ int row_bcast(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, 0x150 + J, 0xF, 0xF, false);
+// Lane J of every aligned group of G lanes, broadcast to its group, with DPP
+// moves (VALU only, no LDS traffic).  G = 16: row_newbcast (gfx90a and
+// later); G = 8: row_newbcast of lane J and of lane 8 + J, selected by the
+// half-row; G = 4: quad_perm [J, J, J, J].  Every lane of the wave must be
+// active.
+template <int G, int J>
+__device__ __forceinline__ int group_bcast(int v) {
+  static_assert(G == 4 || G == 8 || G == 16, "group of 4, 8 or 16 lanes");
+  if constexpr (G == 4) {
+    return __builtin_amdgcn_update_dpp(0, v, J * 0x55, 0xF, 0xF, false);
+  } else if constexpr (G == 8) {
+    const int a = __builtin_amdgcn_update_dpp(0, v, 0x150 + J, 0xF, 0xF, false);
+    const int b = __builtin_amdgcn_update_dpp(0, v, 0x158 + J, 0xF, 0xF, false);
+    return (threadIdx.x & 8) ? b : a;
+  } else {
+    return __builtin_amdgcn_update_dpp(0, v, 0x150 + J, 0xF, 0xF, false);
+  }
 }
-template <int J>
-__device__ __forceinline__ double row_bcast(double v) {
+template <int G, int J>
+__device__ __forceinline__ double group_bcast(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = row_bcast<J>((int)(b & 0xffffffffLL));
-  const int hi = row_bcast<J>((int)(b >> 32));
+  const int lo = group_bcast<G, J>((int)(b & 0xffffffffLL));
+  const int hi = group_bcast<G, J>((int)(b >> 32));
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
@@ -191,13 +202,15 @@ template <int UNR, int KB>
 struct ColMeta {
   int c[UNR][KB];
 };
-// One nonzero per lane: lane j of every 16-lane DPP row holds column index
-// and value of nonzero j of its row (one coalesced load per array instead of
-// KB same-address loads per lane); row_bcast hands slot j to the whole row.
-template <int UNR>
+// Nonzeros spread over the lanes of a row: with G = min(LPR, 16) lanes per
+// row inside a DPP row, lane j holds nonzeros j, j + G, ... (E of them) of its
+// row, column index and value, loaded coalesced (E loads per array instead of
+// KB same-address loads per lane); group_bcast hands slot jj to every lane of
+// the row.  Rows wider than 16 lanes load the same nonzeros in each DPP row.
+template <int UNR, int E>
 struct LaneMeta {
-  int c[UNR];
-  double hr[UNR], hi[UNR];
+  int c[UNR][E];
+  double hr[UNR][E], hi[UNR][E];
 };
 
 // Segmented schedule (SEG = true, batched realisations): workgroup b owns the
@@ -222,9 +235,11 @@ __global__ void __launch_bounds__(kBlock)
   constexpr int TR = 4 * RPW * UNR;
   constexpr int ND = MODE == MODE_DBL ? 2 : (MODE == MODE_LR ? 1 : 0);
   constexpr int NDA = ND > 0 ? ND : 1;
-  // rows that span whole DPP rows take their nonzeros one per lane
-  constexpr bool kLaneMeta = LPR >= 16 && KB <= 16;
-  using CMeta = std::conditional_t<kLaneMeta, LaneMeta<UNR>, ColMeta<UNR, KB>>;
+  // rows of 4 or more lanes take their nonzeros spread over the lanes
+  constexpr int MG = LPR >= 16 ? 16 : LPR;
+  constexpr bool kLaneMeta = MG >= 4 && KB <= 16;
+  constexpr int ME = kLaneMeta ? (KB + MG - 1) / MG : 1;
+  using CMeta = std::conditional_t<kLaneMeta, LaneMeta<UNR, ME>, ColMeta<UNR, KB>>;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -277,23 +292,26 @@ __global__ void __launch_bounds__(kBlock)
   // kLaneMeta also their values (invalid slots: weight 0)
   auto load_cols = [&](const RowMeta<UNR>& R, CMeta& Cm) {
     if constexpr (kLaneMeta) {
-      const int j = lane & 15;
+      const int j = lane & (MG - 1);
 #pragma unroll
-      for (int u = 0; u < UNR; ++u) {
-        const bool v = j < R.cnt[u];
-        const int k = v ? R.s[u] + j : 0;
-        const int c = colind[k];
-        Cm.c[u] = v ? c : 0;
-        if constexpr (REALH) {
-          const double h = rvals[k];
-          Cm.hr[u] = v ? h : 0.0;
-          Cm.hi[u] = 0.0;
-        } else {
-          const double2 h = cvals[k];
-          Cm.hr[u] = v ? h.x : 0.0;
-          Cm.hi[u] = v ? h.y : 0.0;
+      for (int u = 0; u < UNR; ++u)
+#pragma unroll
+        for (int e = 0; e < ME; ++e) {
+          const int idx = e * MG + j;
+          const bool v = idx < R.cnt[u];
+          const int k = v ? R.s[u] + idx : 0;
+          const int c = colind[k];
+          Cm.c[u][e] = v ? c : 0;
+          if constexpr (REALH) {
+            const double h = rvals[k];
+            Cm.hr[u][e] = v ? h : 0.0;
+            Cm.hi[u][e] = 0.0;
+          } else {
+            const double2 h = cvals[k];
+            Cm.hr[u][e] = v ? h.x : 0.0;
+            Cm.hi[u][e] = v ? h.y : 0.0;
+          }
         }
-      }
     } else {
 #pragma unroll
       for (int u = 0; u < UNR; ++u)
@@ -374,9 +392,10 @@ __global__ void __launch_bounds__(kBlock)
       for (int u = 0; u < UNR; ++u)
         static_for<0, KB>([&](auto J) {
           constexpr int jj = decltype(J)::value;
-          const int c = row_bcast<jj>(C0.c[u]);
-          hr[u][jj] = row_bcast<jj>(C0.hr[u]);
-          hi[u][jj] = REALH ? 0.0 : row_bcast<jj>(C0.hi[u]);
+          constexpr int e = jj / MG, src = jj % MG;
+          const int c = group_bcast<MG, src>(C0.c[u][e]);
+          hr[u][jj] = group_bcast<MG, src>(C0.hr[u][e]);
+          hi[u][jj] = REALH ? 0.0 : group_bcast<MG, src>(C0.hi[u][e]);
           xv[u][jj] = x[(size_t)c * RW + wcol];
         });
     } else {

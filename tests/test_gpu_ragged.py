@@ -1,0 +1,76 @@
+"""Ragged sparsity on the device: Hermitian operators whose rows run from
+empty to 41 nonzeros, so every nonzeros-per-row bucket of the recurrence pass
+(4, 8, 12, 16 and the long-row tail beyond 16) is taken, crossed with probe
+widths whose rows span 1, 4, 8, 16 and 64 lanes.  kpm_1d is compared with the
+oracle at 1e-12 (FP64 vectors) and bounded at FP32 rounding (FP32 vectors),
+and the left/right form with distinct probes is checked the same way.
+CUSPARSE's SpMM (ext/KPMCUDAExt.jl:84-88) takes any CSR; the lattice models of
+the other tests have near-uniform rows, so this is where empty and long rows
+are exercised.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import kpmhip as K
+import oracle
+from kpmhip.rng import Xoshiro256pp
+
+pytestmark = pytest.mark.gpu
+
+# band half-width -> the nonzeros-per-row bucket the widest row falls in
+BUCKETS = {1: (1, 4), 3: (5, 8), 5: (9, 12), 7: (13, 16), 20: (17, 41)}
+
+
+def ragged_hermitian(n, w, seed):
+    """Random complex Hermitian with row i coupled to a random subset of rows
+    i+1 .. i+w (and their mirror), a random real diagonal, every 11th row and
+    column empty, and one full-width row, rescaled into (-1, 1) by its
+    Gershgorin bound."""
+    rng = np.random.default_rng(seed)
+    rows, cols, vals = [], [], []
+    full = n // 2
+    for i in range(n):
+        for d in range(1, w + 1):
+            j = i + d
+            if j >= n:
+                break
+            if i == full or j == full or rng.random() < 0.6:
+                rows.append(i)
+                cols.append(j)
+                vals.append(complex(rng.normal(), rng.normal()))
+    up = sp.coo_matrix((vals, (rows, cols)), shape=(n, n))
+    H = (up + up.conj().T + sp.diags(rng.normal(size=n))).tocsr()
+    keep = np.ones(n)
+    keep[::11] = 0.0
+    D = sp.diags(keep)
+    H = (D @ H @ D).tocsr()
+    H.eliminate_zeros()
+    bound = np.max(np.asarray(abs(H).sum(axis=1)).ravel())
+    return (H / (1.05 * bound)).tocsc()
+
+
+@pytest.mark.parametrize("w", sorted(BUCKETS))
+@pytest.mark.parametrize("NR", [1, 4, 8, 16, 64])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_ragged_rows_vs_oracle(w, NR, prec):
+    n, NC = 600, 32
+    H = ragged_hermitian(n, w, seed=w)
+    counts = np.diff(H.tocsr().indptr)
+    lo, hi = BUCKETS[w]
+    assert lo <= counts.max() <= hi and counts.min() == 0
+    print(f"ragged kpm_1d prec={prec} NR={NR} w={w} max_row={counts.max()}", flush=True)
+    psi = K.random_phase_vectors(Xoshiro256pp(7 + NR), n, NR)
+    ref = oracle.kpm_1d(H, NC, psi)
+    got = K.kpm_1d(H, NC, NR, psi_in=psi, avg_output=False, precision=prec)
+    if prec == "f64":
+        np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12)
+    else:  # FP32-rounded vectors, FP64 arithmetic: a few ulp of 2^-24 per step
+        assert np.max(np.abs(got - ref)) < 2e-5
+    pl = K.random_phase_vectors(Xoshiro256pp(100 + NR), n, NR)
+    ref_lr = oracle.kpm_1d_lr(H, 17, pl, psi)
+    got_lr = K.kpm_1d(H, 17, NR, psi_in_l=pl, psi_in_r=psi, avg_output=False, precision=prec)
+    if prec == "f64":
+        np.testing.assert_allclose(got_lr, ref_lr, rtol=1e-12, atol=1e-12)
+    else:
+        assert np.max(np.abs(got_lr - ref_lr)) < 2e-5
