@@ -258,9 +258,13 @@ int kpm_bench_trilinear(kpm_table3* t, int64_t Q, int64_t reps, float* ms);
  * kpm_1d!, src/moment.jl:420-492) and g[n] = dL/dmu_n (NC reals), writes
  * grad[k] = dL/dH_k for every stored entry k of H, in the order of the CSC
  * arrays H was built from (interleaved complex; dL = sum_k grad[k] dH_k).
- * Reverse Chebyshev (Clenshaw) recurrence with the forward vectors
- * recomputed from checkpoints every ckpt_stride steps (0: ceil(sqrt(NC))).
- * H must come from kpm_csr_from_csc. */
+ * Reverse Chebyshev (Clenshaw) recurrence.  ckpt_stride 1 keeps every
+ * forward vector; K > 1 keeps checkpoint pairs every K steps and recomputes
+ * one segment at a time; 0 keeps every vector when they fit in 60% of the
+ * free device memory (plus this context's cached adjoint workspace) and
+ * uses K = ceil(sqrt(NC)) otherwise.  The workspace stays cached in the
+ * context (grow-only, released by kpm_ctx_destroy), so repeated gradient
+ * steps allocate it once.  H must come from kpm_csr_from_csc. */
 int kpm_1d_adjoint(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t NR, int64_t NH,
                    const double* psi, const double* g, double* grad, int vec_prec,
                    int64_t ckpt_stride);

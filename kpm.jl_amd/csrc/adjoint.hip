@@ -192,17 +192,21 @@ int kpm_1d_adjoint(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t NR, int64
   const size_t bbytes = (size_t)(NH * NRp) * elem_bytes(vec_prec);
   const int64_t nbr = kWin + 2;   // beta ring: the recurrence's two plus a window
   // storage plan: every alpha when asked (stride 1) or when it fits in 60% of
-  // the free memory (stride 0); checkpoint pairs every K steps otherwise
+  // the free memory plus the workspace this context already holds (stride 0);
+  // checkpoint pairs every K steps otherwise
   size_t fr = 0, tot = 0;
   KPM_HIP_TRY(hipMemGetInfo(&fr, &tot));
+  const size_t avail = fr + ctx->adjoint.bytes;
   const bool full = ckpt_stride == 1 ||
-                    (ckpt_stride == 0 && bbytes * (size_t)(1 + nbr + nalpha) <= fr / 10 * 6);
+                    (ckpt_stride == 0 && bbytes * (size_t)(1 + nbr + nalpha) <= avail / 10 * 6);
   const int64_t K = full ? 1 : (ckpt_stride > 1 ? ckpt_stride : (int64_t)std::ceil(std::sqrt((double)NC)));
   const int64_t nseg = full ? 0 : (nalpha + K - 1) / K;
   // workspace: v | beta ring | (full) alpha_0..alpha_{nalpha-1}  or
   //                           (checkpoints) forward ring, 2*nseg pairs, K segment
+  // It stays with the context (grow-only, released by kpm_ctx_destroy), so a
+  // loop of gradient steps allocates it once.
   const int64_t nblk = full ? 1 + nbr + nalpha : 1 + nbr + 2 + 2 * nseg + K;
-  Buffer ws;
+  Buffer& ws = ctx->adjoint;
   KPM_TRY(ensure(ctx, ws, bbytes * (size_t)nblk));
   auto blk = [&](int64_t i) {
     kpm_block b;
@@ -221,7 +225,7 @@ int kpm_1d_adjoint(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t NR, int64
   auto ckA = [&](int64_t s) { return blk(1 + nbr + 2 + 2 * s); };
   auto ckB = [&](int64_t s) { return blk(1 + nbr + 3 + 2 * s); };
   auto seg = [&](int64_t i) { return blk(1 + nbr + 2 + 2 * nseg + i); };
-  Buffer Mbuf;
+  Buffer& Mbuf = ctx->adj_grad;
   int rc = KPM_OK;
   do {
     const size_t mbytes = (size_t)std::max<int64_t>(H->nnz, 1) * sizeof(double2);
@@ -320,8 +324,6 @@ int kpm_1d_adjoint(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t NR, int64
     }
   } while (false);
   (void)hipStreamSynchronize(ctx->stream);
-  if (Mbuf.ptr) (void)hipFree(Mbuf.ptr);
-  if (ws.ptr) (void)hipFree(ws.ptr);
   return rc;
 }
 

@@ -75,6 +75,8 @@ struct kpm_ctx {
   kpm::Buffer slots_r;    // kpm_2d right slot array
   kpm::Buffer ring[5];    // kpm_2d / kpm_3d ring and starter blocks
   kpm::Buffer pinned;     // page-locked host staging for operator uploads
+  kpm::Buffer adjoint;    // adjoint workspace: v | beta ring | alpha store
+  kpm::Buffer adj_grad;   // adjoint gradient accumulator (nnz complex)
 };
 
 struct kpm_csr {

@@ -291,6 +291,8 @@ int kpm_ctx_destroy(kpm_ctx* ctx) {
   free_buffer(ctx->slots_l);
   free_buffer(ctx->slots_r);
   for (auto& b : ctx->ring) free_buffer(b);
+  free_buffer(ctx->adjoint);
+  free_buffer(ctx->adj_grad);
   if (ctx->pinned.ptr) (void)hipHostFree(ctx->pinned.ptr);
   for (auto& ev : ctx->events)
     if (ev) (void)hipEventDestroy(ev);
