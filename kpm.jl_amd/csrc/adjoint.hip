@@ -119,6 +119,109 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// The same sums for rows of LPR >= 4 lanes, at most KB <= 16 nonzeros and a
+// single row chunk, with every memory access of a row issued together: with
+// G = min(LPR, 16), lane j of a row holds column index and accumulator entry
+// of nonzeros j, j + G, ... (one coalesced load each), group_bcast hands slot
+// jj's column to the row, the window's KB gathers per step go out at once,
+// and after the row reduction (the xor order above) each lane adds its own
+// entries and stores them coalesced.  Per entry the terms are summed in the
+// order of k_sddmm, so both kernels give bitwise the same M.
+template <int PREC, int LPR, int KB>
+__global__ void __launch_bounds__(kBlock)
+    k_sddmm_rows(const int* __restrict__ rowptr, const int* __restrict__ colind,
+                 SddmmWindow<PREC> win, double2* __restrict__ M, int NH, int RW) {
+  using WT = Word<PREC>;
+  constexpr int CPL = WT::CPL;
+  constexpr int RPW = 64 / LPR;
+  constexpr int MG = LPR >= 16 ? 16 : LPR;
+  constexpr int ME = (KB + MG - 1) / MG;
+  static_assert(MG >= 4 && KB <= 16, "rows of 4+ lanes, at most 16 nonzeros");
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int sub = lane % LPR;
+  const int rslot = lane / LPR;
+  const int j = lane & (MG - 1);
+  const bool owner = sub < MG;  // rows wider than 16 lanes hold each entry once per DPP row
+  const int rows_per_block = 4 * RPW;
+  for (int base = blockIdx.x * rows_per_block; base < NH; base += gridDim.x * rows_per_block) {
+    const int row = base + wave * RPW + rslot;
+    const bool live = row < NH;
+    const int rc = live ? row : 0;
+    const int s0 = rowptr[rc];
+    const int n = live ? rowptr[rc + 1] - s0 : 0;
+    int cl[ME];
+    double2 m[ME];
+#pragma unroll
+    for (int e = 0; e < ME; ++e) {
+      const int idx = e * MG + j;
+      const bool v = idx < n;
+      const int k = v ? s0 + idx : 0;
+      const int c = colind[k];
+      const double2 mv = M[k];
+      cl[e] = v ? c : 0;
+      m[e] = mv;
+    }
+    double br[kWin][CPL], bi[kWin][CPL];
+#pragma unroll
+    for (int w = 0; w < kWin; ++w) {
+      if (w < win.W) {
+        WT::unpack(win.beta[w][(size_t)rc * RW + sub], br[w], bi[w]);
+      } else {
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) br[w][q] = bi[w][q] = 0.0;
+      }
+    }
+    int col[KB];
+    static_for<0, KB>([&](auto J) {
+      constexpr int jj = decltype(J)::value;
+      col[jj] = group_bcast<MG, jj % MG>(cl[jj / MG]);
+    });
+    double pr[KB], pi[KB];
+#pragma unroll
+    for (int jj = 0; jj < KB; ++jj) pr[jj] = pi[jj] = 0.0;
+#pragma unroll
+    for (int w = 0; w < kWin; ++w) {
+      if (w < win.W) {  // uniform
+        typename WT::T av[KB];
+#pragma unroll
+        for (int jj = 0; jj < KB; ++jj) av[jj] = win.alpha[w][(size_t)col[jj] * RW + sub];
+#pragma unroll
+        for (int jj = 0; jj < KB; ++jj) {
+          double ar[CPL], ai[CPL];
+          WT::unpack(av[jj], ar, ai);
+          double qr = 0.0, qi = 0.0;
+#pragma unroll
+          for (int q = 0; q < CPL; ++q) {
+            qr = fma(br[w][q], ar[q], fma(bi[w][q], ai[q], qr));
+            qi = fma(br[w][q], ai[q], fma(-bi[w][q], ar[q], qi));
+          }
+          pr[jj] = fma(win.c[w], qr, pr[jj]);
+          pi[jj] = fma(win.c[w], qi, pi[jj]);
+        }
+      }
+    }
+    static_for<0, KB>([&](auto J) {
+      constexpr int jj = decltype(J)::value;
+      double tr = pr[jj], ti = pi[jj];
+#pragma unroll
+      for (int o = 1; o < LPR; o <<= 1) {
+        tr += __shfl_xor(tr, o);
+        ti += __shfl_xor(ti, o);
+      }
+      if (j == jj % MG) {
+        m[jj / MG].x += tr;
+        m[jj / MG].y += ti;
+      }
+    });
+#pragma unroll
+    for (int e = 0; e < ME; ++e) {
+      const int idx = e * MG + j;
+      if (owner && idx < n) M[s0 + idx] = m[e];
+    }
+  }
+}
+
 template <int PREC>
 static int launch_sddmm_t(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* const* beta,
                           const kpm_block* const* alpha, const double* c, int W, double2* M) {
@@ -136,6 +239,26 @@ static int launch_sddmm_t(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* const
   const int64_t rows_per_block = 4LL * (64 / lpr);
   const int64_t nb = (beta[0]->NH + rows_per_block - 1) / rows_per_block;
   const unsigned G = (unsigned)std::min<int64_t>(nb, (int64_t)std::max(ctx->num_cus, 8) * 8);
+  const int w = A->max_row_nnz;
+  if (lpr >= 4 && chunks == 1 && w <= 16) {
+    const int kb = w <= 4 ? 4 : (w <= 8 ? 8 : (w <= 12 ? 12 : 16));
+#define KPM_SDDMM_ROWS(L, KBv)                                                                    \
+  if (lpr == L && kb == KBv) {                                                                    \
+    hipLaunchKernelGGL((k_sddmm_rows<PREC, L, KBv>), dim3(G), dim3(kBlock), 0, ctx->stream,       \
+                       A->rowptr, A->colind, win, M, (int)beta[0]->NH, RW);                       \
+    KPM_HIP_TRY(hipGetLastError());                                                               \
+    return KPM_OK;                                                                                \
+  }
+#define KPM_SDDMM_KB(L) KPM_SDDMM_ROWS(L, 4) KPM_SDDMM_ROWS(L, 8) KPM_SDDMM_ROWS(L, 12) KPM_SDDMM_ROWS(L, 16)
+    KPM_SDDMM_KB(4)
+    KPM_SDDMM_KB(8)
+    KPM_SDDMM_KB(16)
+    KPM_SDDMM_KB(32)
+    KPM_SDDMM_KB(64)
+#undef KPM_SDDMM_KB
+#undef KPM_SDDMM_ROWS
+    return fail(KPM_EINVAL, "sddmm: no row kernel for this shape");
+  }
 #define KPM_SDDMM(L)                                                                              \
   case L:                                                                                         \
     hipLaunchKernelGGL((k_sddmm<PREC, L>), dim3(G, chunks), dim3(kBlock), 0, ctx->stream,         \

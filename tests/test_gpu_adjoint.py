@@ -8,6 +8,7 @@ import kpmhip as K
 import oracle
 from kpmhip import models
 from kpmhip.rng import Xoshiro256pp
+from matrices import BUCKETS, ragged_hermitian
 
 pytestmark = pytest.mark.gpu
 
@@ -116,3 +117,25 @@ def test_site_and_bond_gradients_vs_fd():
         u = np.sign(np.real(Hphys[i, j]))  # real hoppings: unit phase = sign
         fd = (L_of(Hphys + h * u * E) - L_of(Hphys - h * u * E)) / (2 * h)
         assert abs(gb.data[k] - fd) <= 1e-5 * max(abs(fd), 1e-9), (k, gb.data[k], fd)
+
+
+@pytest.mark.parametrize("w", sorted(BUCKETS))
+@pytest.mark.parametrize("NR", [1, 8, 16, 64, 65])
+def test_adjoint_ragged_rows_vs_tangent(w, NR):
+    """Ragged rows (empty up to 41 nonzeros) across row widths of 1 to 64
+    lanes and a two-chunk row (NR = 65): every shape of the gradient
+    accumulation, checked along two random Hermitian directions."""
+    H = ragged_hermitian(300, w, seed=40 + w)
+    NC = 24
+    print(f"adjoint ragged w={w} NR={NR} max_row={np.diff(H.tocsr().indptr).max()}", flush=True)
+    psi = K.random_phase_vectors(Xoshiro256pp(60 + NR), H.shape[0], NR)
+    g = Xoshiro256pp(61 + w).rand(NC) - 0.5
+    grad = K.kpm_1d_adjoint(H, NC, psi, g)
+    G = sp.csc_matrix(grad)
+    G.sort_indices()
+    for seed in (5, 6):
+        A, dH = hermitian_direction(H, seed)
+        pred = np.real(np.sum(G.data * dH))
+        _, dmu = oracle.kpm_1d_tangent(A, dH, NC, psi)
+        tang = float(np.sum(g * np.real(dmu.mean(axis=0))))
+        assert abs(pred - tang) <= 1e-10 * max(1.0, abs(tang)), (seed, pred, tang)
