@@ -48,7 +48,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--kernel-launches", type=int, default=100)
     ap.add_argument("--workload", choices=["graphene", "ensemble", "conductivity", "core", "adjoint",
-                                           "cpge", "cond_ensemble"],
+                                           "cpge", "cond_ensemble", "spmm"],
                     default="graphene",
                     help="graphene: the headline single-lattice DOS; ensemble: batched disorder "
                          "realisations (cubic Anderson, one fused launch per step); "
@@ -57,7 +57,12 @@ def main() -> int:
                          "adjoint: DOS-loss gradient on a 512^2 disordered square lattice; "
                          "cpge: three-current response from a Haldane kpm_3d table; "
                          "cond_ensemble: one antithetic-pair task of the Anderson conductivity "
-                         "ensemble pipeline")
+                         "ensemble pipeline; spmm: one tuned recurrence pass of --op at --nr "
+                         "probes and --prec storage")
+    ap.add_argument("--op", choices=["graphene", "haldane", "cubic"], default="haldane",
+                    help="operator of the spmm workload (side --cells; cubic uses L = 128 "
+                         "unless --cells is below 1024)")
+    ap.add_argument("--prec", choices=["f64", "f32"], default="f64")
     ap.add_argument("--nreal", type=int, default=256)
     args = ap.parse_args()
 
@@ -78,6 +83,8 @@ def main() -> int:
         return bench_cpge(args)
     if args.workload == "cond_ensemble":
         return bench_cond_ensemble(args)
+    if args.workload == "spmm":
+        return bench_spmm(args)
 
     import numpy as np
 
@@ -228,6 +235,66 @@ CORE_PUBLISHED = {
     "chebyshev_iter_single_NH262144_NR8": (0.005135421, 0.003572621),
     "chebyshev_action_NH65536_NC1024_NR8_K2": (1.8632278, 1.60499539),
 }
+
+
+def bench_spmm(args) -> int:
+    """One recurrence pass (k_step, MODE_DBL: out = 2 H x - z plus the two
+    doubling dots) of a lattice operator at NR probes, after the autotuner
+    has picked the launch shape; timed with HIP events over --kernel-launches
+    back-to-back passes.  Operators: graphene (cells x cells x 2, disorder
+    U[-0.5, 0.5], real H), haldane (cells x cells x 2, t2 = 0.1 e^{i pi/2},
+    complex H, 10 stored entries per row), cubic (L^3, W = 2, real H).
+    Algorithmic bytes: H once (values, column indices, row pointers) and three
+    vector rows (read v_n, read v_{n-1}, write v_{n+1})."""
+    import ctypes
+
+    import numpy as np
+
+    import kpmhip as K
+    from kpmhip import models
+    from kpmhip.device import DeviceBlock
+
+    if not K.activate(0):
+        raise SystemExit("no HIP device visible")
+    ctx = K.get_context()
+    L = args.cells
+    if args.op == "graphene":
+        H = (models.graphene(L, L, disorder=0.5, seed=1) / 3.6).tocsr()
+    elif args.op == "haldane":
+        H = (models.haldane_model(L, L, t=1.0, t2=0.1)[0] / 3.8).tocsr()
+    else:
+        L = L if L < 1024 else 128
+        H = (models.cubic_model(L, W=2.0)[0] / 8.5).tocsr()
+    NH, nnz, NR = H.shape[0], H.nnz, args.nr
+    real = not (H.dtype.kind == "c" and np.any(H.data.imag != 0))
+    pcode = 0 if args.prec == "f64" else 1
+    NRp = K.moment._padded_nr(NR, pcode)
+    byts = nnz * ((8 if real else 16) + 4) + (NH + 1) * 4 + 3 * NH * NRp * (16 if pcode == 0 else 8)
+    Hd = K.to_device(H)
+    tuned = K.autotune(Hd, NR, args.prec)
+    x, z = DeviceBlock(NH, NR, args.prec, ctx), DeviceBlock(NH, NR, args.prec, ctx)
+    ms = ctypes.c_float(0.0)
+    n = max(args.kernel_launches, 1)
+    K._lib.check(ctx.lib.kpm_bench_step(ctx.handle, Hd.handle, x.handle, z.handle, 3, ctypes.byref(ms)),
+                 "bench_step")
+    K._lib.check(ctx.lib.kpm_bench_step(ctx.handle, Hd.handle, x.handle, z.handle, n, ctypes.byref(ms)),
+                 "bench_step")
+    t = ms.value / n * 1e-3
+    gbs = byts / t / 1e9
+    print(f"[bench] spmm {args.op} L={L} NR={NR} {args.prec}: {t * 1e6:.1f} us per pass, "
+          f"{gbs:.0f} GB/s algorithmic, tuned {tuned}", file=sys.stderr, flush=True)
+    res = {"metric": "recurrence pass time (k_step MODE_DBL)", "value": round(t * 1e6, 2), "unit": "us",
+           "n_gpus": 1, "steps": n, "warmup": 3, "ms_per_step": round(t * 1e3, 5),
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": "c128" if pcode == 0 else "c32-storage/f64-accumulate", "data": "synthetic",
+           "config": {"workload": f"{args.op} L={L} spmm", "NH": NH, "nnz": nnz, "NR": NR,
+                      "real_H": bool(real)},
+           "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                        "bytes_per_launch": int(byts), "avg_launch_ms": round(t * 1e3, 5)},
+           "tuned": tuned}
+    print(json.dumps(res))
+    return 0
 
 
 def bench_core(args) -> int:
