@@ -114,6 +114,188 @@ static int run_2d(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, const kpm_c
   return KPM_OK;
 }
 
+// ---- kpm_3d with the outer orders batched as probe columns ------------------
+//
+// kpm_3d's NC nested kpm_2d calls share the left chain T_m psi_l and differ
+// only in the right starter Jg T_{p-1} Ja psi_r.  With every left order
+// resident, the left chain is built once and P outer orders run as one right
+// chain over a wide block of P * NRp columns: column block q holds outer
+// order b + q.  Every column sees exactly the arithmetic of the narrow chain,
+// so every contraction input is bitwise that of the nested form.  After each
+// order's Jb pass the wide block is scattered into P dense per-order slot
+// arrays (the contraction's layout, one pitch), and the whole batch is one
+// contraction of the NC left slots against its P * NC right slots.
+
+// column block q of a wide block (NRw per row) <- a narrow block (NRp per row)
+template <typename E>
+__global__ void __launch_bounds__(256)
+    k_to_wide(const E* __restrict__ src, E* __restrict__ dst, int64_t NH, int NRp, int NRw, int q) {
+  const int64_t n = NH * NRp;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const int64_t row = i / NRp;
+    const int c = (int)(i - row * NRp);
+    dst[row * NRw + (int64_t)q * NRp + c] = src[i];
+  }
+}
+
+// wide block -> P narrow slots: slot q at dst + q * qstride (elements)
+template <typename E>
+__global__ void __launch_bounds__(256)
+    k_from_wide(const E* __restrict__ src, E* __restrict__ dst, int64_t NH, int NRp, int P,
+                int64_t qstride) {
+  const int NRw = NRp * P;
+  const int64_t n = NH * NRw;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const int64_t row = i / NRw;
+    const int c = (int)(i - row * NRw);
+    const int q = c / NRp;
+    const int r = c - q * NRp;
+    dst[(int64_t)q * qstride + row * NRp + r] = src[i];
+  }
+}
+
+static unsigned copy_grid(kpm_ctx* ctx, int64_t n) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)std::max(ctx->num_cus, 8) * 16));
+}
+
+static int to_wide(kpm_ctx* ctx, const kpm_block* src, void* dst, int NRw, int q) {
+  const int64_t n = src->NH * src->NRp;
+  if (src->prec == KPM_VEC_F32)
+    hipLaunchKernelGGL(k_to_wide<float2>, dim3(copy_grid(ctx, n)), dim3(256), 0, ctx->stream,
+                       (const float2*)src->d, (float2*)dst, src->NH, (int)src->NRp, NRw, q);
+  else
+    hipLaunchKernelGGL(k_to_wide<double2>, dim3(copy_grid(ctx, n)), dim3(256), 0, ctx->stream,
+                       (const double2*)src->d, (double2*)dst, src->NH, (int)src->NRp, NRw, q);
+  KPM_HIP_TRY(hipGetLastError());
+  return KPM_OK;
+}
+
+static int from_wide(kpm_ctx* ctx, const kpm_block* src, int64_t NRp, int P, void* dst,
+                     int64_t qstride) {
+  const int64_t n = src->NH * NRp * P;
+  if (src->prec == KPM_VEC_F32)
+    hipLaunchKernelGGL(k_from_wide<float2>, dim3(copy_grid(ctx, n)), dim3(256), 0, ctx->stream,
+                       (const float2*)src->d, (float2*)dst, src->NH, (int)NRp, P, qstride);
+  else
+    hipLaunchKernelGGL(k_from_wide<double2>, dim3(copy_grid(ctx, n)), dim3(256), 0, ctx->stream,
+                       (const double2*)src->d, (double2*)dst, src->NH, (int)NRp, P, qstride);
+  KPM_HIP_TRY(hipGetLastError());
+  return KPM_OK;
+}
+
+// Outer orders per batch: the largest power of two P <= NC, <= 64, such that
+// the NC left slots, P * NC right slots and four wide blocks fit in half of
+// the free memory (plus what this context already holds for them); 0 when
+// fewer than two fit.
+static int64_t batch_3d(kpm_ctx* ctx, const Plan2d& p) {
+  const int64_t NRp = padded_nr(p.NR, p.prec);
+  const int64_t eb = elem_bytes(p.prec);
+  const int64_t Kp = (p.NH * NRp + 15) / 16 * 16;
+  const int64_t stride = (Kp + 256 / eb) * eb;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 0;
+  size_t held = ctx->slots_l.bytes + ctx->slots_r.bytes + ctx->mu.bytes;
+  for (const auto& b : ctx->ring) held += b.bytes;
+  const int64_t budget = (int64_t)((fr + held) / 2);
+  const int64_t block = p.NH * NRp * eb;
+  int64_t P = 1;
+  while (P * 2 <= std::min<int64_t>(p.NC, 64)) {
+    const int64_t Q = P * 2;
+    const int64_t need = stride * p.NC * (1 + Q) + 4 * block * Q + Q * p.NC * p.NC * 16;
+    if (need > budget) break;
+    P = Q;
+  }
+  return P >= 2 ? P : 0;
+}
+
+static int run_3d_batched(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, const kpm_csr* Jb,
+                          const kpm_csr* Jg, const Plan2d& p, int64_t P, const kpm_block* l,
+                          const kpm_block* r, kpm_block* outer, double* mu) {
+  const int64_t NH = p.NH, NR = p.NR, NC = p.NC;
+  const int prec = p.prec;
+  const int64_t NRp = padded_nr(NR, prec);
+  const int64_t NRw = NRp * P;
+  if (padded_nr(NRw, prec) != NRw) return fail(KPM_EINVAL, "kpm_3d batch width is not a row width");
+  const int64_t used = NH * NRp;
+  const int64_t Kp = (used + 15) / 16 * 16;
+  const int64_t eb = elem_bytes(prec);
+  const int64_t pitch = Kp + 256 / eb;
+  const int64_t stride = pitch * eb;
+  KPM_TRY(ensure(ctx, ctx->slots_l, (size_t)(stride * NC)));
+  KPM_TRY(ensure(ctx, ctx->slots_r, (size_t)(stride * NC * P)));
+  if (Kp > used) {
+    KPM_HIP_TRY(hipMemset2DAsync((char*)ctx->slots_l.ptr + used * eb, (size_t)stride, 0,
+                                 (size_t)((Kp - used) * eb), (size_t)NC, ctx->stream));
+    KPM_HIP_TRY(hipMemset2DAsync((char*)ctx->slots_r.ptr + used * eb, (size_t)stride, 0,
+                                 (size_t)((Kp - used) * eb), (size_t)(NC * P), ctx->stream));
+  }
+  const size_t slab = (size_t)(NC * NC);
+  KPM_TRY(ensure(ctx, ctx->mu, slab * (size_t)P * sizeof(double2)));
+  kpm_block W[3], Rw, tmp;
+  for (int i = 0; i < 3; ++i) KPM_TRY(work_block(ctx, ctx->ring[i], NH, NRw, prec, &W[i]));
+  KPM_TRY(work_block(ctx, ctx->ring[3], NH, NRw, prec, &Rw));
+  KPM_TRY(work_block(ctx, ctx->ring[4], NH, NR, prec, &tmp));
+  std::vector<kpm_block> L(NC);
+  for (int64_t i = 0; i < NC; ++i) slot_view(ctx->slots_l.ptr, stride, i, NH, NR, prec, ctx, &L[i]);
+
+  // the left chain, once (:1013-1018 and chebyshev_iter over all NC orders)
+  KPM_TRY(kpm_block_copy(&L[0], l));
+  if (NC > 1) KPM_TRY(spmm(ctx, H, l, nullptr, &L[1], 1.0, 0.0));
+  for (int64_t i = 3; i <= NC; ++i) KPM_TRY(spmm(ctx, H, &L[i - 2], &L[i - 3], &L[i - 1], 2.0, -1.0));
+
+  const double scale = 1.0 / (double)NR;
+  const int64_t qstride = stride / eb;  // one right slot, in elements
+  std::vector<double> host;
+  for (int64_t b = 1; b <= NC; b += P) {
+    const int64_t Pb = std::min<int64_t>(P, NC - b + 1);
+    // wide starter: column block q = Jg T_{b+q-1} Ja psi_r (unused blocks stay zero)
+    KPM_HIP_TRY(hipMemsetAsync(W[0].d, 0, (size_t)(NH * NRw * eb), ctx->stream));
+    for (int64_t q = 0; q < Pb; ++q) {
+      const int64_t n1 = b + q;
+      const int ri = (int)((n1 - 1) % 3), rip = (int)((n1 + 1) % 3), ripp = (int)(n1 % 3);
+      if (n1 == 1) {
+        KPM_TRY(spmm(ctx, Ja, r, nullptr, &outer[0], 1.0, 0.0));
+      } else if (n1 == 2) {
+        KPM_TRY(spmm(ctx, H, &outer[rip], nullptr, &outer[1], 1.0, 0.0));
+      } else {
+        KPM_TRY(spmm(ctx, H, &outer[rip], &outer[ripp], &outer[ri], 2.0, -1.0));
+      }
+      KPM_TRY(spmm(ctx, Jg, &outer[ri], nullptr, &tmp, 1.0, 0.0));
+      KPM_TRY(to_wide(ctx, &tmp, W[0].d, (int)NRw, (int)q));
+    }
+    // the right chain over all outer orders of the batch (:1039-1051)
+    for (int64_t n = 1; n <= NC; ++n) {
+      const int ri = (int)((n - 1) % 3), rip = (int)((n + 1) % 3), ripp = (int)(n % 3);
+      if (n == 2) {
+        KPM_TRY(spmm(ctx, H, &W[0], nullptr, &W[1], 1.0, 0.0));
+      } else if (n >= 3) {
+        KPM_TRY(spmm(ctx, H, &W[rip], &W[ripp], &W[ri], 2.0, -1.0));
+      }
+      KPM_TRY(spmm(ctx, Jb, &W[ri], nullptr, &Rw, 1.0, 0.0));
+      KPM_TRY(from_wide(ctx, &Rw, NRp, (int)P, (char*)ctx->slots_r.ptr + stride * (n - 1),
+                        qstride * NC));
+    }
+    // one contraction of the NC left slots against all Pb * NC right slots of
+    // the batch (one pitch), into the stacked table
+    // mu_b[(q NC + n) + Pb NC m], then reordered into the Pb outer slabs
+    const int64_t ld = Pb * NC;
+    KPM_HIP_TRY(hipMemsetAsync(ctx->mu.ptr, 0, slab * (size_t)Pb * sizeof(double2), ctx->stream));
+    KPM_TRY(launch_contract(ctx, ctx->slots_l.ptr, (int)NC, ctx->slots_r.ptr, (int)ld, Kp, pitch,
+                            prec, p.gemm_mode, scale, ctx->mu.ptr, ld, 0, 0, KPM_PARITY_NONE));
+    host.resize(2 * slab * (size_t)Pb);
+    KPM_HIP_TRY(hipMemcpyAsync(host.data(), ctx->mu.ptr, slab * (size_t)Pb * sizeof(double2),
+                               hipMemcpyDeviceToHost, ctx->stream));
+    KPM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int64_t q = 0; q < Pb; ++q) {
+      double* dst = mu + 2 * slab * (size_t)(b - 1 + q);
+      for (int64_t m = 0; m < NC; ++m)
+        std::copy(host.begin() + 2 * (q * NC + ld * m), host.begin() + 2 * (q * NC + ld * m + NC),
+                  dst + 2 * NC * m);
+    }
+  }
+  return KPM_OK;
+}
+
 static void apply_mn_sym(std::vector<double>& mu, int64_t NC) {  // :1078-1086
   for (int64_t m = 0; m < NC; ++m)
     for (int64_t n = m + 1; n < NC; ++n) {
@@ -232,6 +414,12 @@ int kpm_3d(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, const kpm_csr* Jb,
   KPM_TRY(launch_upload(ctx, ctx->staging.ptr, &l));
   KPM_HIP_TRY(hipMemcpyAsync(ctx->staging.ptr, psi_r, (size_t)(NH * NR) * 16, hipMemcpyHostToDevice, ctx->stream));
   KPM_TRY(launch_upload(ctx, ctx->staging.ptr, &r));
+  // every left order resident: the batched form (left chain once, P outer
+  // orders per right chain) when at least two outer orders fit
+  if (K_left >= NC) {
+    const int64_t P = batch_3d(ctx, p);
+    if (P >= 2) return run_3d_batched(ctx, H, Ja, Jb, Jg, p, P, &l, &r, outer, mu);
+  }
   KPM_TRY(ensure(ctx, ctx->mu, (size_t)(NC * NC) * sizeof(double2)));
   const size_t slab = (size_t)(NC * NC);
   for (int64_t n1 = 1; n1 <= NC; ++n1) {

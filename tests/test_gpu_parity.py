@@ -258,6 +258,28 @@ def test_kpm3d_vs_oracle():
     np.testing.assert_allclose(auto, ref, **TOL)
 
 
+@pytest.mark.parametrize("NR", [1, 3, 8, 33])
+def test_kpm3d_batched_outer_orders(NR):
+    """The engine's kpm_3d with every left order resident runs the outer
+    orders as column blocks of one wide right chain (powers of two, so NC = 11
+    gives a partial last batch), with three distinct current operators, against
+    the oracle's nested form and the explicit nested blocking; the FP32 form is
+    bounded against FP64."""
+    H, Jx, Jy, area = models.haldane_model(5, 5, t2=0.2)
+    Hn = (H / 4.0).tocsc()
+    Jz = ((Jx + 2.0 * Jy) / 3.0).tocsc()  # a third, distinct current
+    NC = 11
+    pl, pr = phases(31 + NR, H.shape[0], NR), phases(32 + NR, H.shape[0], NR)
+    print(f"kpm_3d batched NR={NR} NC={NC}", flush=True)
+    ref = oracle.kpm_3d(Hn, Jx, Jy, Jz, NC, pl, pr, 3, 3)
+    got = K.kpm_3d(Hn, Jx, Jy, Jz, NC, NR, psi_in_l=pl, psi_in_r=pr)
+    np.testing.assert_allclose(got, ref, **TOL)
+    nested = K.kpm_3d(Hn, Jx, Jy, Jz, NC, NR, psi_in_l=pl, psi_in_r=pr, arr_size=NC - 1, right_block=4)
+    np.testing.assert_allclose(nested, ref, **TOL)
+    g32 = K.kpm_3d(Hn, Jx, Jy, Jz, NC, NR, psi_in_l=pl, psi_in_r=pr, precision="f32")
+    assert np.max(np.abs(g32 - got)) < 1e-5
+
+
 @pytest.mark.parametrize("prec,NR", [("f64", 16), ("f32", 32), ("f64", 64), ("f64", 13)])
 def test_kpm2d_wide_blocks_vs_oracle(prec, NR):
     """kpm_2d at wider probe blocks (NRp = 16, 32, 64 complex per row; NR = 13
