@@ -119,7 +119,7 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// The same sums for rows of LPR >= 4 lanes, at most KB <= 16 nonzeros and a
+// The same sums for rows of LPR >= 2 lanes, at most KB <= 16 nonzeros and a
 // single row chunk, with every memory access of a row issued together: with
 // G = min(LPR, 16), lane j of a row holds column index and accumulator entry
 // of nonzeros j, j + G, ... (one coalesced load each), group_bcast hands slot
@@ -136,7 +136,7 @@ __global__ void __launch_bounds__(kBlock)
   constexpr int RPW = 64 / LPR;
   constexpr int MG = LPR >= 16 ? 16 : LPR;
   constexpr int ME = (KB + MG - 1) / MG;
-  static_assert(MG >= 4 && KB <= 16, "rows of 4+ lanes, at most 16 nonzeros");
+  static_assert(MG >= 2 && KB <= 16, "rows of 2+ lanes, at most 16 nonzeros");
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int sub = lane % LPR;
@@ -240,7 +240,7 @@ static int launch_sddmm_t(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* const
   const int64_t nb = (beta[0]->NH + rows_per_block - 1) / rows_per_block;
   const unsigned G = (unsigned)std::min<int64_t>(nb, (int64_t)std::max(ctx->num_cus, 8) * 8);
   const int w = A->max_row_nnz;
-  if (lpr >= 4 && chunks == 1 && w <= 16) {
+  if (lpr >= 2 && chunks == 1 && w <= 16) {
     const int kb = w <= 4 ? 4 : (w <= 8 ? 8 : (w <= 12 ? 12 : 16));
 #define KPM_SDDMM_ROWS(L, KBv)                                                                    \
   if (lpr == L && kb == KBv) {                                                                    \
@@ -250,6 +250,7 @@ static int launch_sddmm_t(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* const
     return KPM_OK;                                                                                \
   }
 #define KPM_SDDMM_KB(L) KPM_SDDMM_ROWS(L, 4) KPM_SDDMM_ROWS(L, 8) KPM_SDDMM_ROWS(L, 12) KPM_SDDMM_ROWS(L, 16)
+    KPM_SDDMM_KB(2)
     KPM_SDDMM_KB(4)
     KPM_SDDMM_KB(8)
     KPM_SDDMM_KB(16)

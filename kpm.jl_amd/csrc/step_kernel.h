@@ -93,12 +93,15 @@ __device__ __forceinline__ void static_for(F&& f) {
 // Lane J of every aligned group of G lanes, broadcast to its group, with DPP
 // moves (VALU only, no LDS traffic).  G = 16: row_newbcast (gfx90a and
 // later); G = 8: row_newbcast of lane J and of lane 8 + J, selected by the
-// half-row; G = 4: quad_perm [J, J, J, J].  Every lane of the wave must be
-// active.
+// half-row; G = 4: quad_perm [J, J, J, J]; G = 2: quad_perm
+// [J, J, 2 + J, 2 + J].  Every lane of the wave must be active.
 template <int G, int J>
 __device__ __forceinline__ int group_bcast(int v) {
-  static_assert(G == 4 || G == 8 || G == 16, "group of 4, 8 or 16 lanes");
-  if constexpr (G == 4) {
+  static_assert(G == 2 || G == 4 || G == 8 || G == 16, "group of 2, 4, 8 or 16 lanes");
+  if constexpr (G == 2) {
+    return __builtin_amdgcn_update_dpp(0, v, J | (J << 2) | ((2 + J) << 4) | ((2 + J) << 6), 0xF,
+                                       0xF, false);
+  } else if constexpr (G == 4) {
     return __builtin_amdgcn_update_dpp(0, v, J * 0x55, 0xF, 0xF, false);
   } else if constexpr (G == 8) {
     const int a = __builtin_amdgcn_update_dpp(0, v, 0x150 + J, 0xF, 0xF, false);
@@ -235,9 +238,9 @@ __global__ void __launch_bounds__(kBlock)
   constexpr int TR = 4 * RPW * UNR;
   constexpr int ND = MODE == MODE_DBL ? 2 : (MODE == MODE_LR ? 1 : 0);
   constexpr int NDA = ND > 0 ? ND : 1;
-  // rows of 4 or more lanes take their nonzeros spread over the lanes
+  // rows of 2 or more lanes take their nonzeros spread over the lanes
   constexpr int MG = LPR >= 16 ? 16 : LPR;
-  constexpr bool kLaneMeta = MG >= 4 && KB <= 16;
+  constexpr bool kLaneMeta = MG >= 2 && KB <= 16;
   constexpr int ME = kLaneMeta ? (KB + MG - 1) / MG : 1;
   using CMeta = std::conditional_t<kLaneMeta, LaneMeta<UNR, ME>, ColMeta<UNR, KB>>;
 

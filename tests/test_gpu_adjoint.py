@@ -120,7 +120,7 @@ def test_site_and_bond_gradients_vs_fd():
 
 
 @pytest.mark.parametrize("w", sorted(BUCKETS))
-@pytest.mark.parametrize("NR", [1, 8, 16, 64, 65])
+@pytest.mark.parametrize("NR", [1, 2, 8, 16, 64, 65])
 def test_adjoint_ragged_rows_vs_tangent(w, NR):
     """Ragged rows (empty up to 41 nonzeros) across row widths of 1 to 64
     lanes and a two-chunk row (NR = 65): every shape of the gradient

@@ -1,7 +1,7 @@
 """Ragged sparsity on the device: Hermitian operators whose rows run from
 empty to 41 nonzeros, so every nonzeros-per-row bucket of the recurrence pass
 (4, 8, 12, 16 and the long-row tail beyond 16) is taken, crossed with probe
-widths whose rows span 1, 4, 8, 16 and 64 lanes.  kpm_1d is compared with the
+widths whose rows span 1, 2, 4, 8, 16 and 64 lanes.  kpm_1d is compared with the
 oracle at 1e-12 (FP64 vectors) and bounded at FP32 rounding (FP32 vectors),
 and the left/right form with distinct probes is checked the same way.
 CUSPARSE's SpMM (ext/KPMCUDAExt.jl:84-88) takes any CSR; the lattice models of
@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("w", sorted(BUCKETS))
-@pytest.mark.parametrize("NR", [1, 4, 8, 16, 64])
+@pytest.mark.parametrize("NR", [1, 2, 4, 8, 16, 64])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_ragged_rows_vs_oracle(w, NR, prec):
     n, NC = 600, 32
