@@ -16,8 +16,9 @@ from .action import (chebyshev_action, chebyshev_action_, chern_marker, chern_ma
                      evolution_tail, evolve, evolve_, fermi_coefficients, fermi_projector)
 from .adjoint import (bond_gradient, dos_loss, kpm_1d_adjoint, moments_rrule,
                       site_gradient)
-from .applications import (chebyshev_lin_trans, chebyshevT, chebyshevT_0, chebyshevT_xn,
-                           fermiFunction, fermiFunctions, gausschebyshevt)
+from .applications import (Gamma_nm, Gamma_nm_mu, chebyshev_lin_trans, chebyshevT,
+                           chebyshevT_0, chebyshevT_accurate, chebyshevT_xn, d_dc_cond, dc_cond0,
+                           dc_cond_single, fermiFunction, fermiFunctions, gausschebyshevt)
 from .applications import dos as dos_raw
 from .applications import dos0 as dos0_raw
 from .applications import kubo_bastin_cond as kubo_bastin_cond_raw

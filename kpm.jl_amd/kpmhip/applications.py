@@ -254,3 +254,144 @@ def kubo_bastin_cond(mu2D, a: float, Ef: float, *, b: float = 0.0, NH: int, area
                                         dptr(out)), "kubo_bastin_cond")
     acc = complex(out[0], out[1])
     return float(-2 * NH / (area * a ** 2) * acc.real)
+
+
+# ---- Kubo–Bastin helpers (src/applications/conductivity.jl, dc_cond_util.jl) ---
+
+
+def chebyshevT_accurate(n, x):
+    """T_n(x) by the three-term recurrence, exact at x = 0 (src/utils/chebyshev.jl:23-29)."""
+    n = int(n)
+    if x == 0:
+        return float(chebyshevT_0(n))
+    t0, t1 = 1.0, float(x)
+    if n == 0:
+        return t0
+    for _ in range(n - 1):
+        t0, t1 = t1, 2.0 * x * t1 - t0
+    return t1
+
+
+def Gamma_nm(n, m, eps):
+    """The Kubo–Bastin kernel Γnm(ε) of Garcia et al., PRL 114, 116602 (2015),
+    elementwise over broadcast n, m (src/applications/dc_cond_util.jl:9-12)."""
+    n = np.asarray(n)
+    m = np.asarray(m)
+    s = math.sqrt(1 - eps * eps)
+    th = math.acos(eps)
+    return ((eps - 1j * m * s) * np.exp(1j * m * th) * chebyshevT(n, eps) +
+            (eps + 1j * n * s) * np.exp(-1j * n * th) * chebyshevT(m, eps))
+
+
+def Gamma_nm_mu(mu_tilde, eps: float, NC: int):
+    """Σ_nm Γnm(ε) μ̃nm by the rank-one factorisation of Γ: two matrix-vector
+    products, no NC x NC Γ (src/applications/dc_cond_util.jl:21-32)."""
+    mt = np.asarray(mu_tilde)
+    if mt.shape != (NC, NC):
+        raise AssertionError("size(μtilde) == (NC, NC)")
+    th = math.acos(eps)
+    s = math.sqrt(1 - eps * eps)
+    ns = np.arange(NC)
+    c = np.cos(ns * th)
+    e = np.exp(1j * ns * th)
+    w_m = (eps - 1j * ns * s) * e
+    w_n = (eps + 1j * ns * s) * np.conj(e)
+    return c @ (mt @ w_m) + w_n @ (mt @ c)
+
+
+def _tmul_rows(A, B):
+    """Row-wise truncated series products of (N, k) coefficient arrays."""
+    k = A.shape[1]
+    out = np.zeros((A.shape[0], k), dtype=np.result_type(A, B))
+    for i in range(k):
+        for j in range(i + 1):
+            out[:, i] += A[:, j] * B[:, i - j]
+    return out
+
+
+def _dsigma_taylor(mu_tilde, a, b, E, order):
+    """Taylor coefficients in E of Re[Σ Γnm μ̃nm] / ((1-x²)² a²), x = (E-b)/a:
+    T_n and U_{n-1} by their recurrences on series, s = √(1-x²) as a series,
+    e^{±inθ} = T_n ± i s U_{n-1}; returns the order-th E derivative."""
+    k = order + 1
+    NC = mu_tilde.shape[0]
+    x = np.zeros(k)
+    x[0] = (E - b) / a
+    if k > 1:
+        x[1] = 1.0 / a
+    one_minus = -_tmul(x, x)
+    one_minus[0] += 1.0
+    s = _tsqrt(one_minus)
+    T = np.zeros((NC, k))
+    U = np.zeros((NC, k))  # U[n] = U_{n-1}(x); U_{-1} = 0
+    T[0, 0] = 1.0
+    if NC > 1:
+        T[1] = x
+        U[1, 0] = 1.0
+    for n in range(2, NC):
+        T[n] = 2.0 * _tmul(x, T[n - 1]) - T[n - 2]
+        U[n] = 2.0 * _tmul(x, U[n - 1]) - U[n - 2]
+    sU = _tmul_rows(np.broadcast_to(s, (NC, k)), U)
+    ns = np.arange(NC)[:, None]
+    e_plus = T + 1j * sU
+    e_minus = T - 1j * sU
+    xs = np.broadcast_to(x, (NC, k))
+    w_m = _tmul_rows(xs - 1j * ns * np.broadcast_to(s, (NC, k)), e_plus)
+    w_n = _tmul_rows(xs + 1j * ns * np.broadcast_to(s, (NC, k)), e_minus)
+    mw = mu_tilde @ w_m
+    mc = mu_tilde @ T
+    acc = _tmul_rows(T.astype(np.complex128), mw).sum(axis=0) + _tmul_rows(w_n, mc).sum(axis=0)
+    denom = _tmul(one_minus, one_minus) * a * a
+    series = _tdiv(acc, denom.astype(np.complex128))
+    return float(np.real(series[order])) * math.factorial(order)
+
+
+def d_dc_cond(mu, a: float, E, *, b: float = 0.0, NC: int = 0, kernel=JacksonKernel,
+              dE_order: int = 0) -> np.ndarray:
+    """Kubo–Bastin integrand dσ(E) = Re[Σ_nm Γnm(x) μ̃nm] / ((1-x²)² a²),
+    x = (E-b)/a, at the energies E (0 outside the band), or its dE_order-th
+    energy derivative (src/applications/conductivity.jl:199-245; the
+    reference's nested ForwardDiff derivatives are Taylor coefficients here)."""
+    mu = np.asarray(mu)
+    E_arr = np.atleast_1d(np.asarray(E, dtype=np.float64))
+    if NC == 0:
+        NC = mu.shape[0]
+    else:
+        if NC > mu.shape[0]:
+            warnings.warn(f"NC={NC} exceeds the maximum size of μ; decreased to {mu.shape[0]}.")
+        NC = min(mu.shape[0], NC)
+    mt = mu2D_apply_kernel_and_h(mu[:NC, :NC], NC, kernel)
+    out = np.zeros(E_arr.shape, dtype=np.float64)
+    idx = np.abs(E_arr - b) < abs(a)
+    for i in np.nonzero(idx)[0]:
+        if dE_order == 0:
+            eps = (E_arr[i] - b) / a
+            out[i] = float(np.real(Gamma_nm_mu(mt, eps, NC) / ((1 - eps * eps) ** 2) / (a * a)))
+        else:
+            out[i] = _dsigma_taylor(mt, a, b, E_arr[i], int(dE_order))
+    return out
+
+
+def dc_cond0(mu, H_rescale_factor, *, kernel=JacksonKernel, NC: int | None = None):
+    """Σ_nm μ̃nm T_n(0) T_m(0) / a by mod-4 index slices (not in physical
+    units; src/applications/conductivity.jl:247-262)."""
+    mu = np.asarray(mu)
+    NC = mu.shape[0] if NC is None else int(NC)
+    mt = mu2D_apply_kernel_and_h(mu[:NC, :NC], NC, kernel)
+    oneone = mt[0::4, 0::4].sum()
+    onethree = mt[0::4, 2::4].sum()
+    threeone = mt[2::4, 0::4].sum()
+    threethree = mt[2::4, 2::4].sum()
+    return (oneone + threethree - onethree - threeone) / H_rescale_factor
+
+
+def dc_cond_single(mu, H_rescale_factor, Ef, *, b=0.0, kernel=JacksonKernel,
+                   NC: int | None = None):
+    """Σ_nm μ̃nm T_n(x_F) T_m(x_F) / a with x_F = (Ef - b)/a (not in physical
+    units; src/applications/conductivity.jl:264-285)."""
+    mu = np.asarray(mu)
+    NC = mu.shape[0] if NC is None else int(NC)
+    ef = (Ef - b) / H_rescale_factor
+    mt = mu2D_apply_kernel_and_h(mu[:NC, :NC], NC, kernel)
+    t = np.array([chebyshevT_accurate(n, ef) for n in range(NC)])
+    return (mt * t[:, None] * t[None, :]).sum() / H_rescale_factor
