@@ -19,6 +19,7 @@ identity mu_n = sum_i |<v_i|psi>|^2 T_n(lambda_i) (test/dos_test.jl:39-43).
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -373,6 +374,54 @@ def ed_evolve(H, psi, t):
     ph = np.exp(-1j * lam * t)
     proj = U.conj().T @ p
     return U @ (ph[:, None] * proj if p.ndim == 2 else ph * proj)
+
+
+def _dense(A):
+    return np.asarray(A.toarray() if sp.issparse(A) else A, dtype=np.complex128)
+
+
+def ed_hall_conductivity_T0(H, Ja, Jb, area, *, Ef):
+    """T = 0 Hall conductivity in e^2/h from the eigenbasis (TKNN form),
+    2 pi (-2/A) sum_{occupied m, empty n} Im[va_mn vb_nm] / (E_m - E_n)^2 with
+    v = J / i (test/ed_reference.jl:517-528)."""
+    E, U = np.linalg.eigh(_dense(H))
+    Va = U.conj().T @ (_dense(Ja) / 1j) @ U
+    Vb = U.conj().T @ (_dense(Jb) / 1j) @ U
+    occ, emp = E <= Ef, E > Ef
+    num = np.imag(Va[np.ix_(occ, emp)] * Vb[np.ix_(emp, occ)].T)
+    den = (E[occ][:, None] - E[emp][None, :]) ** 2
+    return float(2 * np.pi * (-2.0 / area) * np.sum(num / den))
+
+
+def ed_kubo_bastin_broadened(H, Ja, Jb, area, *, Ef, eta, beta=np.inf, grid_N=4000):
+    """Kubo-Bastin conductivity in e^2/h with both delta(e - H) and G(e)
+    broadened into width-eta Lorentzians and the energy integral done on a
+    uniform grid over the occupied window padded by 20 eta
+    (test/ed_reference.jl:349-385)."""
+    E, U = np.linalg.eigh(_dense(H))
+    Va = U.conj().T @ (_dense(Ja) / 1j) @ U
+    Vb = U.conj().T @ (_dense(Jb) / 1j) @ U
+    W = Va * Vb.T  # W_mn = va_mn vb_nm
+
+    def f(e):
+        if np.isinf(beta):
+            return 1.0 if e <= Ef else 0.0
+        return 1.0 / (np.exp(beta * (e - Ef)) + 1.0)
+
+    lo = E.min() - 20 * eta
+    hi = Ef if np.isinf(beta) else E.max() + 20 * eta
+    grid_N = max(grid_N, int(math.ceil(20 * (hi - lo) / eta)))
+    es = np.linspace(lo, hi, grid_N)
+    de = es[1] - es[0]
+    acc = 0j
+    for e in es:
+        Lv = (eta / np.pi) / ((e - E) ** 2 + eta ** 2)
+        Gp = 1.0 / (e - E + 1j * eta) ** 2
+        Gm = 1.0 / (e - E - 1j * eta) ** 2
+        t1 = -(Gp @ (W @ Lv))
+        t2 = -(Lv @ (W @ Gm))
+        acc += f(e) * (t1 - t2) * de
+    return float(2 * np.pi * np.real(1j * acc / area))
 
 
 def ed_chern_marker(H, x, y, Ef, beta=np.inf):

@@ -227,3 +227,22 @@ def test_tangent_vs_finite_differences():
     Hm = H.copy(); Hm.data = H.data - h * d
     fd = (oracle.kpm_1d(Hp, NC, psi) - oracle.kpm_1d(Hm, NC, psi)) / (2 * h)
     np.testing.assert_allclose(dmu, fd, atol=1e-7)
+
+
+def test_ed_hall_and_broadened_kubo_bastin_pins():
+    """ED conductivity references (test/ed_reference.jl:349-385,517-528) on the
+    6 x 6 Haldane torus: the T = 0 Hall value is quantised to the Chern number
+    C = +1 of phi = pi/2 within 0.02 (kubo_bastin_test.jl:38-39), the broadened
+    form agrees within 1e-2 at eta = 0.05 (:43-44), the trivial phase m = 1.6
+    gives its finite-size value near -0.11 (:73-81), and exchanging the two
+    currents flips the sign (sigma_xy = -sigma_yx)."""
+    from kpmhip import models
+    H, Jx, Jy, area = models.haldane_model(6, 6, t=1.0, t2=0.2)
+    s0 = oracle.ed_hall_conductivity_T0(H, Jx, Jy, area, Ef=0.0)
+    assert abs(s0 - 1.0) < 0.02
+    assert abs(oracle.ed_hall_conductivity_T0(H, Jy, Jx, area, Ef=0.0) + s0) < 1e-12
+    sb = oracle.ed_kubo_bastin_broadened(H, Jx, Jy, area, Ef=0.0, eta=0.05)
+    assert abs(sb - s0) < 1e-2
+    Ht, Jxt, Jyt, at = models.haldane_model(6, 6, t=1.0, t2=0.2, m=1.6)
+    st = oracle.ed_hall_conductivity_T0(Ht, Jxt, Jyt, at, Ef=0.0)
+    assert abs(st) < 0.2 and abs(st + 0.11) < 0.02
