@@ -159,6 +159,22 @@ def test_argument_errors():
         K.kpm_2d(H, H, H, 8, 2, psi_in=psi, moment_parity="BOTH")
 
 
+def test_stochastic_trace_unbiased():
+    """test/dos_test.jl:66-78: H = 0.8 |u><u| with u the uniform state; probes
+    that were mean-centred would be orthogonal to u and give mu_1 = 0 instead
+    of 0.8 / D.  Checked with explicit host probes and with the device's own
+    random-phase probes."""
+    D = 8
+    u = np.full(D, 1 / np.sqrt(D))
+    H = sp.csc_matrix(0.8 * np.outer(u, u))
+    NR = 512
+    psi = phases(21, D, NR)
+    mu = K.kpm_1d(H, 4, NR, psi_in=psi)
+    assert abs(mu[1] - 0.8 / D) <= 0.3 * 0.8 / D
+    mu_dev = K.kpm_1d(H, 4, NR)
+    assert abs(mu_dev[1] - 0.8 / D) <= 0.3 * 0.8 / D
+
+
 def test_force_norm_and_nr_parallel():
     """test/dos_test.jl:46-64,95-141"""
     H = c1_model()
