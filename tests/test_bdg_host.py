@@ -225,3 +225,27 @@ def test_bdg_assemble_block_combinations():
     np.testing.assert_allclose(np.sort(ev), -np.sort(ev)[::-1], atol=1e-10)
     evp = np.linalg.eigvalsh(O.bdg_matrix(h, -0.5, np.full(4, 2.0), np.full(4, 0.5), np.exp(0.77j) * D))
     np.testing.assert_allclose(ev, evp, atol=1e-10)
+
+
+def test_assembled_spectrum_particle_hole_and_phase_invariance():
+    """test/bdg_test.jl:372-381 on the operator the device runs: the assembled
+    singlet-convention Nambu matrix of a real hopping with complex Δ has a
+    spectrum symmetric about zero, and a global phase on Δ (also on a bond
+    pairing matrix D) leaves it unchanged."""
+    h, pos, disp = sq33()
+    dfield = np.array([0.45 + 0.2j, 0.3, -0.1j, 0.2 + 0.1j, 0.5, 0.0, 0.1, -0.3j, 0.25])
+    op = K.BdGOperator(h, mu=-0.7, U=2.0, n=np.full(9, 0.4), Delta=dfield, hole_convention="singlet")
+    ev = np.linalg.eigvalsh(K.bdg_assemble(op).toarray())
+    np.testing.assert_allclose(np.sort(ev), -np.sort(ev)[::-1], atol=1e-10)
+    oph = K.BdGOperator(h, mu=-0.7, U=2.0, n=np.full(9, 0.4), Delta=np.exp(0.77j) * dfield,
+                        hole_convention="singlet")
+    np.testing.assert_allclose(np.linalg.eigvalsh(K.bdg_assemble(oph).toarray()), ev, atol=1e-10)
+    rng = np.random.default_rng(4)
+    Dm = rng.standard_normal((9, 9)) * (rng.random((9, 9)) < 0.3)
+    Dm = sp.csc_matrix(Dm + Dm.T)  # even (singlet) bond pairing
+    evD = np.linalg.eigvalsh(K.bdg_assemble(K.BdGOperator(h, mu=0.1, U=0.0, D=Dm,
+                                                          hole_convention="singlet")).toarray())
+    evP = np.linalg.eigvalsh(K.bdg_assemble(K.BdGOperator(h, mu=0.1, U=0.0, D=np.exp(0.4j) * Dm,
+                                                          hole_convention="singlet")).toarray())
+    np.testing.assert_allclose(evP, evD, atol=1e-10)
+    np.testing.assert_allclose(np.sort(evD), -np.sort(evD)[::-1], atol=1e-10)
