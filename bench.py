@@ -503,7 +503,7 @@ def bench_conductivity(args) -> int:
                      "kubo_bastin_ms": round(t_rec * 1e3, 2)}
         print(f"[bench] kpm_2d gemm={gemm}: {wall * 1e3:.1f} ms, sigma_xy={sig:.6f} "
               f"(reconstruction {t_rec * 1e3:.1f} ms)", file=sys.stderr, flush=True)
-    flops = 8.0 * NC * NC * NH * NR
+    flops = 8.0 * NC * NC * NH * NR  # 4 real products per complex multiply-add
     Kc = NH * K.moment._padded_nr(NR, 0)
     rl = {}
     for gemm, code, peak in (("f64", 0, 78.6), ("f32", 1, 157.3), ("bf16x3", 2, 2500.0 / 3)):
@@ -511,13 +511,17 @@ def bench_conductivity(args) -> int:
         K._lib.check(ctx.lib.kpm_bench_contract(ctx.handle, NC, NC, Kc, 0, 0, code, 3, ctypes.byref(ms)),
                      "bench_contract")
         t = ms.value / 3 * 1e-3
-        rl[gemm] = {"bound": "mfma", "achieved": round(flops / t / 1e12, 2), "peak": peak,
-                    "unit": "TFLOP/s", "frac": round(flops / t / 1e12 / peak, 4), "traffic": None,
+        # executed MFMA flops: the FP64 mode runs the 3M complex product
+        # (3 real products per complex multiply-add), the others the 4M form
+        executed = flops * 0.75 if code == 0 else flops
+        rl[gemm] = {"bound": "mfma", "achieved": round(executed / t / 1e12, 2), "peak": peak,
+                    "unit": "TFLOP/s", "frac": round(executed / t / 1e12 / peak, 4), "traffic": None,
                     "kernel": ("k_contract_bf16x3<double>" if code == 2 else
                                f"k_contract<double,{'double' if code == 0 else 'float'}>"),
+                    "complex_gemm_4m_equivalent_tflops": round(flops / t / 1e12, 2),
                     "avg_launch_ms": round(t * 1e3, 3)}
-        print(f"[bench] contraction {gemm}: {t * 1e3:.2f} ms, {flops / t / 1e12:.1f} TF/s",
-              file=sys.stderr, flush=True)
+        print(f"[bench] contraction {gemm}: {t * 1e3:.2f} ms, {executed / t / 1e12:.1f} TF/s executed "
+              f"({flops / t / 1e12:.1f} TF/s as a 4M complex GEMM)", file=sys.stderr, flush=True)
     res = {"metric": "kpm_2d moment-table time (Haldane sigma_xy)", "value": out["f64"]["ms_per_step"],
            "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": 1,
            "ms_per_step": out["f64"]["ms_per_step"], "higher_is_better": False, "scaling": "weak",

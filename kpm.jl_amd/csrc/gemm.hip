@@ -71,7 +71,7 @@ constexpr int kStep = 32;  // reals of K per stage (16 complex values per slot r
 // flight while stage s is multiplied.
 // VT: storage real type of the slots; MT: MFMA arithmetic type.
 template <typename VT, typename MT>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 2)
     k_contract(const VT* __restrict__ L, int nL, const VT* __restrict__ R, int nR, int64_t K2,
                int64_t P2, int64_t kchunk, int tm, int ntiles, int splits,
                double2* __restrict__ part, int ldn, int ldm) {
@@ -141,11 +141,16 @@ __global__ void __launch_bounds__(256)
     }
   };
 
-  acc_t are[2][2], aim[2][2];
+  // FP64 MFMA: the 3M complex product (3 real products per complex MAC
+  // instead of 4): are = sum Ar Lr, aim = sum Ai Li, a3 = sum (Ar+Ai)(Lr-Li);
+  // Re = are + aim, Im = a3 - are + aim.  Other types: interleaved 4M form,
+  // are / aim being Re / Im directly.
+  constexpr bool k3M = sizeof(MT) == 8;
+  acc_t are[2][2], aim[2][2], a3[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) are[a][b] = aim[a][b] = MF::zero();
+    for (int b = 0; b < 2; ++b) are[a][b] = aim[a][b] = a3[a][b] = MF::zero();
 
   // FP32 MFMA mode: the FP32 accumulators are added into FP64 ones every
   // kFlush stages (256 reals of K), so FP32 rounding accumulates over short
@@ -193,16 +198,36 @@ __global__ void __launch_bounds__(256)
         bv[t][j] = pb[j];
       }
     }
+    if constexpr (k3M) {
+      // per complex element c of the lane's 8 reals: P1 += Ar Lr, P2 += Ai Li,
+      // P3 += (Ar + Ai)(Lr - Li)
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+      for (int c = 0; c < 4; ++c) {
 #pragma unroll
-      for (int tm = 0; tm < 2; ++tm) {
-        const MT bre = bv[tm][j];
-        const MT bim = (j & 1) ? bv[tm][j - 1] : -bv[tm][j + 1];
+        for (int tm = 0; tm < 2; ++tm) {
+          const MT lr = bv[tm][2 * c], li = bv[tm][2 * c + 1];
+          const MT ld = lr - li;
 #pragma unroll
-        for (int tn = 0; tn < 2; ++tn) {
-          are[tn][tm] = MF::mac(av[tn][j], bre, are[tn][tm]);
-          aim[tn][tm] = MF::mac(av[tn][j], bim, aim[tn][tm]);
+          for (int tn = 0; tn < 2; ++tn) {
+            const MT ar = av[tn][2 * c], ai = av[tn][2 * c + 1];
+            are[tn][tm] = MF::mac(ar, lr, are[tn][tm]);
+            aim[tn][tm] = MF::mac(ai, li, aim[tn][tm]);
+            a3[tn][tm] = MF::mac(ar + ai, ld, a3[tn][tm]);
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+#pragma unroll
+        for (int tm = 0; tm < 2; ++tm) {
+          const MT bre = bv[tm][j];
+          const MT bim = (j & 1) ? bv[tm][j - 1] : -bv[tm][j + 1];
+#pragma unroll
+          for (int tn = 0; tn < 2; ++tn) {
+            are[tn][tm] = MF::mac(av[tn][j], bre, are[tn][tm]);
+            aim[tn][tm] = MF::mac(av[tn][j], bim, aim[tn][tm]);
+          }
         }
       }
     }
@@ -220,10 +245,18 @@ __global__ void __launch_bounds__(256)
       for (int v = 0; v < 4; ++v) {
         const int n = tile_n + n0 + 16 * tn + MF::row(g, v);
         const int m = tile_m + m0 + 16 * tm + i;
-        if (n < nR && m < nL)
-          out[(size_t)n * ldm + m] =
-              kPromote ? make_double2(fre[tn][tm][v], fim[tn][tm][v])
-                       : make_double2((double)are[tn][tm][v], (double)aim[tn][tm][v]);
+        if (n < nR && m < nL) {
+          double2 o;
+          if constexpr (kPromote) {
+            o = make_double2(fre[tn][tm][v], fim[tn][tm][v]);
+          } else if constexpr (k3M) {
+            const double p1 = (double)are[tn][tm][v], p2 = (double)aim[tn][tm][v];
+            o = make_double2(p1 + p2, ((double)a3[tn][tm][v] - p1) + p2);
+          } else {
+            o = make_double2((double)are[tn][tm][v], (double)aim[tn][tm][v]);
+          }
+          out[(size_t)n * ldm + m] = o;
+        }
       }
 }
 
