@@ -73,6 +73,13 @@ int kpm_device_count(int* count);
 int kpm_ctx_create(int device, kpm_ctx** out);
 int kpm_ctx_destroy(kpm_ctx* ctx);
 int kpm_ctx_synchronize(kpm_ctx* ctx);
+/* No counterpart in the reference (a switch of this engine).  enable = 1
+ * (default): kpm_2d and kpm_3d compute each right-chain step and the Jb
+ * product of the previous order in one pass over H and Jb merged onto the
+ * union of their patterns (operators with sorted rows, two or more lanes per
+ * row).  enable = 0: two separate passes.  The tables are the same either
+ * way; the switch exists to show that. */
+int kpm_ctx_set_dual_pass(kpm_ctx* ctx, int enable);
 /* device_free_memory(::Dev) -- ext/KPMCUDAExt.jl:45, src/device.jl:66-67 */
 int kpm_device_free_memory(kpm_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
 /* name of the device, e.g. for logs */

@@ -12,7 +12,16 @@
 //
 // Slots are stored back to back with a stride of NH*NRp complex values
 // rounded up to 16 (zero tail), so the contraction sees dense vectors.
+//
+// Right chain and Jb in one pass: the order-n step of the right ring already
+// gathers T_{n-1} Ja psi_r, which is what Jb needs for order n-1.  H and Jb
+// merged onto the union of their patterns (each entry carrying both values,
+// zero where one operator has no entry) give both products from one set of
+// gathers (MODE_DUAL), so each order's R is emitted one step late and every
+// right block ends with one standalone Jb pass.  Each product sums its own
+// nonzeros in its own row order, so the tables are those of separate passes.
 #include <algorithm>
+#include <climits>
 #include <vector>
 
 #include "kpm_internal.h"
@@ -25,6 +34,150 @@ static int spmm(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* x, const kpm_bl
                 kpm_block* out, double alpha, double beta) {
   StepLaunch s{A, x, z, out, nullptr, alpha, beta, MODE_NONE};
   return launch_step(ctx, s, tuned_config(ctx, A, x->NR, x->prec, MODE_NONE), nullptr);
+}
+
+// ---- merged operator of the dual pass -----------------------------------------
+
+// entries per row of the union of two operators with strictly increasing
+// columns per row; rpu[i + 1] <- count, maxrow <- the largest count
+__global__ void k_union_count(const int* __restrict__ rpa, const int* __restrict__ cia,
+                              const int* __restrict__ rpb, const int* __restrict__ cib, int n,
+                              int* __restrict__ rpu, int* __restrict__ maxrow) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) rpu[0] = 0;
+  if (i >= n) return;
+  int a = rpa[i], b = rpb[i], c = 0;
+  const int ae = rpa[i + 1], be = rpb[i + 1];
+  while (a < ae || b < be) {
+    const int ca = a < ae ? cia[a] : INT_MAX;
+    const int cb = b < be ? cib[b] : INT_MAX;
+    a += ca <= cb ? 1 : 0;
+    b += cb <= ca ? 1 : 0;
+    ++c;
+  }
+  rpu[i + 1] = c;
+  atomicMax(maxrow, c);
+}
+
+// rp[1..n] counts -> running sums (rp[0] = 0 stays): one workgroup, each
+// thread a contiguous chunk, chunk totals scanned in LDS
+__global__ void __launch_bounds__(1024) k_scan_counts(int* __restrict__ rp, int n) {
+  __shared__ int part[1024];
+  const int tid = threadIdx.x;
+  const int chunk = (n + 1023) / 1024;
+  const int64_t lo = 1 + (int64_t)tid * chunk;
+  const int64_t hi = std::min<int64_t>(lo + chunk, (int64_t)n + 1);
+  int sum = 0;
+  for (int64_t k = lo; k < hi; ++k) sum += rp[k];
+  part[tid] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = tid >= off ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int run = tid > 0 ? part[tid - 1] : 0;
+  for (int64_t k = lo; k < hi; ++k) {
+    run += rp[k];
+    rp[k] = run;
+  }
+}
+
+__device__ inline double2 csr_value(const void* v, int real, int k) {
+  return real ? make_double2(reinterpret_cast<const double*>(v)[k], 0.0)
+              : reinterpret_cast<const double2*>(v)[k];
+}
+
+// union entries in column order: columns, A's values, B's values
+__global__ void k_union_fill(const int* __restrict__ rpa, const int* __restrict__ cia,
+                             const void* __restrict__ va, int a_real, const int* __restrict__ rpb,
+                             const int* __restrict__ cib, const void* __restrict__ vb, int b_real,
+                             int n, const int* __restrict__ rpu, int* __restrict__ ciu,
+                             double2* __restrict__ vua, double2* __restrict__ vub) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int a = rpa[i], b = rpb[i], o = rpu[i];
+  const int ae = rpa[i + 1], be = rpb[i + 1];
+  while (a < ae || b < be) {
+    const int ca = a < ae ? cia[a] : INT_MAX;
+    const int cb = b < be ? cib[b] : INT_MAX;
+    const bool ta = ca <= cb, tb = cb <= ca;
+    ciu[o] = ta ? ca : cb;
+    vua[o] = ta ? csr_value(va, a_real, a) : make_double2(0.0, 0.0);
+    vub[o] = tb ? csr_value(vb, b_real, b) : make_double2(0.0, 0.0);
+    a += ta ? 1 : 0;
+    b += tb ? 1 : 0;
+    ++o;
+  }
+}
+
+// A and B merged (M: A's values on the union pattern, *vals2: B's), built
+// on the device and kept in the context for the same pair of operators.
+// *ok stays false when the dual pass does not apply: unsorted rows, rows of
+// one lane at this block width (no spread metadata), or the dual pass
+// switched off.
+static int dual_operator(kpm_ctx* ctx, const kpm_csr* A, const kpm_csr* B, int64_t NR, int prec,
+                         kpm_csr* M, const void** vals2, bool* ok) {
+  *ok = false;
+  if (!ctx->dual_pass || !A->sorted || !B->sorted || A->n != B->n || A->n < 1) return KPM_OK;
+  const int64_t n = A->n, cap = std::max<int64_t>(A->nnz + B->nnz, 1);
+  if (cap >= INT32_MAX) return KPM_OK;
+  const size_t brp = ((size_t)(n + 1) * 4 + 255) / 256 * 256;
+  const size_t bci = ((size_t)cap * 4 + 255) / 256 * 256;
+  const size_t bv = (size_t)cap * sizeof(double2);
+  char* base = nullptr;
+  if (ctx->merged_key[0] != A->serial || ctx->merged_key[1] != B->serial) {
+    ctx->merged_key[0] = ctx->merged_key[1] = 0;
+    KPM_TRY(ensure(ctx, ctx->merged, brp + bci + 2 * bv + 256));
+    base = (char*)ctx->merged.ptr;
+    int* rpu = (int*)base;
+    int* dmax = (int*)(base + brp + bci + 2 * bv);
+    KPM_HIP_TRY(hipMemsetAsync(dmax, 0, sizeof(int), ctx->stream));
+    const unsigned g = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_union_count, dim3(g), dim3(256), 0, ctx->stream, A->rowptr, A->colind,
+                       B->rowptr, B->colind, (int)n, rpu, dmax);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, ctx->stream, rpu, (int)n);
+    hipLaunchKernelGGL(k_union_fill, dim3(g), dim3(256), 0, ctx->stream, A->rowptr, A->colind,
+                       A->vals, A->is_real, B->rowptr, B->colind, B->vals, B->is_real, (int)n,
+                       rpu, (int*)(base + brp), (double2*)(base + brp + bci),
+                       (double2*)(base + brp + bci + bv));
+    KPM_HIP_TRY(hipGetLastError());
+    int hm[2] = {0, 0};
+    KPM_HIP_TRY(hipMemcpyAsync(&hm[0], dmax, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    KPM_HIP_TRY(hipMemcpyAsync(&hm[1], rpu + n, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    KPM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->merged_max_row = hm[0];
+    ctx->merged_nnz = hm[1];
+    ctx->merged_key[0] = A->serial;
+    ctx->merged_key[1] = B->serial;
+  }
+  base = (char*)ctx->merged.ptr;
+  M->ctx = ctx;
+  M->n = n;
+  M->nnz = ctx->merged_nnz;
+  M->is_real = 0;
+  M->max_row_nnz = ctx->merged_max_row;
+  M->sorted = 1;
+  M->serial = 0;
+  M->rowptr = (int*)base;
+  M->colind = (int*)(base + brp);
+  M->vals = base + brp + bci;
+  *vals2 = base + brp + bci + bv;
+  if (step_geom(M, NR, prec, StepConfig{}).lpr < 2) return KPM_OK;
+  *ok = true;
+  return KPM_OK;
+}
+
+// out = alpha*H*x + beta*z and out2 = Jb*x in one pass over the merged operator
+static int spmm_dual(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* M, const void* vals2,
+                     const kpm_block* x, const kpm_block* z, kpm_block* out, kpm_block* out2,
+                     double alpha, double beta) {
+  StepLaunch s{M, x, z, out, nullptr, alpha, beta, MODE_DUAL};
+  s.gamma = 1.0;
+  s.vals2 = vals2;
+  s.out2 = out2;
+  return launch_step(ctx, s, tuned_config(ctx, H, x->NR, x->prec, MODE_NONE), nullptr);
 }
 
 static void slot_view(void* base, int64_t stride_bytes, int64_t idx, int64_t NH, int64_t NR,
@@ -72,6 +225,10 @@ static int run_2d(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, const kpm_c
   for (int64_t i = 0; i < KL; ++i) slot_view(ctx->slots_l.ptr, stride, i, NH, NR, prec, ctx, &L[i]);
   for (int64_t i = 0; i < KR; ++i) slot_view(ctx->slots_r.ptr, stride, i, NH, NR, prec, ctx, &R[i]);
 
+  kpm_csr M;
+  const void* v2 = nullptr;
+  bool dual = false;
+  KPM_TRY(dual_operator(ctx, H, Jb, NR, prec, &M, &v2, &dual));
   KPM_HIP_TRY(hipMemsetAsync(mu_dev, 0, (size_t)(NC * NC) * sizeof(double2), ctx->stream));
   // left starter (:1013-1018) and right starter (:1021)
   KPM_TRY(kpm_block_copy(&L[0], psi_l));
@@ -93,15 +250,22 @@ static int run_2d(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, const kpm_c
       for (int64_t j = 1; j <= n_count; ++j) {
         const int64_t n = n1 + j - 1;
         const int ri = (int)((n - 1) % 3), rip = (int)((n + 1) % 3), ripp = (int)(n % 3);
+        // dual: this step also emits R of order n - 1 (rr[rip]) when that
+        // order belongs to this block
+        const bool fuse = dual && j >= 2;
         if (n == 1) {
           KPM_TRY(kpm_block_copy(&rr[0], &jpsi0r));
         } else if (n == 2) {
-          KPM_TRY(spmm(ctx, H, &jpsi0r, nullptr, &rr[1], 1.0, 0.0));
+          if (fuse) KPM_TRY(spmm_dual(ctx, H, &M, v2, &jpsi0r, nullptr, &rr[1], &R[j - 2], 1.0, 0.0));
+          else KPM_TRY(spmm(ctx, H, &jpsi0r, nullptr, &rr[1], 1.0, 0.0));
         } else {
-          KPM_TRY(spmm(ctx, H, &rr[rip], &rr[ripp], &rr[ri], 2.0, -1.0));
+          if (fuse) KPM_TRY(spmm_dual(ctx, H, &M, v2, &rr[rip], &rr[ripp], &rr[ri], &R[j - 2], 2.0, -1.0));
+          else KPM_TRY(spmm(ctx, H, &rr[rip], &rr[ripp], &rr[ri], 2.0, -1.0));
         }
-        KPM_TRY(spmm(ctx, Jb, &rr[ri], nullptr, &R[j - 1], 1.0, 0.0));
+        if (!dual) KPM_TRY(spmm(ctx, Jb, &rr[ri], nullptr, &R[j - 1], 1.0, 0.0));
       }
+      if (dual)  // the block's last order
+        KPM_TRY(spmm(ctx, Jb, &rr[(int)((n2 - 1) % 3)], nullptr, &R[n_count - 1], 1.0, 0.0));
       KPM_TRY(launch_contract(ctx, ctx->slots_l.ptr, (int)rep_size, ctx->slots_r.ptr, (int)n_count,
                               Kp, pitch, prec, p.gemm_mode, scale, mu_dev, NC, n1 - 1, m1 - 1,
                               p.parity));
@@ -237,6 +401,10 @@ static int run_3d_batched(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, con
   KPM_TRY(work_block(ctx, ctx->ring[4], NH, NR, prec, &tmp));
   std::vector<kpm_block> L(NC);
   for (int64_t i = 0; i < NC; ++i) slot_view(ctx->slots_l.ptr, stride, i, NH, NR, prec, ctx, &L[i]);
+  kpm_csr M;
+  const void* v2 = nullptr;
+  bool dual = false;
+  KPM_TRY(dual_operator(ctx, H, Jb, NRw, prec, &M, &v2, &dual));
 
   // the left chain, once (:1013-1018 and chebyshev_iter over all NC orders)
   KPM_TRY(kpm_block_copy(&L[0], l));
@@ -263,16 +431,29 @@ static int run_3d_batched(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, con
       KPM_TRY(spmm(ctx, Jg, &outer[ri], nullptr, &tmp, 1.0, 0.0));
       KPM_TRY(to_wide(ctx, &tmp, W[0].d, (int)NRw, (int)q));
     }
-    // the right chain over all outer orders of the batch (:1039-1051)
+    // the right chain over all outer orders of the batch (:1039-1051); with
+    // the dual pass, the step of order n emits Jb W of order n - 1
     for (int64_t n = 1; n <= NC; ++n) {
       const int ri = (int)((n - 1) % 3), rip = (int)((n + 1) % 3), ripp = (int)(n % 3);
       if (n == 2) {
-        KPM_TRY(spmm(ctx, H, &W[0], nullptr, &W[1], 1.0, 0.0));
+        if (dual) KPM_TRY(spmm_dual(ctx, H, &M, v2, &W[0], nullptr, &W[1], &Rw, 1.0, 0.0));
+        else KPM_TRY(spmm(ctx, H, &W[0], nullptr, &W[1], 1.0, 0.0));
       } else if (n >= 3) {
-        KPM_TRY(spmm(ctx, H, &W[rip], &W[ripp], &W[ri], 2.0, -1.0));
+        if (dual) KPM_TRY(spmm_dual(ctx, H, &M, v2, &W[rip], &W[ripp], &W[ri], &Rw, 2.0, -1.0));
+        else KPM_TRY(spmm(ctx, H, &W[rip], &W[ripp], &W[ri], 2.0, -1.0));
       }
-      KPM_TRY(spmm(ctx, Jb, &W[ri], nullptr, &Rw, 1.0, 0.0));
-      KPM_TRY(from_wide(ctx, &Rw, NRp, (int)P, (char*)ctx->slots_r.ptr + stride * (n - 1),
+      if (!dual) {
+        KPM_TRY(spmm(ctx, Jb, &W[ri], nullptr, &Rw, 1.0, 0.0));
+        KPM_TRY(from_wide(ctx, &Rw, NRp, (int)P, (char*)ctx->slots_r.ptr + stride * (n - 1),
+                          qstride * NC));
+      } else if (n >= 2) {
+        KPM_TRY(from_wide(ctx, &Rw, NRp, (int)P, (char*)ctx->slots_r.ptr + stride * (n - 2),
+                          qstride * NC));
+      }
+    }
+    if (dual) {  // the last order
+      KPM_TRY(spmm(ctx, Jb, &W[(int)((NC - 1) % 3)], nullptr, &Rw, 1.0, 0.0));
+      KPM_TRY(from_wide(ctx, &Rw, NRp, (int)P, (char*)ctx->slots_r.ptr + stride * (NC - 1),
                         qstride * NC));
     }
     // one contraction of the NC left slots against all Pb * NC right slots of

@@ -77,6 +77,14 @@ struct kpm_ctx {
   kpm::Buffer pinned;     // page-locked host staging for operator uploads
   kpm::Buffer adjoint;    // adjoint workspace: v | beta ring | alpha store
   kpm::Buffer adj_grad;   // adjoint gradient accumulator (nnz complex)
+  // merged operator of the dual pass: the union pattern of two operators
+  // (rowptr | colind | first values | second values), rebuilt when the pair
+  // of operator serials changes
+  kpm::Buffer merged;
+  uint64_t merged_key[2] = {0, 0};
+  int merged_max_row = 0;
+  int64_t merged_nnz = 0;
+  int dual_pass = 1;      // fuse the kpm_2d / kpm_3d right chain with the Jb pass
 };
 
 struct kpm_csr {
@@ -88,6 +96,8 @@ struct kpm_csr {
   int64_t nnz = 0;
   int is_real = 0;
   int max_row_nnz = 0;
+  int sorted = 0;         // every row's column indices strictly increase
+  uint64_t serial = 0;    // process-unique identity (never reused)
   int* rowptr = nullptr;  // n+1
   int* colind = nullptr;  // nnz
   void* vals = nullptr;   // nnz x (double2 | double)
@@ -123,7 +133,10 @@ int work_block(kpm_ctx* ctx, Buffer& buf, int64_t NH, int64_t NR, int prec, kpm_
 // ---- kernels (spmm.hip) -------------------------------------------------------
 // MODE_DBL_SEG: MODE_DBL on the segmented schedule of the batched engine
 // MODE_SRC: out += gamma * l (no dots), the reverse Clenshaw step of the adjoint
-enum StepMode { MODE_NONE = 0, MODE_DBL = 1, MODE_LR = 2, MODE_DBL_SEG = 3, MODE_SRC = 4 };
+// MODE_DUAL: also out2 = gamma * B * x from the same gathers, where A and B
+// share one (merged) pattern and B's complex values are vals2 (no dots)
+enum StepMode { MODE_NONE = 0, MODE_DBL = 1, MODE_LR = 2, MODE_DBL_SEG = 3, MODE_SRC = 4,
+                MODE_DUAL = 5 };
 
 // One fused pass out <- alpha*A*x + beta*z with optional dot partials:
 //   MODE_DBL: d0 = <x|x>, d1 = <x|out>;  MODE_LR: d0 = <l|out>.
@@ -137,9 +150,11 @@ struct StepLaunch {
   const kpm_block* l;
   double alpha, beta;
   int mode;
-  double gamma = 0.0;                     // MODE_SRC only
+  double gamma = 0.0;                     // MODE_SRC, MODE_DUAL only
   double2* part_out = nullptr;            // dot partials (null: ctx->partials)
   const PendingCombine* pend = nullptr;   // previous step's combine, run in this launch
+  const void* vals2 = nullptr;            // MODE_DUAL: second operator's values (complex)
+  kpm_block* out2 = nullptr;              // MODE_DUAL: second output
 };
 // Launch geometry picked by the tuner for (LPR, rows per lane, grid size).
 struct StepConfig {

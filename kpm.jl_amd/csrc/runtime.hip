@@ -6,6 +6,7 @@
 //   to_device / maybe_to_host of dense blocks   ext/KPMCUDAExt.jl:28,36-39
 //   device_zeros / device_free_memory           ext/KPMCUDAExt.jl:43-45
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -55,6 +56,23 @@ int work_block(kpm_ctx* ctx, Buffer& buf, int64_t NH, int64_t NR, int prec, kpm_
   return KPM_OK;
 }
 
+// identity and row order of a new operator: the serial keys the merged
+// operator of the dual pass (a freed operator's address can be reused, its
+// serial cannot), and a merge of two operators needs strictly increasing
+// column indices in every row
+static void stamp_csr(kpm_csr* A, const int* rp, const int* ci) {
+  static std::atomic<uint64_t> next{1};
+  A->serial = next.fetch_add(1);
+  bool sorted = true;
+  for (int64_t i = 0; i < A->n && sorted; ++i)
+    for (int k = rp[i] + 1; k < rp[i + 1]; ++k)
+      if (ci[k] <= ci[k - 1]) {
+        sorted = false;
+        break;
+      }
+  A->sorted = sorted ? 1 : 0;
+}
+
 static void free_buffer(Buffer& b) {
   if (b.ptr) (void)hipFree(b.ptr);
   b.ptr = nullptr;
@@ -86,6 +104,7 @@ int upload_csr(kpm_ctx* ctx, int64_t n, const std::vector<int64_t>& rp,
   A->nnz = nnz;
   A->is_real = is_real ? 1 : 0;
   A->max_row_nnz = maxrow;
+  stamp_csr(A, rp32.data(), ci32.data());
   const size_t vbytes = (size_t)std::max<int64_t>(nnz, 1) * (is_real ? sizeof(double) : 2 * sizeof(double));
   hipError_t e = hipMalloc(&A->rowptr, sizeof(int) * (n + 1));
   if (e == hipSuccess) e = hipMalloc(&A->colind, sizeof(int) * std::max<int64_t>(nnz, 1));
@@ -196,6 +215,7 @@ int csr_from_csc_staged(kpm_ctx* ctx, int64_t n, const int64_t* colptr, const in
   A->nnz = nnz;
   A->is_real = is_real ? 1 : 0;
   A->max_row_nnz = maxrow;
+  stamp_csr(A, rp, ci);
   hipError_t e = hipMalloc(&A->rowptr, brp);
   if (e == hipSuccess) e = hipMalloc(&A->colind, bci);
   if (e == hipSuccess) e = hipMalloc(&A->vals, bv);
@@ -293,6 +313,7 @@ int kpm_ctx_destroy(kpm_ctx* ctx) {
   for (auto& b : ctx->ring) free_buffer(b);
   free_buffer(ctx->adjoint);
   free_buffer(ctx->adj_grad);
+  free_buffer(ctx->merged);
   if (ctx->pinned.ptr) (void)hipHostFree(ctx->pinned.ptr);
   for (auto& ev : ctx->events)
     if (ev) (void)hipEventDestroy(ev);
@@ -304,6 +325,12 @@ int kpm_ctx_destroy(kpm_ctx* ctx) {
 int kpm_ctx_synchronize(kpm_ctx* ctx) {
   if (!ctx) return fail(KPM_EINVAL, "null context");
   KPM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return KPM_OK;
+}
+
+int kpm_ctx_set_dual_pass(kpm_ctx* ctx, int enable) {
+  if (!ctx) return fail(KPM_EINVAL, "null context");
+  ctx->dual_pass = enable ? 1 : 0;
   return KPM_OK;
 }
 

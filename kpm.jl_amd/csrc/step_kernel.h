@@ -210,10 +210,12 @@ struct ColMeta {
 // row, column index and value, loaded coalesced (E loads per array instead of
 // KB same-address loads per lane); group_bcast hands slot jj to every lane of
 // the row.  Rows wider than 16 lanes load the same nonzeros in each DPP row.
-template <int UNR, int E>
+// DUAL adds the values of a second operator on the same pattern.
+template <int UNR, int E, bool DUAL = false>
 struct LaneMeta {
   int c[UNR][E];
   double hr[UNR][E], hi[UNR][E];
+  double gr[DUAL ? UNR : 1][DUAL ? E : 1], gi[DUAL ? UNR : 1][DUAL ? E : 1];
 };
 
 // Segmented schedule (SEG = true, batched realisations): workgroup b owns the
@@ -222,6 +224,13 @@ struct LaneMeta {
 // dot partials to slot seg0[b] + k whenever the realisation changes.  The
 // change is uniform over the workgroup, so the barrier inside the flush is
 // reached by all of its threads.
+//
+// Dual pass (MODE == MODE_DUAL): vals2_ holds the complex values of a second
+// operator B on the same pattern, and the pass also writes out2 = gamma * B x
+// from the gathers of x it already holds.  With A and B merged onto the union
+// of their patterns, each output sums its own nonzeros in its own row order
+// (the other operator's entries weigh exactly zero), so out and out2 are those
+// of two separate passes.
 template <int PREC, int LPR, int UNR, int KB, bool REALH, int MODE, bool SEG>
 __global__ void __launch_bounds__(kBlock)
     k_step(const int* __restrict__ rowptr, const int* __restrict__ colind,
@@ -230,7 +239,8 @@ __global__ void __launch_bounds__(kBlock)
            const typename Word<PREC>::T* __restrict__ l, double2* __restrict__ partials, int NH,
            int RW, int NRp, int ntiles, double alpha, double beta, int has_tail,
            const int* __restrict__ tile_lo, const int* __restrict__ treal,
-           const int* __restrict__ seg0, double gamma, PendingCombine pc) {
+           const int* __restrict__ seg0, double gamma, PendingCombine pc,
+           const void* __restrict__ vals2_, typename Word<PREC>::T* __restrict__ out2) {
   using WT = Word<PREC>;
   using T = typename WT::T;
   constexpr int CPL = WT::CPL;
@@ -242,7 +252,9 @@ __global__ void __launch_bounds__(kBlock)
   constexpr int MG = LPR >= 16 ? 16 : LPR;
   constexpr bool kLaneMeta = MG >= 2 && KB <= 16;
   constexpr int ME = kLaneMeta ? (KB + MG - 1) / MG : 1;
-  using CMeta = std::conditional_t<kLaneMeta, LaneMeta<UNR, ME>, ColMeta<UNR, KB>>;
+  constexpr bool kDual = MODE == MODE_DUAL;
+  static_assert(!kDual || (kLaneMeta && !REALH && !SEG), "dual pass: complex values, spread metadata");
+  using CMeta = std::conditional_t<kLaneMeta, LaneMeta<UNR, ME, kDual>, ColMeta<UNR, KB>>;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -255,6 +267,7 @@ __global__ void __launch_bounds__(kBlock)
 
   const double2* cvals = reinterpret_cast<const double2*>(vals_);
   const double* rvals = reinterpret_cast<const double*>(vals_);
+  const double2* cvals2 = reinterpret_cast<const double2*>(vals2_);
 
   double accr[NDA][CPL], acci[NDA][CPL];
 #pragma unroll
@@ -341,6 +354,11 @@ __global__ void __launch_bounds__(kBlock)
             Cm.hr[u][e] = v ? h.x : 0.0;
             Cm.hi[u][e] = v ? h.y : 0.0;
           }
+          if constexpr (kDual) {
+            const double2 g = cvals2[k];
+            Cm.gr[u][e] = v ? g.x : 0.0;
+            Cm.gi[u][e] = v ? g.y : 0.0;
+          }
         }
     } else {
 #pragma unroll
@@ -416,6 +434,7 @@ __global__ void __launch_bounds__(kBlock)
   for (; t < t1; t += Bx) {
     // 1. this tile's values, gathers and row-local operands, all in flight
     double hr[UNR][KB], hi[UNR][KB];
+    double gr[kDual ? UNR : 1][kDual ? KB : 1], gi[kDual ? UNR : 1][kDual ? KB : 1];
     T xv[UNR][KB];
     if constexpr (kLaneMeta) {  // every lane is active here (uniform loop)
 #pragma unroll
@@ -426,6 +445,10 @@ __global__ void __launch_bounds__(kBlock)
           const int c = group_bcast<MG, src>(C0.c[u][e]);
           hr[u][jj] = group_bcast<MG, src>(C0.hr[u][e]);
           hi[u][jj] = REALH ? 0.0 : group_bcast<MG, src>(C0.hi[u][e]);
+          if constexpr (kDual) {
+            gr[u][jj] = group_bcast<MG, src>(C0.gr[u][e]);
+            gi[u][jj] = group_bcast<MG, src>(C0.gi[u][e]);
+          }
           xv[u][jj] = x[(size_t)c * RW + wcol];
         });
     } else {
@@ -464,12 +487,19 @@ __global__ void __launch_bounds__(kBlock)
 
     // 3. accumulate
     double yr[UNR][CPL], yi[UNR][CPL];
+    double y2r[kDual ? UNR : 1][CPL], y2i[kDual ? UNR : 1][CPL];
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
 #pragma unroll
       for (int c = 0; c < CPL; ++c) yr[u][c] = yi[u][c] = 0.0;
 #pragma unroll
       for (int jj = 0; jj < KB; ++jj) mac(hr[u][jj], hi[u][jj], xv[u][jj], yr[u], yi[u]);
+      if constexpr (kDual) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) y2r[u][c] = y2i[u][c] = 0.0;
+#pragma unroll
+        for (int jj = 0; jj < KB; ++jj) mac(gr[u][jj], gi[u][jj], xv[u][jj], y2r[u], y2i[u]);
+      }
     }
     if (has_tail) {  // uniform: rows longer than KB exist in this operator
 #pragma unroll
@@ -485,7 +515,12 @@ __global__ void __launch_bounds__(kBlock)
             h_r = h.x;
             h_i = h.y;
           }
-          mac(h_r, h_i, x[(size_t)colind[k] * RW + wcol], yr[u], yi[u]);
+          const T xw = x[(size_t)colind[k] * RW + wcol];
+          mac(h_r, h_i, xw, yr[u], yi[u]);
+          if constexpr (kDual) {
+            const double2 g = cvals2[k];
+            mac(g.x, g.y, xw, y2r[u], y2i[u]);
+          }
         }
     }
 
@@ -511,6 +546,15 @@ __global__ void __launch_bounds__(kBlock)
       }
       const T ow = WT::pack(orr, oii);
       if (live) st_stream(out + (size_t)R0.row[u] * RW + wcol, ow);
+      if constexpr (kDual) {
+        double pr[CPL], pi[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+          pr[c] = gamma * y2r[u][c];
+          pi[c] = gamma * y2i[u][c];
+        }
+        if (live) st_stream(out2 + (size_t)R0.row[u] * RW + wcol, WT::pack(pr, pi));
+      }
       if constexpr (MODE == MODE_DBL || MODE == MODE_LR) {
         double sr[CPL], si[CPL], ar[CPL], ai[CPL];
         WT::unpack(ow, sr, si);
@@ -562,18 +606,18 @@ __global__ void __launch_bounds__(kBlock)
 using StepLauncher = void (*)(dim3, hipStream_t, const int*, const int*, const void*, const void*,
                               const void*, void*, const void*, double2*, int, int, int, int,
                               double, double, int, const int*, const int*, const int*, double,
-                              const PendingCombine&);
+                              const PendingCombine&, const void*, void*);
 
 template <int PREC, int LPR, int UNR, int KB, bool REALH, int MODE, bool SEG>
 static void launch_step_t(dim3 grid, hipStream_t st, const int* rp, const int* ci, const void* v,
                           const void* x, const void* z, void* out, const void* l, double2* part,
                           int NH, int RW, int NRp, int ntiles, double alpha, double beta,
                           int has_tail, const int* tile_lo, const int* treal, const int* seg0,
-                          double gamma, const PendingCombine& pc) {
+                          double gamma, const PendingCombine& pc, const void* v2, void* out2) {
   using T = typename Word<PREC>::T;
   hipLaunchKernelGGL((k_step<PREC, LPR, UNR, KB, REALH, MODE, SEG>), grid, dim3(kBlock), 0, st, rp,
                      ci, v, (const T*)x, (const T*)z, (T*)out, (const T*)l, part, NH, RW, NRp,
-                     ntiles, alpha, beta, has_tail, tile_lo, treal, seg0, gamma, pc);
+                     ntiles, alpha, beta, has_tail, tile_lo, treal, seg0, gamma, pc, v2, (T*)out2);
 }
 
 template <int PREC, int LPR, int UNR, int KB, bool REALH>
@@ -583,6 +627,9 @@ static StepLauncher pick_mode(int mode) {
     case MODE_LR: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_LR, false>;
     case MODE_DBL_SEG: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_DBL, true>;
     case MODE_SRC: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_SRC, false>;
+    case MODE_DUAL:  // complex merged values on the spread-metadata path only
+      if constexpr (!REALH && LPR >= 2) return &launch_step_t<PREC, LPR, UNR, KB, false, MODE_DUAL, false>;
+      return nullptr;
     default: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_NONE, false>;
   }
 }

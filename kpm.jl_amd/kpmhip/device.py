@@ -91,6 +91,11 @@ class Context:
     def synchronize(self):
         check(self.lib.kpm_ctx_synchronize(self.handle), "kpm_ctx_synchronize")
 
+    def set_dual_pass(self, enable: bool):
+        """Fuse the kpm_2d / kpm_3d right-chain step with the Jb product of the
+        previous order (default) or run them as two passes; same tables."""
+        check(self.lib.kpm_ctx_set_dual_pass(self.handle, 1 if enable else 0), "kpm_ctx_set_dual_pass")
+
     def free_memory(self) -> int:
         f = C.c_size_t(0)
         t = C.c_size_t(0)
