@@ -38,14 +38,32 @@ static int spmm(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* x, const kpm_bl
 
 // ---- merged operator of the dual pass -----------------------------------------
 
-// entries per row of the union of two operators with strictly increasing
-// columns per row; rpu[i + 1] <- count, maxrow <- the largest count
-__global__ void k_union_count(const int* __restrict__ rpa, const int* __restrict__ cia,
-                              const int* __restrict__ rpb, const int* __restrict__ cib, int n,
-                              int* __restrict__ rpu, int* __restrict__ maxrow) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) rpu[0] = 0;
-  if (i >= n) return;
+// The union's row pointers come from a three-kernel scan: k_union_count
+// writes each row's entry count and each 256-row block's total,
+// k_scan_blocks turns the block totals into block offsets, and k_union_fill
+// scans its block's counts in LDS, adds the block offset and writes the
+// entries.
+
+constexpr int kMergeBlock = 256;
+
+// inclusive scan of one value per thread over a 256-thread workgroup
+__device__ inline int block_scan(int v, int* sh) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int off = 1; off < kMergeBlock; off <<= 1) {
+    const int u = tid >= off ? sh[tid - off] : 0;
+    __syncthreads();
+    sh[tid] += u;
+    __syncthreads();
+  }
+  return sh[tid];
+}
+
+// entries of row i of the union of two operators with strictly increasing
+// columns per row
+__device__ inline int union_count(const int* __restrict__ rpa, const int* __restrict__ cia,
+                                  const int* __restrict__ rpb, const int* __restrict__ cib, int i) {
   int a = rpa[i], b = rpb[i], c = 0;
   const int ae = rpa[i + 1], be = rpb[i + 1];
   while (a < ae || b < be) {
@@ -55,32 +73,39 @@ __global__ void k_union_count(const int* __restrict__ rpa, const int* __restrict
     b += cb <= ca ? 1 : 0;
     ++c;
   }
-  rpu[i + 1] = c;
-  atomicMax(maxrow, c);
+  return c;
 }
 
-// rp[1..n] counts -> running sums (rp[0] = 0 stays): one workgroup, each
-// thread a contiguous chunk, chunk totals scanned in LDS
-__global__ void __launch_bounds__(1024) k_scan_counts(int* __restrict__ rp, int n) {
-  __shared__ int part[1024];
-  const int tid = threadIdx.x;
-  const int chunk = (n + 1023) / 1024;
-  const int64_t lo = 1 + (int64_t)tid * chunk;
-  const int64_t hi = std::min<int64_t>(lo + chunk, (int64_t)n + 1);
-  int sum = 0;
-  for (int64_t k = lo; k < hi; ++k) sum += rp[k];
-  part[tid] = sum;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int v = tid >= off ? part[tid - off] : 0;
-    __syncthreads();
-    part[tid] += v;
-    __syncthreads();
+// cnt[i] <- entries of union row i, bsum[block] <- the block's total,
+// maxrow <- the largest row
+__global__ void __launch_bounds__(kMergeBlock)
+    k_union_count(const int* __restrict__ rpa, const int* __restrict__ cia,
+                  const int* __restrict__ rpb, const int* __restrict__ cib, int n,
+                  int* __restrict__ cnt, int* __restrict__ bsum, int* __restrict__ maxrow) {
+  __shared__ int sh[kMergeBlock];
+  const int i = blockIdx.x * kMergeBlock + threadIdx.x;
+  const int c = i < n ? union_count(rpa, cia, rpb, cib, i) : 0;
+  if (i < n) {
+    cnt[i] = c;
+    atomicMax(maxrow, c);
   }
-  int run = tid > 0 ? part[tid - 1] : 0;
-  for (int64_t k = lo; k < hi; ++k) {
-    run += rp[k];
-    rp[k] = run;
+  const int tot = block_scan(c, sh);
+  if (threadIdx.x == kMergeBlock - 1) bsum[blockIdx.x] = tot;
+}
+
+// bsum[0..nb) totals -> exclusive offsets, in place; one workgroup, each
+// thread a contiguous chunk
+__global__ void __launch_bounds__(kMergeBlock) k_scan_blocks(int* __restrict__ bsum, int nb) {
+  __shared__ int sh[kMergeBlock];
+  const int chunk = (nb + kMergeBlock - 1) / kMergeBlock;
+  const int lo = min((int)threadIdx.x * chunk, nb), hi = min(lo + chunk, nb);
+  int s = 0;
+  for (int k = lo; k < hi; ++k) s += bsum[k];
+  int run = block_scan(s, sh) - s;
+  for (int k = lo; k < hi; ++k) {
+    const int v = bsum[k];
+    bsum[k] = run;
+    run += v;
   }
 }
 
@@ -89,15 +114,22 @@ __device__ inline double2 csr_value(const void* v, int real, int k) {
               : reinterpret_cast<const double2*>(v)[k];
 }
 
-// union entries in column order: columns, A's values, B's values
-__global__ void k_union_fill(const int* __restrict__ rpa, const int* __restrict__ cia,
-                             const void* __restrict__ va, int a_real, const int* __restrict__ rpb,
-                             const int* __restrict__ cib, const void* __restrict__ vb, int b_real,
-                             int n, const int* __restrict__ rpu, int* __restrict__ ciu,
-                             double2* __restrict__ vua, double2* __restrict__ vub) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// union row pointers (rpu[0] = 0, rpu[i + 1] = end of row i) and entries in
+// column order: columns, A's values, B's values
+__global__ void __launch_bounds__(kMergeBlock)
+    k_union_fill(const int* __restrict__ rpa, const int* __restrict__ cia,
+                 const void* __restrict__ va, int a_real, const int* __restrict__ rpb,
+                 const int* __restrict__ cib, const void* __restrict__ vb, int b_real, int n,
+                 const int* __restrict__ cnt, const int* __restrict__ boff, int* __restrict__ rpu,
+                 int* __restrict__ ciu, double2* __restrict__ vua, double2* __restrict__ vub) {
+  __shared__ int sh[kMergeBlock];
+  const int i = blockIdx.x * kMergeBlock + threadIdx.x;
+  const int c = i < n ? cnt[i] : 0;
+  const int end = boff[blockIdx.x] + block_scan(c, sh);
+  if (i == 0) rpu[0] = 0;
   if (i >= n) return;
-  int a = rpa[i], b = rpb[i], o = rpu[i];
+  rpu[i + 1] = end;
+  int a = rpa[i], b = rpb[i], o = end - c;
   const int ae = rpa[i + 1], be = rpb[i + 1];
   while (a < ae || b < be) {
     const int ca = a < ae ? cia[a] : INT_MAX;
@@ -126,22 +158,27 @@ static int dual_operator(kpm_ctx* ctx, const kpm_csr* A, const kpm_csr* B, int64
   const size_t brp = ((size_t)(n + 1) * 4 + 255) / 256 * 256;
   const size_t bci = ((size_t)cap * 4 + 255) / 256 * 256;
   const size_t bv = (size_t)cap * sizeof(double2);
+  const int64_t nb = (n + kMergeBlock - 1) / kMergeBlock;
+  // scratch after the operator: row counts (n), block offsets (nb), max row
+  const size_t bcnt = ((size_t)n * 4 + 255) / 256 * 256;
+  const size_t bblk = ((size_t)nb * 4 + 255) / 256 * 256;
   char* base = nullptr;
   if (ctx->merged_key[0] != A->serial || ctx->merged_key[1] != B->serial) {
     ctx->merged_key[0] = ctx->merged_key[1] = 0;
-    KPM_TRY(ensure(ctx, ctx->merged, brp + bci + 2 * bv + 256));
+    KPM_TRY(ensure(ctx, ctx->merged, brp + bci + 2 * bv + bcnt + bblk + 256));
     base = (char*)ctx->merged.ptr;
     int* rpu = (int*)base;
-    int* dmax = (int*)(base + brp + bci + 2 * bv);
+    int* cnt = (int*)(base + brp + bci + 2 * bv);
+    int* bsum = (int*)(base + brp + bci + 2 * bv + bcnt);
+    int* dmax = (int*)(base + brp + bci + 2 * bv + bcnt + bblk);
     KPM_HIP_TRY(hipMemsetAsync(dmax, 0, sizeof(int), ctx->stream));
-    const unsigned g = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_union_count, dim3(g), dim3(256), 0, ctx->stream, A->rowptr, A->colind,
-                       B->rowptr, B->colind, (int)n, rpu, dmax);
-    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, ctx->stream, rpu, (int)n);
-    hipLaunchKernelGGL(k_union_fill, dim3(g), dim3(256), 0, ctx->stream, A->rowptr, A->colind,
-                       A->vals, A->is_real, B->rowptr, B->colind, B->vals, B->is_real, (int)n,
-                       rpu, (int*)(base + brp), (double2*)(base + brp + bci),
-                       (double2*)(base + brp + bci + bv));
+    hipLaunchKernelGGL(k_union_count, dim3((unsigned)nb), dim3(kMergeBlock), 0, ctx->stream,
+                       A->rowptr, A->colind, B->rowptr, B->colind, (int)n, cnt, bsum, dmax);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kMergeBlock), 0, ctx->stream, bsum, (int)nb);
+    hipLaunchKernelGGL(k_union_fill, dim3((unsigned)nb), dim3(kMergeBlock), 0, ctx->stream,
+                       A->rowptr, A->colind, A->vals, A->is_real, B->rowptr, B->colind, B->vals,
+                       B->is_real, (int)n, cnt, bsum, rpu, (int*)(base + brp),
+                       (double2*)(base + brp + bci), (double2*)(base + brp + bci + bv));
     KPM_HIP_TRY(hipGetLastError());
     int hm[2] = {0, 0};
     KPM_HIP_TRY(hipMemcpyAsync(&hm[0], dmax, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
