@@ -273,16 +273,17 @@ __global__ void __launch_bounds__(256, 2)
 // the hi and lo planes as bf16 pairs.
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ uint32_t bf16_rne(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return u >> 16;
-}
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef __bf16 b2v __attribute__((ext_vector_type(2)));
 
-// (hi, lo) bf16 bit patterns of x
-__device__ __forceinline__ void split_bf16(float x, uint32_t& hi, uint32_t& lo) {
-  hi = bf16_rne(x);
-  lo = bf16_rne(x - __uint_as_float(hi << 16));
+// (hi, lo) bf16 splits of the pair (x0, x1), packed x0 in the low half: the
+// FP32 -> BF16 conversions are gfx950's packed v_cvt_pk_bf16_f32 (round to
+// nearest even), the residual x - hi is exact in FP32
+__device__ __forceinline__ void split_bf16_pair(float x0, float x1, uint32_t& hi, uint32_t& lo) {
+  hi = __builtin_bit_cast(uint32_t, __builtin_convertvector((f2v){x0, x1}, b2v));
+  const float r0 = x0 - __uint_as_float(hi << 16);
+  const float r1 = x1 - __uint_as_float(hi & 0xFFFF0000u);
+  lo = __builtin_bit_cast(uint32_t, __builtin_convertvector((f2v){r0, r1}, b2v));
 }
 
 // L'' = (-Li, Lr) on packed (Re, Im) bf16 pairs: swap the halves, negate the new low half
@@ -350,11 +351,10 @@ __global__ void __launch_bounds__(256)
       uint32_t* pl = &llo[buf][op][row * LDB + wi * EPW / 2];
 #pragma unroll
       for (int t = 0; t < EPW / 2; ++t) {
-        uint32_t h0, l0, h1, l1;
-        split_bf16((float)e[2 * t], h0, l0);
-        split_bf16((float)e[2 * t + 1], h1, l1);
-        ph[t] = h0 | (h1 << 16);
-        pl[t] = l0 | (l1 << 16);
+        uint32_t h, l;
+        split_bf16_pair((float)e[2 * t], (float)e[2 * t + 1], h, l);
+        ph[t] = h;
+        pl[t] = l;
       }
     }
   };
@@ -495,11 +495,10 @@ __global__ void __launch_bounds__(256)
       uint32_t* pl = &llo[buf][op][row * LDB + wi * EPW / 2];
 #pragma unroll
       for (int t = 0; t < EPW / 2; ++t) {
-        uint32_t h0, l0, h1, l1;
-        split_bf16((float)e[2 * t], h0, l0);
-        split_bf16((float)e[2 * t + 1], h1, l1);
-        ph[t] = h0 | (h1 << 16);
-        pl[t] = l0 | (l1 << 16);
+        uint32_t h, l;
+        split_bf16_pair((float)e[2 * t], (float)e[2 * t + 1], h, l);
+        ph[t] = h;
+        pl[t] = l;
       }
     }
   };
