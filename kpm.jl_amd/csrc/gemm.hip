@@ -433,25 +433,30 @@ __global__ void __launch_bounds__(256)
       }
 }
 
-// Large-table form of the split-BF16 contraction: a 128 x 128 workgroup tile,
-// four waves of 64 x 64 (4 x 4 MFMA tiles), so every operand fragment read
-// from LDS feeds 4 tiles instead of 2 and the LDS traffic per MFMA halves.
-// The MFMA accumulators hold FP32 sums over one K-chunk of at most
-// kChunkBF16 reals (no FP64 registers, which is what frees the room for the
-// larger tile); the host sizes the split so that chunks stay within that
-// bound, and the FP32 chunk partials are added in FP64 in a fixed order.
+// Large-table form of the split-BF16 contraction: a 128 x 128 workgroup tile
+// on eight waves of 64 x 32 (4 x 2 MFMA tiles), two waves per SIMD, so one
+// wave's VALU split of the next stage runs beside the other's MFMAs.  Each
+// operand fragment read from LDS feeds 2 or 4 tiles.  The MFMA accumulators
+// hold FP32 sums over one K-chunk of at most kChunkBF16 reals (no FP64
+// registers, which is what frees the room for two waves); the host sizes the
+// split so that chunks stay within that bound, and the FP32 chunk partials
+// are added in FP64 in a fixed order.
 constexpr int64_t kChunkBF16 = 8192;
+constexpr int kT128Threads = 512;
 
 template <typename VT>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kT128Threads)
     k_contract_bf16x3_t128(const VT* __restrict__ L, int nL, const VT* __restrict__ R, int nR,
                            int64_t K2, int64_t P2, int64_t kchunk, int tm, int ntiles, int splits,
                            float2* __restrict__ part, int ldn, int ldm) {
   constexpr int TT = 128;
+  constexpr int NT = kT128Threads;
+  constexpr int WN = 4, WM = 2;  // MFMA tiles per wave along n (R rows) and m (L rows)
   constexpr int LDB = kStep / 2 + 4;
   constexpr int WPR = kStep * (int)sizeof(VT) / 16;
   constexpr int EPW = 16 / (int)sizeof(VT);
-  constexpr int NW = 2 * TT * WPR / 256;
+  constexpr int NW = 2 * TT * WPR / NT;
+  static_assert((TT / (16 * WN)) * (TT / (16 * WM)) == NT / 64, "waves cover the tile");
   __shared__ uint32_t lhi[2][2][TT * LDB], llo[2][2][TT * LDB];
 
   int tile, z;
@@ -466,7 +471,8 @@ __global__ void __launch_bounds__(256)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, i = lane & 15;
   const int tile_n = (tile / tm) * TT, tile_m = (tile % tm) * TT;
-  const int n0 = (wave >> 1) * 64, m0 = (wave & 1) * 64;
+  constexpr int WAVES_M = TT / (16 * WM);
+  const int n0 = (wave / WAVES_M) * (16 * WN), m0 = (wave % WAVES_M) * (16 * WM);
   const int64_t kbeg = (int64_t)z * kchunk;
   const int64_t kend = min(kbeg + kchunk, K2);
   const int nstage = kend > kbeg ? (int)((kend - kbeg) / kStep) : 0;
@@ -474,7 +480,7 @@ __global__ void __launch_bounds__(256)
   const VT* src[NW];
 #pragma unroll
   for (int q = 0; q < NW; ++q) {
-    const int w = tid + 256 * q;
+    const int w = tid + NT * q;
     const int op = w / (TT * WPR), rem = w % (TT * WPR);
     const int row = rem / WPR, wi = rem % WPR;
     const int grow = op == 0 ? min(tile_n + row, nR - 1) : min(tile_m + row, nL - 1);
@@ -487,7 +493,7 @@ __global__ void __launch_bounds__(256)
   auto lstore = [&](int buf, const float4* regs) {
 #pragma unroll
     for (int q = 0; q < NW; ++q) {
-      const int w = tid + 256 * q;
+      const int w = tid + NT * q;
       const int op = w / (TT * WPR), rem = w % (TT * WPR);
       const int row = rem / WPR, wi = rem % WPR;
       const VT* e = reinterpret_cast<const VT*>(&regs[q]);
@@ -503,11 +509,11 @@ __global__ void __launch_bounds__(256)
     }
   };
 
-  f4 are[4][4], aim[4][4];
+  f4 are[WN][WM], aim[WN][WM];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < WN; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) are[a][b] = aim[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int b = 0; b < WM; ++b) are[a][b] = aim[a][b] = f4{0.f, 0.f, 0.f, 0.f};
 
   float4 regs[NW];
   if (nstage > 0) {
@@ -518,21 +524,21 @@ __global__ void __launch_bounds__(256)
   for (int st = 0; st < nstage; ++st) {
     const int buf = st & 1;
     if (st + 1 < nstage) gload(st + 1, regs);
-    uint4 ah[4], al[4];
+    uint4 ah[WN], al[WN];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < WN; ++t) {
       const int ra = (n0 + 16 * t + i) * LDB + 4 * g;
       ah[t] = *reinterpret_cast<const uint4*>(&lhi[buf][0][ra]);
       al[t] = *reinterpret_cast<const uint4*>(&llo[buf][0][ra]);
     }
 #pragma unroll
-    for (int tmi = 0; tmi < 4; ++tmi) {
+    for (int tmi = 0; tmi < WM; ++tmi) {
       const int rb = (m0 + 16 * tmi + i) * LDB + 4 * g;
       const uint4 bh = *reinterpret_cast<const uint4*>(&lhi[buf][1][rb]);
       const uint4 bl = *reinterpret_cast<const uint4*>(&llo[buf][1][rb]);
       const uint4 b2h = swap_neg(bh), b2l = swap_neg(bl);
 #pragma unroll
-      for (int tn = 0; tn < 4; ++tn) {
+      for (int tn = 0; tn < WN; ++tn) {
         are[tn][tmi] = mfma_bf16(al[tn], bh, are[tn][tmi]);
         are[tn][tmi] = mfma_bf16(ah[tn], bl, are[tn][tmi]);
         are[tn][tmi] = mfma_bf16(ah[tn], bh, are[tn][tmi]);
@@ -546,9 +552,9 @@ __global__ void __launch_bounds__(256)
   }
   float2* out = part + (size_t)z * ldn * ldm;
 #pragma unroll
-  for (int tn = 0; tn < 4; ++tn)
+  for (int tn = 0; tn < WN; ++tn)
 #pragma unroll
-    for (int tmi = 0; tmi < 4; ++tmi)
+    for (int tmi = 0; tmi < WM; ++tmi)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int n = tile_n + n0 + 16 * tn + 4 * g + v;
@@ -628,11 +634,11 @@ static int launch_contract_bf16_t128(kpm_ctx* ctx, const void* Lslots, int nL, c
   const dim3 grid((unsigned)(ntiles * splits));
   const int sp = (int)splits;
   if (prec == KPM_VEC_F64)
-    hipLaunchKernelGGL((k_contract_bf16x3_t128<double>), grid, dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL((k_contract_bf16x3_t128<double>), grid, dim3(kT128Threads), 0, ctx->stream,
                        (const double*)Lslots, nL, (const double*)Rslots, nR, K2, P2, kchunk, tm, ntiles, sp,
                        (float2*)ctx->gemm_part.ptr, ldn, ldm);
   else
-    hipLaunchKernelGGL((k_contract_bf16x3_t128<float>), grid, dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL((k_contract_bf16x3_t128<float>), grid, dim3(kT128Threads), 0, ctx->stream,
                        (const float*)Lslots, nL, (const float*)Rslots, nR, K2, P2, kchunk, tm, ntiles, sp,
                        (float2*)ctx->gemm_part.ptr, ldn, ldm);
   KPM_HIP_TRY(hipGetLastError());
