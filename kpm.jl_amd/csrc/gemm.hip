@@ -434,15 +434,17 @@ __global__ void __launch_bounds__(256)
 }
 
 // Large-table form of the split-BF16 contraction: a 128 x 128 workgroup tile
-// on eight waves of 64 x 32 (4 x 2 MFMA tiles), two waves per SIMD, so one
-// wave's VALU split of the next stage runs beside the other's MFMAs.  Each
-// operand fragment read from LDS feeds 2 or 4 tiles.  The MFMA accumulators
-// hold FP32 sums over one K-chunk of at most kChunkBF16 reals (no FP64
-// registers, which is what frees the room for two waves); the host sizes the
-// split so that chunks stay within that bound, and the FP32 chunk partials
-// are added in FP64 in a fixed order.
+// on sixteen waves of 32 x 32 (2 x 2 MFMA tiles), four waves per SIMD, so
+// the VALU split of the next stage by some waves runs beside the MFMAs of
+// the others (measured on the 512 x 512 table: one wave per SIMD with 64 x 64
+// per wave 18.7 ms, two waves 14.5 ms, four waves 13.5 ms).  Each operand
+// fragment read from LDS feeds 2 tiles.  The MFMA accumulators hold FP32 sums
+// over one K-chunk of at most kChunkBF16 reals (no FP64 registers, which is
+// what leaves room for four waves); the host sizes the split so that chunks
+// stay within that bound, and the FP32 chunk partials are added in FP64 in a
+// fixed order.
 constexpr int64_t kChunkBF16 = 8192;
-constexpr int kT128Threads = 512;
+constexpr int kT128Threads = 1024;
 
 template <typename VT>
 __global__ void __launch_bounds__(kT128Threads)
@@ -451,7 +453,7 @@ __global__ void __launch_bounds__(kT128Threads)
                            float2* __restrict__ part, int ldn, int ldm) {
   constexpr int TT = 128;
   constexpr int NT = kT128Threads;
-  constexpr int WN = 4, WM = 2;  // MFMA tiles per wave along n (R rows) and m (L rows)
+  constexpr int WN = 2, WM = 2;  // MFMA tiles per wave along n (R rows) and m (L rows)
   constexpr int LDB = kStep / 2 + 4;
   constexpr int WPR = kStep * (int)sizeof(VT) / 16;
   constexpr int EPW = 16 / (int)sizeof(VT);
