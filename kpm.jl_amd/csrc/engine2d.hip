@@ -147,8 +147,8 @@ __global__ void __launch_bounds__(kMergeBlock)
 // A and B merged (M: A's values on the union pattern, *vals2: B's), built
 // on the device and kept in the context for the same pair of operators.
 // *ok stays false when the dual pass does not apply: unsorted rows, rows of
-// one lane at this block width (no spread metadata), or the dual pass
-// switched off.
+// one lane at this block width (no spread metadata), no device memory for
+// the merged operator, or the dual pass switched off.
 static int dual_operator(kpm_ctx* ctx, const kpm_csr* A, const kpm_csr* B, int64_t NR, int prec,
                          kpm_csr* M, const void** vals2, bool* ok) {
   *ok = false;
@@ -165,7 +165,8 @@ static int dual_operator(kpm_ctx* ctx, const kpm_csr* A, const kpm_csr* B, int64
   char* base = nullptr;
   if (ctx->merged_key[0] != A->serial || ctx->merged_key[1] != B->serial) {
     ctx->merged_key[0] = ctx->merged_key[1] = 0;
-    KPM_TRY(ensure(ctx, ctx->merged, brp + bci + 2 * bv + bcnt + bblk + 256));
+    // no room for the merged operator: the two passes run separately
+    if (ensure(ctx, ctx->merged, brp + bci + 2 * bv + bcnt + bblk + 256) != KPM_OK) return KPM_OK;
     base = (char*)ctx->merged.ptr;
     int* rpu = (int*)base;
     int* cnt = (int*)(base + brp + bci + 2 * bv);
