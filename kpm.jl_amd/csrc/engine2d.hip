@@ -207,14 +207,18 @@ static int dual_operator(kpm_ctx* ctx, const kpm_csr* A, const kpm_csr* B, int64
   return KPM_OK;
 }
 
-// out = alpha*H*x + beta*z and out2 = Jb*x in one pass over the merged operator
+// out = alpha*H*x + beta*z and out2 = Jb*x in one pass over the merged
+// operator; with out2_cols > 0, out2 is split into blocks of that many
+// complex columns, out2_stride complex values apart
 static int spmm_dual(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* M, const void* vals2,
                      const kpm_block* x, const kpm_block* z, kpm_block* out, kpm_block* out2,
-                     double alpha, double beta) {
+                     double alpha, double beta, int64_t out2_cols = 0, int64_t out2_stride = 0) {
   StepLaunch s{M, x, z, out, nullptr, alpha, beta, MODE_DUAL};
   s.gamma = 1.0;
   s.vals2 = vals2;
   s.out2 = out2;
+  s.out2_cols = out2_cols;
+  s.out2_stride = out2_stride;
   return launch_step(ctx, s, tuned_config(ctx, H, x->NR, x->prec, MODE_NONE), nullptr);
 }
 
@@ -470,22 +474,22 @@ static int run_3d_batched(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, con
       KPM_TRY(to_wide(ctx, &tmp, W[0].d, (int)NRw, (int)q));
     }
     // the right chain over all outer orders of the batch (:1039-1051); with
-    // the dual pass, the step of order n emits Jb W of order n - 1
+    // the dual pass, the step of order n writes Jb W of order n - 1 straight
+    // into that order's P slots (column block q -> slot q * NC + n - 2)
     for (int64_t n = 1; n <= NC; ++n) {
       const int ri = (int)((n - 1) % 3), rip = (int)((n + 1) % 3), ripp = (int)(n % 3);
+      kpm_block slot;
+      if (dual && n >= 2) slot_view(ctx->slots_r.ptr, stride, n - 2, NH, NRw, prec, ctx, &slot);
       if (n == 2) {
-        if (dual) KPM_TRY(spmm_dual(ctx, H, &M, v2, &W[0], nullptr, &W[1], &Rw, 1.0, 0.0));
+        if (dual) KPM_TRY(spmm_dual(ctx, H, &M, v2, &W[0], nullptr, &W[1], &slot, 1.0, 0.0, NRp, qstride * NC));
         else KPM_TRY(spmm(ctx, H, &W[0], nullptr, &W[1], 1.0, 0.0));
       } else if (n >= 3) {
-        if (dual) KPM_TRY(spmm_dual(ctx, H, &M, v2, &W[rip], &W[ripp], &W[ri], &Rw, 2.0, -1.0));
+        if (dual) KPM_TRY(spmm_dual(ctx, H, &M, v2, &W[rip], &W[ripp], &W[ri], &slot, 2.0, -1.0, NRp, qstride * NC));
         else KPM_TRY(spmm(ctx, H, &W[rip], &W[ripp], &W[ri], 2.0, -1.0));
       }
       if (!dual) {
         KPM_TRY(spmm(ctx, Jb, &W[ri], nullptr, &Rw, 1.0, 0.0));
         KPM_TRY(from_wide(ctx, &Rw, NRp, (int)P, (char*)ctx->slots_r.ptr + stride * (n - 1),
-                          qstride * NC));
-      } else if (n >= 2) {
-        KPM_TRY(from_wide(ctx, &Rw, NRp, (int)P, (char*)ctx->slots_r.ptr + stride * (n - 2),
                           qstride * NC));
       }
     }

@@ -155,6 +155,15 @@ struct StepLaunch {
   const PendingCombine* pend = nullptr;   // previous step's combine, run in this launch
   const void* vals2 = nullptr;            // MODE_DUAL: second operator's values (complex)
   kpm_block* out2 = nullptr;              // MODE_DUAL: second output
+  // MODE_DUAL: out2 split into blocks of out2_cols complex columns, each an
+  // NH x out2_cols array, out2_stride complex values apart (0: out's layout)
+  int64_t out2_cols = 0, out2_stride = 0;
+};
+// Word placement of the dual pass's second output (words of the block type):
+// word w of a row goes to block w / cw, qs words apart, at row * rw + w % cw.
+struct SplitOut {
+  int rw = 0, cw = 1;
+  int64_t qs = 0;
 };
 // Launch geometry picked by the tuner for (LPR, rows per lane, grid size).
 struct StepConfig {

@@ -186,13 +186,21 @@ int launch_step(kpm_ctx* ctx, const StepLaunch& s, const StepConfig& cfg, int* g
   if ((s.mode == MODE_LR || s.mode == MODE_SRC) && !s.l) return fail(KPM_EINVAL, "missing operand block");
   if (s.mode == MODE_DUAL && (!s.vals2 || !s.out2 || s.A->is_real))
     return fail(KPM_EINVAL, "dual pass needs complex merged values and a second output");
+  SplitOut o2{g.RW, g.RW, 0};
+  if (s.mode == MODE_DUAL && s.out2_cols > 0) {
+    const int64_t cpw = cplx_per_word(prec);
+    if (s.out2_cols % cpw != 0 || s.out2_stride % cpw != 0 || s.out2_cols > g.RW * cpw ||
+        (g.RW * cpw) % s.out2_cols != 0)
+      return fail(KPM_EINVAL, "dual pass: second output blocks do not tile the row");
+    o2 = SplitOut{(int)(s.out2_cols / cpw), (int)(s.out2_cols / cpw), s.out2_stride / cpw};
+  }
   const PendingCombine pc = s.pend ? *s.pend : PendingCombine{};
   StepLauncher fn = pick(prec, g, s.A->is_real != 0, s.mode);
   if (!fn) return fail(KPM_EINVAL, "no dual pass for rows of one lane");
   fn(dim3(G, g.chunks), ctx->stream, s.A->rowptr, s.A->colind, s.A->vals, x->d, s.z ? s.z->d : x->d,
      s.out->d, s.l ? s.l->d : nullptr, part, (int)x->NH, g.RW, (int)x->NRp,
      (int)ntiles, s.alpha, s.beta, g.has_tail, nullptr, nullptr, nullptr, s.gamma, pc, s.vals2,
-     s.out2 ? s.out2->d : nullptr);
+     s.out2 ? s.out2->d : nullptr, o2);
   KPM_HIP_TRY(hipGetLastError());
   if (grid_used) *grid_used = G;
   return KPM_OK;
@@ -204,7 +212,7 @@ int launch_step_seg(kpm_ctx* ctx, const StepLaunch& s, const StepGeom& g, int G,
   StepLauncher fn = pick(x->prec, g, s.A->is_real != 0, MODE_DBL_SEG);
   fn(dim3(G, g.chunks), ctx->stream, s.A->rowptr, s.A->colind, s.A->vals, x->d, s.z ? s.z->d : x->d,
      s.out->d, nullptr, partials, (int)x->NH, g.RW, (int)x->NRp, (int)ntiles, s.alpha, s.beta,
-     g.has_tail, tile_lo, treal, seg0, 0.0, PendingCombine{}, nullptr, nullptr);
+     g.has_tail, tile_lo, treal, seg0, 0.0, PendingCombine{}, nullptr, nullptr, SplitOut{});
   KPM_HIP_TRY(hipGetLastError());
   return KPM_OK;
 }

@@ -230,7 +230,10 @@ struct LaneMeta {
 // from the gathers of x it already holds.  With A and B merged onto the union
 // of their patterns, each output sums its own nonzeros in its own row order
 // (the other operator's entries weigh exactly zero), so out and out2 are those
-// of two separate passes.
+// of two separate passes.  o2 places out2's words: word w of a row goes to
+// block w / o2.cw, o2.qs words apart, at row * o2.rw + w % o2.cw (one block
+// of RW words per row is the layout of out; narrower blocks split a wide row
+// into separate arrays).
 template <int PREC, int LPR, int UNR, int KB, bool REALH, int MODE, bool SEG>
 __global__ void __launch_bounds__(kBlock)
     k_step(const int* __restrict__ rowptr, const int* __restrict__ colind,
@@ -240,7 +243,8 @@ __global__ void __launch_bounds__(kBlock)
            int RW, int NRp, int ntiles, double alpha, double beta, int has_tail,
            const int* __restrict__ tile_lo, const int* __restrict__ treal,
            const int* __restrict__ seg0, double gamma, PendingCombine pc,
-           const void* __restrict__ vals2_, typename Word<PREC>::T* __restrict__ out2) {
+           const void* __restrict__ vals2_, typename Word<PREC>::T* __restrict__ out2,
+           SplitOut o2) {
   using WT = Word<PREC>;
   using T = typename WT::T;
   constexpr int CPL = WT::CPL;
@@ -264,6 +268,11 @@ __global__ void __launch_bounds__(kBlock)
   const int wcol = blockIdx.y * LPR + sub;
   const int rbase = wave * (RPW * UNR) + rslot;
   const bool has_z = beta != 0.0;  // uniform
+  size_t o2col = 0;
+  if constexpr (MODE == MODE_DUAL) {
+    const int qb = wcol / o2.cw;
+    o2col = (size_t)qb * o2.qs + (size_t)(wcol - qb * o2.cw);
+  }
 
   const double2* cvals = reinterpret_cast<const double2*>(vals_);
   const double* rvals = reinterpret_cast<const double*>(vals_);
@@ -553,7 +562,7 @@ __global__ void __launch_bounds__(kBlock)
           pr[c] = gamma * y2r[u][c];
           pi[c] = gamma * y2i[u][c];
         }
-        if (live) st_stream(out2 + (size_t)R0.row[u] * RW + wcol, WT::pack(pr, pi));
+        if (live) st_stream(out2 + (size_t)R0.row[u] * o2.rw + o2col, WT::pack(pr, pi));
       }
       if constexpr (MODE == MODE_DBL || MODE == MODE_LR) {
         double sr[CPL], si[CPL], ar[CPL], ai[CPL];
@@ -606,18 +615,19 @@ __global__ void __launch_bounds__(kBlock)
 using StepLauncher = void (*)(dim3, hipStream_t, const int*, const int*, const void*, const void*,
                               const void*, void*, const void*, double2*, int, int, int, int,
                               double, double, int, const int*, const int*, const int*, double,
-                              const PendingCombine&, const void*, void*);
+                              const PendingCombine&, const void*, void*, const SplitOut&);
 
 template <int PREC, int LPR, int UNR, int KB, bool REALH, int MODE, bool SEG>
 static void launch_step_t(dim3 grid, hipStream_t st, const int* rp, const int* ci, const void* v,
                           const void* x, const void* z, void* out, const void* l, double2* part,
                           int NH, int RW, int NRp, int ntiles, double alpha, double beta,
                           int has_tail, const int* tile_lo, const int* treal, const int* seg0,
-                          double gamma, const PendingCombine& pc, const void* v2, void* out2) {
+                          double gamma, const PendingCombine& pc, const void* v2, void* out2,
+                          const SplitOut& o2) {
   using T = typename Word<PREC>::T;
   hipLaunchKernelGGL((k_step<PREC, LPR, UNR, KB, REALH, MODE, SEG>), grid, dim3(kBlock), 0, st, rp,
                      ci, v, (const T*)x, (const T*)z, (T*)out, (const T*)l, part, NH, RW, NRp,
-                     ntiles, alpha, beta, has_tail, tile_lo, treal, seg0, gamma, pc, v2, (T*)out2);
+                     ntiles, alpha, beta, has_tail, tile_lo, treal, seg0, gamma, pc, v2, (T*)out2, o2);
 }
 
 template <int PREC, int LPR, int UNR, int KB, bool REALH>
