@@ -455,7 +455,9 @@ static int run_3d_batched(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, con
 
   const double scale = 1.0 / (double)NR;
   const int64_t qstride = stride / eb;  // one right slot, in elements
-  std::vector<double> host;
+  // each batch table comes back through page-locked staging
+  KPM_TRY(ensure_pinned(ctx, slab * (size_t)P * sizeof(double2)));
+  const double* host = (const double*)ctx->pinned.ptr;
   for (int64_t b = 1; b <= NC; b += P) {
     const int64_t Pb = std::min<int64_t>(P, NC - b + 1);
     // wide starter: column block q = Jg T_{b+q-1} Ja psi_r (unused blocks stay zero)
@@ -505,15 +507,13 @@ static int run_3d_batched(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, con
     KPM_HIP_TRY(hipMemsetAsync(ctx->mu.ptr, 0, slab * (size_t)Pb * sizeof(double2), ctx->stream));
     KPM_TRY(launch_contract(ctx, ctx->slots_l.ptr, (int)NC, ctx->slots_r.ptr, (int)ld, Kp, pitch,
                             prec, p.gemm_mode, scale, ctx->mu.ptr, ld, 0, 0, KPM_PARITY_NONE));
-    host.resize(2 * slab * (size_t)Pb);
-    KPM_HIP_TRY(hipMemcpyAsync(host.data(), ctx->mu.ptr, slab * (size_t)Pb * sizeof(double2),
+    KPM_HIP_TRY(hipMemcpyAsync(ctx->pinned.ptr, ctx->mu.ptr, slab * (size_t)Pb * sizeof(double2),
                                hipMemcpyDeviceToHost, ctx->stream));
     KPM_HIP_TRY(hipStreamSynchronize(ctx->stream));
     for (int64_t q = 0; q < Pb; ++q) {
       double* dst = mu + 2 * slab * (size_t)(b - 1 + q);
       for (int64_t m = 0; m < NC; ++m)
-        std::copy(host.begin() + 2 * (q * NC + ld * m), host.begin() + 2 * (q * NC + ld * m + NC),
-                  dst + 2 * NC * m);
+        std::copy(host + 2 * (q * NC + ld * m), host + 2 * (q * NC + ld * m + NC), dst + 2 * NC * m);
     }
   }
   return KPM_OK;

@@ -116,6 +116,9 @@ struct kpm_block {
 namespace kpm {
 
 int ensure(kpm_ctx* ctx, Buffer& b, size_t bytes);
+// page-locked host staging (ctx->pinned, grow-only; synchronises the stream
+// before it replaces a smaller buffer)
+int ensure_pinned(kpm_ctx* ctx, size_t bytes);
 // host CSR (0-based int64 rowptr/colind, interleaved complex values) -> device CSR
 int upload_csr(kpm_ctx* ctx, int64_t n, const std::vector<int64_t>& rp, const std::vector<int64_t>& ci,
                const std::vector<double>& v_cplx, kpm_csr** out);

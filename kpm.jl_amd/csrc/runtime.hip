@@ -138,7 +138,7 @@ int upload_csr(kpm_ctx* ctx, int64_t n, const std::vector<int64_t>& rp,
   return KPM_OK;
 }
 
-static int ensure_pinned(kpm_ctx* ctx, size_t bytes) {
+int ensure_pinned(kpm_ctx* ctx, size_t bytes) {
   Buffer& b = ctx->pinned;
   if (b.bytes >= bytes && b.ptr) return KPM_OK;
   if (b.ptr) {

@@ -420,13 +420,18 @@ def kpm_3d_(H, Ja, Jb, Jg, NC: int, NR: int, NH: int, mu: np.ndarray, psi_in_l, 
         raise ArgumentError("ψall_l must have at least two slots")
     ctx = get_context()
     Hd, Jad, Jbd, Jgd = (to_device(X, ctx) for X in (H, Ja, Jb, Jg))
-    out = np.zeros((NC, NC, NC), dtype=np.complex128, order="F")
+    # the engine writes every element: fill the caller's array in place when
+    # it is a column-major ComplexF64 NC^3 array, else go through a temporary
+    direct = (isinstance(mu, np.ndarray) and mu.dtype == np.complex128
+              and mu.shape == (NC, NC, NC) and mu.flags.f_contiguous)
+    out = mu if direct else np.empty((NC, NC, NC), dtype=np.complex128, order="F")
     gm = _GEMM_MODES[str(gemm).lower()]
     check(ctx.lib.kpm_3d(ctx.handle, Hd.handle, Jad.handle, Jbd.handle, Jgd.handle, NC, NR, NH,
                          dptr(psl), dptr(psr), int(arr_size),
                          right_block, prec_code(precision), gm, dptr(out)),
           "kpm_3d!")
-    mu[...] = out
+    if not direct:
+        mu[...] = out
 
 
 def kpm_3d(H, Ja, Jb, Jg, NC: int, NR: int, NH: int | None = None, *, arr_size="auto",
