@@ -31,20 +31,27 @@ namespace kpm {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-constexpr int kTQ = 32;  // forms per workgroup: two 16-row MFMA blocks
+constexpr int kTQ = 64;  // forms per workgroup: four 16-row MFMA blocks
 constexpr int kTJ = 64;  // table columns per workgroup: 16 per wave
 constexpr int kStage = 32;  // reals of a column per stage (8 per lane group)
+constexpr int kRB = kTQ / 16;
+constexpr int kLdaA = kStage + 2;  // LDS row stride of the form rows (16-byte pad)
 
 // Each lane (g = lane / 16, i = lane % 16) owns column j0 + i of its wave's
 // 16 columns and, in each 32-real stage, reals k0 + 8g .. k0 + 8g + 7 of that
-// column and of its two form rows q0 + i, q0 + 16 + i; MFMA t of the stage
+// column and of its four form rows q0 + 16 rb + i; MFMA t of the stage
 // contracts element t of every lane group (a fixed permutation of k', the
-// same for both operands).
+// same for both operands).  The form rows are common to the four waves: each
+// stage of them is loaded once per workgroup (coalesced, double-buffered in
+// LDS, the 16-byte row pad puts the 16 rows a lane group reads at one offset
+// in 16 different bank quads); the table columns stream from global memory
+// one stage ahead.
 __global__ void __launch_bounds__(256)
     k_trilinear(const double* __restrict__ mu, int NC, const double* __restrict__ A,
                 const double2* __restrict__ B, const double2* __restrict__ C, int Q, int ntx,
                 int nqy, double2* __restrict__ part) {
   __shared__ double2 red[4][kTQ];
+  __shared__ double As[2][kTQ * kLdaA];
   // Workgroup b runs on XCD b % 8.  XCD x takes the column tiles x, x + 8, ...
   // and, for each, all form blocks in consecutive dispatch slots, so a tile
   // is fetched into that XCD's L2 once and reused by every form block; the
@@ -53,7 +60,8 @@ __global__ void __launch_bounds__(256)
   const int b = blockIdx.x, local = b >> 3;
   const int tile = (local / nqy) * 8 + (b & 7), qblk = local % nqy;
   if (tile >= ntx) return;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, i = lane & 15;
   const int64_t NC2 = (int64_t)NC * NC;
   const int64_t j = (int64_t)tile * kTJ + wave * 16 + i;
@@ -62,64 +70,100 @@ __global__ void __launch_bounds__(256)
   const int q0 = qblk * kTQ;
   const int K2 = 2 * NC;
   const double* mcol = mu + (int64_t)K2 * jc;
-  const double* arow[2];
+  // this thread's share of a stage of form rows: kTQ rows x 16 double2
+  // words, four words per thread (rows beyond Q repeat row Q - 1)
+  constexpr int kAW = kTQ * (kStage / 2) / 256;
+  const double* asrc[kAW];
+  int adst[kAW], awi[kAW];
 #pragma unroll
-  for (int rb = 0; rb < 2; ++rb) {
-    const int q = q0 + rb * 16 + i;
-    arow[rb] = A + (int64_t)K2 * (q < Q ? q : Q - 1);
+  for (int w = 0; w < kAW; ++w) {
+    const int word = tid + 256 * w;
+    const int row = word / (kStage / 2), wi = word % (kStage / 2);
+    const int q = q0 + row;
+    asrc[w] = A + (int64_t)K2 * (q < Q ? q : Q - 1);
+    adst[w] = row * kLdaA + 2 * wi;
+    awi[w] = 2 * wi;
   }
-  // reals beyond 2NC (the last stage) and rows beyond Q read as zero; the
-  // clamped addresses stay inside the column / row
-  auto load = [&](int st, double* bb, double (*aa)[8]) {
+  // reals beyond 2NC (the last stage) read as zero; the clamped addresses
+  // stay inside the column / row
+  auto load_b = [&](int st, double* bb) {
     const int kb = st * kStage + 8 * g;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int k = kb + 2 * t;
       const bool ok = k < K2;
-      const int kc = ok ? k : 0;
-      const double2 v = *reinterpret_cast<const double2*>(mcol + kc);
+      const double2 v = *reinterpret_cast<const double2*>(mcol + (ok ? k : 0));
       bb[2 * t] = ok ? v.x : 0.0;
       bb[2 * t + 1] = ok ? v.y : 0.0;
-#pragma unroll
-      for (int rb = 0; rb < 2; ++rb) {
-        const double2 w = *reinterpret_cast<const double2*>(arow[rb] + kc);
-        aa[rb][2 * t] = ok ? w.x : 0.0;
-        aa[rb][2 * t + 1] = ok ? w.y : 0.0;
-      }
     }
   };
+  auto load_a = [&](int st, double2* ra) {
+#pragma unroll
+    for (int w = 0; w < kAW; ++w) {
+      const int k = st * kStage + awi[w];
+      const bool ok = k < K2;
+      const double2 v = *reinterpret_cast<const double2*>(asrc[w] + (ok ? k : 0));
+      ra[w] = ok ? v : make_double2(0.0, 0.0);
+    }
+  };
+  auto store_a = [&](int buf, const double2* ra) {
+#pragma unroll
+    for (int w = 0; w < kAW; ++w) *reinterpret_cast<double2*>(&As[buf][adst[w]]) = ra[w];
+  };
 
-  d4 are[2], aim[2];
+  d4 are[kRB], aim[kRB];
 #pragma unroll
-  for (int rb = 0; rb < 2; ++rb) are[rb] = aim[rb] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int rb = 0; rb < kRB; ++rb) are[rb] = aim[rb] = d4{0.0, 0.0, 0.0, 0.0};
   const int nst = (K2 + kStage - 1) / kStage;
-  double bc[8], ac[2][8];
-  load(0, bc, ac);
+  double bc[8];
+  double2 ra[kAW];
+  load_b(0, bc);
+  load_a(0, ra);
+  store_a(0, ra);
+  __syncthreads();
   for (int st = 0; st < nst; ++st) {
-    double bn[8], an[2][8];
-    load(min(st + 1, nst - 1), bn, an);  // next stage in flight during the MFMAs
+    const int buf = st & 1;
+    const bool more = st + 1 < nst;  // uniform
+    double bn[8];
+    if (more) {  // next stage in flight during the MFMAs
+      load_b(st + 1, bn);
+      load_a(st + 1, ra);
+    }
+    // all four row blocks' fragments first, then the MFMAs with the eight
+    // accumulators interleaved (each accumulator still takes t in order), so
+    // a dependent MFMA never waits on the one just issued
+    double ac[kRB][8];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
+    for (int rb = 0; rb < kRB; ++rb) {
+      const double* pa = &As[buf][(rb * 16 + i) * kLdaA + 8 * g];
 #pragma unroll
-      for (int rb = 0; rb < 2; ++rb) {
+      for (int t = 0; t < 8; t += 2) {
+        const double2 v = *reinterpret_cast<const double2*>(pa + t);
+        ac[rb][t] = v.x;
+        ac[rb][t + 1] = v.y;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+      for (int rb = 0; rb < kRB; ++rb) {
         const double ar = (t & 1) ? -ac[rb][t] : ac[rb][t];
         const double ai = (t & 1) ? ac[rb][t - 1] : ac[rb][t + 1];
         are[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bc[t], are[rb], 0, 0, 0);
         aim[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bc[t], aim[rb], 0, 0, 0);
       }
-    }
+    if (more) {
+      store_a(buf ^ 1, ra);
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      bc[t] = bn[t];
-      ac[0][t] = an[0][t];
-      ac[1][t] = an[1][t];
+      for (int t = 0; t < 8; ++t) bc[t] = bn[t];
     }
+    __syncthreads();
   }
 
   // epilogue: accumulator element v of lane (g, i) is X[row g + 4v][column i]
   const int m = (int)(jc % NC), p = (int)(jc / NC);
 #pragma unroll
-  for (int rb = 0; rb < 2; ++rb)
+  for (int rb = 0; rb < kRB; ++rb)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int r = rb * 16 + g + 4 * v;
@@ -160,7 +204,20 @@ __global__ void k_trilinear_reduce(const double2* __restrict__ part, int ntiles,
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= Q) return;
   double sx = 0.0, sy = 0.0;
-  for (int t = 0; t < ntiles; ++t) {
+  // loads in batches of 16 (in flight together), added in tile order
+  constexpr int U = 16;
+  int t = 0;
+  for (; t + U <= ntiles; t += U) {
+    double2 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = part[(int64_t)(t + u) * Q + q];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      sx += v[u].x;
+      sy += v[u].y;
+    }
+  }
+  for (; t < ntiles; ++t) {
     const double2 v = part[(int64_t)t * Q + q];
     sx += v.x;
     sy += v.y;
