@@ -14,7 +14,8 @@ Cases:
 - operators whose patterns differ, with union rows beyond 16 nonzeros (the
   spill-over loop of the pass), and a real H merged with a complex Jb;
 - one-lane rows (NR = 1 at FP64), where the fused pass does not apply;
-- kpm_3d with the outer orders batched as column blocks, and nested.
+- kpm_3d with the outer orders batched as column blocks, and nested;
+- an empty Jb or Ja (no stored entries).
 """
 import numpy as np
 import pytest
@@ -116,3 +117,17 @@ def test_kpm3d_fused_equals_separate(NR):
                                                    arr_size=NC - 1, right_block=4))
     np.testing.assert_array_equal(got, sep)
     np.testing.assert_allclose(got, ref, **TOL)
+
+
+def test_kpm2d_empty_current_operator():
+    """An empty Jb (no stored entries): the merged operator is H's pattern with
+    zero second values, and every table entry is zero; an empty Ja likewise."""
+    n, NR, NC = 50, 4, 10
+    H = ragged_hermitian(n, 3, seed=21)
+    J = ragged_hermitian(n, 2, seed=22)
+    Z = sp.csc_matrix((n, n), dtype=np.complex128)
+    pl, pr = phases(23, n, NR), phases(24, n, NR)
+    for Ja, Jb in ((J, Z), (Z, J)):
+        got, sep = fused_and_separate(lambda: K.kpm_2d(H, Ja, Jb, NC, NR, psi_in_l=pl, psi_in_r=pr))
+        np.testing.assert_array_equal(got, sep)
+        assert np.all(got == 0)
