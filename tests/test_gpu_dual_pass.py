@@ -15,7 +15,9 @@ Cases:
   spill-over loop of the pass), and a real H merged with a complex Jb;
 - one-lane rows (NR = 1 at FP64), where the fused pass does not apply;
 - kpm_3d with the outer orders batched as column blocks, and nested;
-- an empty Jb or Ja (no stored entries).
+- an empty Jb or Ja (no stored entries);
+- a Jb with a duplicated stored entry, whose repeated column switches the
+  fused pass off.
 """
 import numpy as np
 import pytest
@@ -131,3 +133,27 @@ def test_kpm2d_empty_current_operator():
         got, sep = fused_and_separate(lambda: K.kpm_2d(H, Ja, Jb, NC, NR, psi_in_l=pl, psi_in_r=pr))
         np.testing.assert_array_equal(got, sep)
         assert np.all(got == 0)
+
+
+def test_kpm2d_duplicate_entries_take_separate_passes():
+    """A Jb stored with a duplicated entry (its value split in two) has a row
+    whose column indices repeat, so the merge does not apply and the passes
+    run separately; the table still equals the oracle's on the summed
+    matrix."""
+    n, NR, NC = 60, 4, 12
+    H = ragged_hermitian(n, 3, seed=31)
+    Jb = ragged_hermitian(n, 2, seed=32).tocsc()
+    Ja = ragged_hermitian(n, 2, seed=33)
+    k = Jb.indptr[5]  # first stored entry of column 5, stored twice at half value
+    data = np.insert(Jb.data, k, Jb.data[k] / 2)
+    data[k + 1] /= 2
+    indices = np.insert(Jb.indices, k, Jb.indices[k])
+    indptr = Jb.indptr.copy()
+    indptr[6:] += 1
+    Jdup = sp.csc_matrix((data, indices, indptr), shape=(n, n))
+    assert Jdup.nnz == Jb.nnz + 1 and not Jdup.has_canonical_format
+    pl, pr = phases(34, n, NR), phases(35, n, NR)
+    ref = oracle.kpm_2d(H, Ja, Jb, NC, pl, pr, NC, NC)
+    got, sep = fused_and_separate(lambda: K.kpm_2d(H, Ja, Jdup, NC, NR, psi_in_l=pl, psi_in_r=pr))
+    np.testing.assert_array_equal(got, sep)
+    np.testing.assert_allclose(got, ref, **TOL)
