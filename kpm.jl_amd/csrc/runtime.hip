@@ -348,7 +348,13 @@ int kpm_device_name(kpm_ctx* ctx, char* buf, size_t len) {
   if (!ctx || !buf || len == 0) return fail(KPM_EINVAL, "bad arguments");
   hipDeviceProp_t prop;
   KPM_HIP_TRY(hipGetDeviceProperties(&prop, ctx->device));
-  std::snprintf(buf, len, "%s (%s)", prop.name, prop.gcnArchName);
+  // the marketing name can be empty (it is on some driver stacks): then the
+  // device index and its CU count name the part
+  if (prop.name[0])
+    std::snprintf(buf, len, "%s (%s)", prop.name, prop.gcnArchName);
+  else
+    std::snprintf(buf, len, "device %d, %d CUs (%s)", ctx->device, prop.multiProcessorCount,
+                  prop.gcnArchName);
   return KPM_OK;
 }
 
