@@ -57,6 +57,14 @@ extern "C" {
 #define KPM_PARITY_ODD 1
 #define KPM_PARITY_EVEN 2
 
+/* indices of kpm_ctx_counters */
+#define KPM_COUNT_STEP 0    /* fused recurrence passes (every mode) */
+#define KPM_COUNT_DUAL 1    /* of which right-chain + Jb dual passes */
+#define KPM_COUNT_DOT 2     /* dot-only passes (re-reads of two blocks) */
+#define KPM_COUNT_COMBINE 3 /* separate moment-reduction launches */
+#define KPM_COUNT_SYNC 4    /* host synchronisations of the moment paths */
+#define KPM_COUNT_N 8
+
 typedef struct kpm_ctx kpm_ctx;     /* device + stream + cached workspaces */
 typedef struct kpm_csr kpm_csr;     /* device CSR operator (H, J_alpha, ...) */
 typedef struct kpm_block kpm_block; /* device NH x NR recurrence block */
@@ -80,6 +88,15 @@ int kpm_ctx_synchronize(kpm_ctx* ctx);
  * row).  enable = 0: two separate passes.  The tables are the same either
  * way; the switch exists to show that. */
 int kpm_ctx_set_dual_pass(kpm_ctx* ctx, int enable);
+/* No counterpart in the reference (a limit of this engine).  The largest
+ * merged operator, in bytes, the dual pass may allocate; a larger one makes
+ * kpm_2d / kpm_3d run the two passes separately, as when the allocation
+ * fails.  bytes < 0 (default): no limit. */
+int kpm_ctx_set_merged_limit(kpm_ctx* ctx, int64_t bytes);
+/* No counterpart in the reference (instrumentation).  Copies the first
+ * min(n, KPM_COUNT_N) launch counters (KPM_COUNT_* indices) of the context;
+ * reset != 0 then zeroes them. */
+int kpm_ctx_counters(kpm_ctx* ctx, int64_t* counts, int n, int reset);
 /* device_free_memory(::Dev) -- ext/KPMCUDAExt.jl:45, src/device.jl:66-67 */
 int kpm_device_free_memory(kpm_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
 /* name of the device, e.g. for logs */
@@ -121,21 +138,45 @@ int kpm_block_zero(kpm_block* b);
  * from lane i % lanes), not Julia's Xoshiro. */
 int kpm_block_random_phase(kpm_block* b, uint64_t seed, int64_t lanes);
 
-/* ---- recurrence primitives: ext/KPMCUDAExt.jl:84-121 ---------------------
+/* ---- recurrence primitives: ext/KPMCUDAExt.jl:84-150 ---------------------
  * chebyshev_iter_single(H, V_pp, V_p[, V_out]) (chebyshev_iteration.jl:39-91,
- * 135-167): V_out <- 2 H V_p - V_pp.  V_out may be V_pp (in place). */
+ * 135-167): V_out <- 2 H V_p - V_pp.  V_out may be V_pp (in place).
+ * The pass also reduces <V_p|V_p> and <V_p|V_out> (FP64, per probe column),
+ * the two dots kpm_1d! asks for next (src/moment.jl:463-484); they are kept
+ * for the columnwise_dot! calls that follow (kpm_accum_dot,
+ * kpm_colwise_dot) until any call writes a block of this context. */
 int kpm_cheb_step(kpm_ctx* ctx, const kpm_csr* H, const kpm_block* Vpp, const kpm_block* Vp,
                   kpm_block* Vout);
-/* mul!(Y, A, X, alpha, beta) with real alpha, beta: Y <- alpha A X + beta Y */
+/* mul!(Y, A, X, alpha, beta) with real alpha, beta: Y <- alpha A X + beta Y;
+ * like kpm_cheb_step it keeps <X|X> and <X|Y> for the next dots
+ * (src/moment.jl:442-449: mul! then columnwise_dot! of mu0 and mu1). */
 int kpm_spmm(kpm_ctx* ctx, const kpm_csr* A, double alpha, const kpm_block* X, double beta,
              kpm_block* Y);
-/* columnwise_dot!(target, n, A, B; alpha, beta_col, beta_scale)
- * (src/utils/vectors.jl:103-117, ext/KPMCUDAExt.jl:128-145):
- * target[i,n] = alpha <A_i|B_i> + beta_scale target[i,beta_col]; the target
- * is a host NR x NC column-major ComplexF64 matrix, n and beta_col are
- * 1-based (beta_col == 0: no beta term).  FP64 accumulation. */
+/* columnwise_dot!(target, n, A, B; alpha, beta_col, beta_scale) with a HOST
+ * target (src/utils/vectors.jl:103-117): target[i,n] = alpha <A_i|B_i> +
+ * beta_scale target[i,beta_col]; the target is a host NR x NC column-major
+ * ComplexF64 matrix, n and beta_col are 1-based (beta_col == 0: no beta
+ * term).  FP64 accumulation; synchronises with the host. */
 int kpm_colwise_dot(kpm_ctx* ctx, double* target, int64_t NC, int64_t n, const kpm_block* A,
                     const kpm_block* B, double alpha, int64_t beta_col, double beta_scale);
+
+/* ---- device moment accumulator: moment_accumulator / columnwise_dot! /
+ *      copy_moment_accumulator! on device arrays (ext/KPMCUDAExt.jl:125-150,
+ *      src/utils/vectors.jl:98-122) ---------------------------------------
+ * A zeroed NR x NC ComplexF64 matrix in device memory.  kpm_accum_dot is
+ * columnwise_dot! into it: when <A|B> is a dot the last kpm_cheb_step /
+ * kpm_spmm of this context reduced (A = its V_p, B = V_p or V_out) and no
+ * block was written since, the call only records a combine of the cached
+ * partials, which runs inside the next primitive step's launch (or in one
+ * small launch at the latest at kpm_accum_download); otherwise it runs one
+ * dot pass and one combine.  It never synchronises with the host.
+ * kpm_accum_download copies the matrix to the host once. */
+typedef struct kpm_accum kpm_accum;
+int kpm_accum_create(kpm_ctx* ctx, int64_t NR, int64_t NC, kpm_accum** out);
+int kpm_accum_destroy(kpm_accum* acc);
+int kpm_accum_dot(kpm_ctx* ctx, kpm_accum* acc, int64_t n, const kpm_block* A, const kpm_block* B,
+                  double alpha, int64_t beta_col, double beta_scale);
+int kpm_accum_download(kpm_ctx* ctx, const kpm_accum* acc, double* host);
 
 /* ---- moment engine: src/moment.jl ----------------------------------------
  * kpm_1d! equal-vector doubling form (src/moment.jl:420-492): NC even,
@@ -312,6 +353,11 @@ int kpm_batch_run(kpm_ctx* ctx, kpm_batch* B, int64_t NC, double* mu, int use_gr
 int kpm_batch_autotune(kpm_ctx* ctx, kpm_batch* B, int reps, int* best_unroll, int* best_grid,
                        float* best_ms_per_step);
 int kpm_batch_info(const kpm_batch* B, int64_t* padded_rows, int64_t* nnz, int* nseg);
+/* Instrumentation (no reference counterpart): how many times kpm_batch_run
+ * has recorded its step graph.  The graph is recorded again whenever the
+ * shape or a device buffer it uses changes (probe upload at another width,
+ * a new launch schedule). */
+int kpm_batch_graph_builds(const kpm_batch* B, int64_t* builds);
 
 /* ---- measurement ---------------------------------------------------------
  * Launch the fused recurrence+doubling-dot pass `nsteps` times on the

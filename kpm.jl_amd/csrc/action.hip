@@ -303,6 +303,7 @@ int kpm_cheb_action_block(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t K,
     return fail(KPM_EINVAL, "out must be an FP64 block of (NH*K) x NR");
   if (out->d == V->d) return fail(KPM_EINVAL, "out must not alias V");
   KPM_HIP_TRY(hipSetDevice(ctx->device));
+  ++ctx->epoch;
   // out stacks the K results: rows [k*NH, (k+1)*NH) hold out[:, :, k]
   return run_action(ctx, H, NC, K, V, C, (double2*)out->d, V->NH * out->NRp, out->NRp, check_every);
 }

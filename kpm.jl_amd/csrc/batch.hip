@@ -37,11 +37,13 @@ struct kpm_batch {
   kpm::Buffer probes;  // resident padded probe block (kpm_batch_upload)
   int64_t probe_NR = 0;
   int probe_prec = -1;
-  // recorded step sequence
+  // recorded step sequence, keyed on the shape and on every device address
+  // the captured nodes use; dropped whenever one of those buffers may move
   hipGraphExec_t graph = nullptr;
   std::tuple<int64_t, int64_t, int, int64_t, int, const void*, const void*, const void*, const void*,
-             const void*>
+             const void*, const void*, const void*, const void*, const void*, const void*>
       gkey{};
+  int64_t graph_builds = 0;  // instantiations so far (kpm_batch_graph_builds)
 };
 
 namespace kpm {
@@ -77,11 +79,15 @@ static int upload_ints(kpm_ctx* ctx, Buffer& b, const std::vector<int>& v) {
   return KPM_OK;
 }
 
+static void drop_graph(kpm_batch* B);
+
 // tile ranges, tile realisations and segment slots for row tiles of TR rows
-// on G workgroups
+// on G workgroups.  A rebuild may reallocate the schedule buffers that a
+// recorded graph reads, so it drops the graph first.
 static int build_sched(kpm_batch* B, int64_t TR, int G) {
   BatchSched& S = B->sched;
   if (S.TR == TR && S.G == G) return KPM_OK;
+  drop_graph(B);
   kpm_ctx* ctx = B->ctx;
   const int64_t ntiles = B->poff[B->nreal] / TR;
   std::vector<int> treal(ntiles), tile_lo(G + 1), seg0(G), sfirst(B->nreal, 0), scount(B->nreal, 0);
@@ -111,6 +117,14 @@ static int build_sched(kpm_batch* B, int64_t TR, int G) {
   S.ntiles = ntiles;
   S.nseg = nseg;
   return KPM_OK;
+}
+
+static void drop_graph(kpm_batch* B) {
+  if (B->graph) {
+    (void)hipStreamSynchronize(B->ctx->stream);
+    (void)hipGraphExecDestroy(B->graph);
+    B->graph = nullptr;
+  }
 }
 
 static void free_buf(Buffer& b) {
@@ -216,8 +230,8 @@ int kpm_batch_create(kpm_ctx* ctx, int nreal, const int64_t* row_offsets, const 
 int kpm_batch_destroy(kpm_batch* B) {
   if (!B) return KPM_OK;
   (void)hipSetDevice(B->ctx->device);
+  drop_graph(B);
   (void)hipStreamSynchronize(B->ctx->stream);
-  if (B->graph) (void)hipGraphExecDestroy(B->graph);
   free_buf(B->sched.tile_lo);
   free_buf(B->sched.treal);
   free_buf(B->sched.seg0);
@@ -235,8 +249,11 @@ int kpm_batch_upload(kpm_ctx* ctx, kpm_batch* B, int64_t NR, const double* psi, 
   if (vec_prec != KPM_VEC_F64 && vec_prec != KPM_VEC_F32) return fail(KPM_EINVAL, "bad vec_prec");
   KPM_HIP_TRY(hipSetDevice(ctx->device));
   const int64_t P = B->poff[B->nreal], n = B->off[B->nreal];
+  // the recorded graph copies from the probe block; it may move here
+  const void* old_probes = B->probes.ptr;
   kpm_block pb;
   KPM_TRY(work_block(ctx, B->probes, P, NR, vec_prec, &pb));
+  if (B->probes.ptr != old_probes) drop_graph(B);
   // stacked (n x NR, column-major) -> padded rows, then the device layout
   std::vector<double> padded((size_t)P * NR * 2, 0.0);
   for (int64_t c = 0; c < NR; ++c)
@@ -281,14 +298,14 @@ int kpm_batch_run(kpm_ctx* ctx, kpm_batch* B, int64_t NC, double* mu, int use_gr
     return batch_steps(ctx, B, NC, g, &w0, &w1, ctx->mu.ptr);
   };
   if (use_graph) {
+    const BatchSched& S = B->sched;
     auto key = std::make_tuple(NC, NR, vec_prec, g.TR, G, (const void*)w0.d, (const void*)w1.d,
                                (const void*)ctx->mu.ptr, (const void*)B->partials.ptr,
-                               (const void*)B->sched.treal.ptr);
+                               (const void*)S.treal.ptr, (const void*)pb.d, (const void*)S.tile_lo.ptr,
+                               (const void*)S.seg0.ptr, (const void*)S.sfirst.ptr,
+                               (const void*)S.scount.ptr);
     if (!B->graph || B->gkey != key) {
-      if (B->graph) {
-        KPM_HIP_TRY(hipGraphExecDestroy(B->graph));
-        B->graph = nullptr;
-      }
+      drop_graph(B);
       hipGraph_t graph;
       KPM_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
       int rc = run();
@@ -299,6 +316,7 @@ int kpm_batch_run(kpm_ctx* ctx, kpm_batch* B, int64_t NC, double* mu, int use_gr
       (void)hipGraphDestroy(graph);
       if (e != hipSuccess) return fail(KPM_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
       B->gkey = key;
+      ++B->graph_builds;
     }
     KPM_HIP_TRY(hipGraphLaunch(B->graph, ctx->stream));
   } else {
@@ -370,6 +388,12 @@ int kpm_batch_autotune(kpm_ctx* ctx, kpm_batch* B, int reps, int* best_unroll, i
   if (best_unroll) *best_unroll = bestc.unroll;
   if (best_grid) *best_grid = bestc.grid;
   if (best_ms_per_step) *best_ms_per_step = best;
+  return KPM_OK;
+}
+
+int kpm_batch_graph_builds(const kpm_batch* B, int64_t* builds) {
+  if (!B || !builds) return fail(KPM_EINVAL, "null argument");
+  *builds = B->graph_builds;
   return KPM_OK;
 }
 

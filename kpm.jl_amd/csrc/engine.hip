@@ -46,6 +46,7 @@ static int mu_buffer(kpm_ctx* ctx, int64_t NR, int64_t NC) {
 static int mu_to_host(kpm_ctx* ctx, double* host, int64_t count) {
   KPM_HIP_TRY(hipMemcpyAsync(host, ctx->mu.ptr, (size_t)count * sizeof(double2),
                              hipMemcpyDeviceToHost, ctx->stream));
+  ++ctx->counters[KPM_COUNT_SYNC];
   KPM_HIP_TRY(hipStreamSynchronize(ctx->stream));
   return KPM_OK;
 }
@@ -232,6 +233,96 @@ int kpm_1d_current(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Jadj, int64_t 
 }
 
 // ---- primitives -----------------------------------------------------------------
+// The reference's device loop (src/moment.jl:440-489 on ext/KPMCUDAExt.jl:84-150)
+// calls chebyshev_iter_single and then columnwise_dot! twice per step.  Here
+// every primitive step is the fused pass that also reduces <Vp|Vp> and
+// <Vp|Vout>; the following columnwise_dot! calls on those blocks combine the
+// cached partials (O(grid * NR)) instead of re-reading the blocks, and the
+// combine runs inside the next primitive step's launch.
+
+}  // extern "C"
+
+namespace kpm {
+
+// run the combine waiting for the next primitive step as a launch of its own
+static int prim_flush(kpm_ctx* ctx) {
+  kpm_prim_state& P = ctx->prim;
+  if (!P.pending) return KPM_OK;
+  P.pending = false;
+  return launch_combine_from(ctx, (const double2*)P.pend_part, P.pend_G, P.pend_NR, P.pend_NRp,
+                             P.pend_cb, P.pend_mu);
+}
+
+// out = alpha*A*x + beta*z (z may be null or alias out) with d0 = <x|x>,
+// d1 = <x|out> into the next half of the primitive partials, running the
+// pending combine in the same launch
+static int prim_step(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* x, const kpm_block* z,
+                     kpm_block* out, double alpha, double beta) {
+  kpm_prim_state& P = ctx->prim;
+  const size_t half_bytes =
+      (size_t)kMaxGridPerCU * std::max(ctx->num_cus, 1) * 2 * x->NRp * sizeof(double2);
+  if (P.part.bytes < 2 * half_bytes) {  // growing frees the halves the pending combine reads
+    KPM_TRY(prim_flush(ctx));
+    P.cached = false;
+    KPM_TRY(ensure(ctx, P.part, 2 * half_bytes));
+  }
+  const int h = P.k & 1;
+  double2* part = (double2*)((char*)P.part.ptr + h * (P.part.bytes / 2));
+  StepLaunch s{A, x, z, out, nullptr, alpha, beta, MODE_DBL};
+  s.part_out = part;
+  PendingCombine pc;
+  if (P.pending) {
+    pc.part = (const double2*)P.pend_part;
+    pc.mu = (double2*)P.pend_mu;
+    pc.cb = P.pend_cb;
+    pc.G = P.pend_G;
+    pc.NRp = P.pend_NRp;
+    pc.NR = P.pend_NR;
+    s.pend = &pc;
+  }
+  int G = 0;
+  KPM_TRY(launch_step(ctx, s, tuned_config(ctx, A, x->NR, x->prec, MODE_DBL), &G));
+  P.pending = false;
+  ++ctx->epoch;  // out changed: dots cached before this step are stale
+  P.cached = true;
+  P.vp = x->d;
+  P.vout = out->d;
+  P.epoch = ctx->epoch;
+  P.part_ptr = part;
+  P.G = G;
+  P.NRp = (int)x->NRp;
+  P.NR = (int)x->NR;
+  ++P.k;
+  return KPM_OK;
+}
+
+// which cached dot of the last primitive step <A|B> is (-1: none)
+static int prim_lookup(kpm_ctx* ctx, const kpm_block* A, const kpm_block* B) {
+  const kpm_prim_state& P = ctx->prim;
+  if (!P.cached || P.epoch != ctx->epoch || A->d != P.vp || A->NR != P.NR || A->NRp != P.NRp)
+    return -1;
+  if (B->d == P.vp) return 0;
+  if (B->d == P.vout) return 1;
+  return -1;
+}
+
+static int check_dot_args(const kpm_block* A, const kpm_block* B, int64_t NC, int64_t n,
+                          int64_t beta_col) {
+  if (A->NH != B->NH || A->NR != B->NR || A->prec != B->prec)
+    return fail(KPM_EINVAL, "block shapes or precisions differ");
+  if (n < 1 || n > NC || beta_col < 0 || beta_col > NC) return fail(KPM_EINVAL, "column out of range");
+  return KPM_OK;
+}
+
+}  // namespace kpm
+
+struct kpm_accum {
+  kpm_ctx* ctx = nullptr;
+  int64_t NR = 0, NC = 0;
+  void* d = nullptr;  // NR x (NC + 1) double2, column-major; column NC is scratch
+};
+
+extern "C" {
 
 int kpm_cheb_step(kpm_ctx* ctx, const kpm_csr* H, const kpm_block* Vpp, const kpm_block* Vp,
                   kpm_block* Vout) {
@@ -240,7 +331,7 @@ int kpm_cheb_step(kpm_ctx* ctx, const kpm_csr* H, const kpm_block* Vpp, const kp
   KPM_TRY(check_block_pair(H, Vp, Vout));
   if (Vp->d == Vout->d) return fail(KPM_EINVAL, "V_p must not alias V_out");
   KPM_HIP_TRY(hipSetDevice(ctx->device));
-  return step(ctx, H, Vp, Vpp, Vout, nullptr, 2.0, -1.0, MODE_NONE, nullptr);
+  return prim_step(ctx, H, Vp, Vpp, Vout, 2.0, -1.0);
 }
 
 int kpm_spmm(kpm_ctx* ctx, const kpm_csr* A, double alpha, const kpm_block* X, double beta,
@@ -249,33 +340,135 @@ int kpm_spmm(kpm_ctx* ctx, const kpm_csr* A, double alpha, const kpm_block* X, d
   KPM_TRY(check_block_pair(A, X, Y));
   if (X->d == Y->d) return fail(KPM_EINVAL, "X must not alias Y");
   KPM_HIP_TRY(hipSetDevice(ctx->device));
-  return step(ctx, A, X, beta != 0.0 ? Y : nullptr, Y, nullptr, alpha, beta, MODE_NONE, nullptr);
+  return prim_step(ctx, A, X, beta != 0.0 ? Y : nullptr, Y, alpha, beta);
 }
 
 int kpm_colwise_dot(kpm_ctx* ctx, double* target, int64_t NC, int64_t n, const kpm_block* A,
                     const kpm_block* B, double alpha, int64_t beta_col, double beta_scale) {
   if (!ctx || !target || !A || !B) return fail(KPM_EINVAL, "null argument");
-  if (A->NH != B->NH || A->NR != B->NR || A->prec != B->prec)
-    return fail(KPM_EINVAL, "block shapes or precisions differ");
-  if (n < 1 || n > NC || beta_col < 0 || beta_col > NC) return fail(KPM_EINVAL, "column out of range");
+  KPM_TRY(check_dot_args(A, B, NC, n, beta_col));
   KPM_HIP_TRY(hipSetDevice(ctx->device));
   const int64_t NR = A->NR;
   KPM_TRY(ensure(ctx, ctx->mu, (size_t)NR * sizeof(double2)));
-  int G = 0;
-  KPM_TRY(launch_dot(ctx, A, B, &G));
-  Combine c{1, {0, 0}, {1.0, 0.0}, {-1, -1}, {0.0, 0.0}};
-  KPM_TRY(launch_combine(ctx, G, NR, A->NRp, c, ctx->mu.ptr, 1));
-  std::vector<double> d(2 * NR);
-  KPM_TRY(mu_to_host(ctx, d.data(), NR));
+  const Combine c{1, {0, 0}, {1.0, 0.0}, {-1, -1}, {0.0, 0.0}};
+  const int d = prim_lookup(ctx, A, B);
+  if (d >= 0) {  // the last primitive step reduced this dot already
+    const kpm_prim_state& P = ctx->prim;
+    // one dot out of the step's pair: entry 0 reads dot d of stride 2
+    Combine cd{2, {0, 0}, {0.0, 0.0}, {-1, -1}, {0.0, 0.0}};
+    cd.tcol[d] = 0;
+    cd.alpha[d] = 1.0;
+    cd.tcol[1 - d] = 1;  // scratch column
+    KPM_TRY(ensure(ctx, ctx->mu, (size_t)2 * NR * sizeof(double2)));
+    KPM_TRY(launch_combine_from(ctx, (const double2*)P.part_ptr, P.G, NR, P.NRp, cd, ctx->mu.ptr));
+  } else {
+    int G = 0;
+    KPM_TRY(launch_dot(ctx, A, B, &G));
+    KPM_TRY(launch_combine(ctx, G, NR, A->NRp, c, ctx->mu.ptr, 1));
+  }
+  std::vector<double> dv(2 * NR);
+  KPM_TRY(mu_to_host(ctx, dv.data(), NR));
   for (int64_t i = 0; i < NR; ++i) {
     double br = 0.0, bi = 0.0;
     if (beta_col != 0) {
       br = beta_scale * target[2 * (i + NR * (beta_col - 1))];
       bi = beta_scale * target[2 * (i + NR * (beta_col - 1)) + 1];
     }
-    target[2 * (i + NR * (n - 1))] = alpha * d[2 * i] + br;
-    target[2 * (i + NR * (n - 1)) + 1] = alpha * d[2 * i + 1] + bi;
+    target[2 * (i + NR * (n - 1))] = alpha * dv[2 * i] + br;
+    target[2 * (i + NR * (n - 1)) + 1] = alpha * dv[2 * i + 1] + bi;
   }
+  return KPM_OK;
+}
+
+int kpm_accum_create(kpm_ctx* ctx, int64_t NR, int64_t NC, kpm_accum** out) {
+  if (!ctx || !out) return fail(KPM_EINVAL, "null argument");
+  if (NR <= 0 || NC <= 0) return fail(KPM_EINVAL, "accumulator dimensions must be positive");
+  KPM_HIP_TRY(hipSetDevice(ctx->device));
+  const size_t bytes = (size_t)(NR * (NC + 1)) * sizeof(double2);
+  kpm_accum* a = new kpm_accum();
+  a->ctx = ctx;
+  a->NR = NR;
+  a->NC = NC;
+  if (hipMalloc(&a->d, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    delete a;
+    return fail(KPM_ENOMEM, "hipMalloc of the moment accumulator failed");
+  }
+  hipError_t e = hipMemsetAsync(a->d, 0, bytes, ctx->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(a->d);
+    delete a;
+    return fail(KPM_EHIP, std::string("accumulator clear failed: ") + hipGetErrorString(e));
+  }
+  *out = a;
+  return KPM_OK;
+}
+
+int kpm_accum_destroy(kpm_accum* a) {
+  if (!a) return KPM_OK;
+  kpm_ctx* ctx = a->ctx;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->prim.pending && ctx->prim.pend_mu == a->d) ctx->prim.pending = false;  // nobody reads it
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(a->d);
+  delete a;
+  return KPM_OK;
+}
+
+int kpm_accum_dot(kpm_ctx* ctx, kpm_accum* acc, int64_t n, const kpm_block* A, const kpm_block* B,
+                  double alpha, int64_t beta_col, double beta_scale) {
+  if (!ctx || !acc || !A || !B) return fail(KPM_EINVAL, "null argument");
+  if (acc->ctx != ctx) return fail(KPM_EINVAL, "accumulator belongs to another context");
+  KPM_TRY(check_dot_args(A, B, acc->NC, n, beta_col));
+  if (A->NR != acc->NR) return fail(KPM_EINVAL, "accumulator rows must equal the probe columns");
+  KPM_HIP_TRY(hipSetDevice(ctx->device));
+  kpm_prim_state& P = ctx->prim;
+  const int tcol = (int)(n - 1), bcol = (int)(beta_col - 1);
+  const int d = prim_lookup(ctx, A, B);
+  if (d >= 0) {
+    // join the waiting combine when it reduces the same partials into the
+    // same accumulator and no entry reads a column another entry writes
+    if (P.pending) {
+      const int o = 1 - d;
+      const bool join = P.pend_mu == acc->d && P.pend_part == P.part_ptr && !P.pend_used[d] &&
+                        !(P.pend_used[o] && (P.pend_cb.tcol[o] == tcol || P.pend_cb.tcol[o] == bcol ||
+                                             P.pend_cb.bcol[o] == tcol));
+      if (!join) KPM_TRY(prim_flush(ctx));
+    }
+    if (!P.pending) {
+      const int scratch = (int)acc->NC;
+      P.pending = true;
+      P.pend_mu = acc->d;
+      P.pend_part = P.part_ptr;
+      P.pend_G = P.G;
+      P.pend_NRp = P.NRp;
+      P.pend_NR = P.NR;
+      P.pend_cb = Combine{2, {scratch, scratch}, {0.0, 0.0}, {-1, -1}, {0.0, 0.0}};
+      P.pend_used[0] = P.pend_used[1] = false;
+    }
+    P.pend_cb.tcol[d] = tcol;
+    P.pend_cb.alpha[d] = alpha;
+    P.pend_cb.bcol[d] = bcol;
+    P.pend_cb.bscale[d] = beta_scale;
+    P.pend_used[d] = true;
+    return KPM_OK;
+  }
+  // not a dot of the last step: one dot-only pass, reduced on the device
+  KPM_TRY(prim_flush(ctx));
+  int G = 0;
+  KPM_TRY(launch_dot(ctx, A, B, &G));
+  const Combine c{1, {tcol, 0}, {alpha, 0.0}, {bcol, -1}, {beta_scale, 0.0}};
+  return launch_combine(ctx, G, A->NR, A->NRp, c, acc->d, acc->NC);
+}
+
+int kpm_accum_download(kpm_ctx* ctx, const kpm_accum* acc, double* host) {
+  if (!ctx || !acc || !host) return fail(KPM_EINVAL, "null argument");
+  KPM_HIP_TRY(hipSetDevice(ctx->device));
+  if (ctx->prim.pending && ctx->prim.pend_mu == acc->d) KPM_TRY(prim_flush(ctx));
+  KPM_HIP_TRY(hipMemcpyAsync(host, acc->d, (size_t)(acc->NR * acc->NC) * sizeof(double2),
+                             hipMemcpyDeviceToHost, ctx->stream));
+  ++ctx->counters[KPM_COUNT_SYNC];
+  KPM_HIP_TRY(hipStreamSynchronize(ctx->stream));
   return KPM_OK;
 }
 
@@ -284,6 +477,7 @@ int kpm_bench_step(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z, i
   if (!ctx || !ms || nsteps < 1) return fail(KPM_EINVAL, "bad arguments");
   KPM_TRY(check_block_pair(H, X, Z));
   KPM_HIP_TRY(hipSetDevice(ctx->device));
+  ++ctx->epoch;
   KPM_HIP_TRY(hipEventRecord(ctx->events[14], ctx->stream));
   for (int64_t s = 0; s < nsteps; ++s)
     KPM_TRY(step(ctx, H, X, Z, Z, nullptr, 2.0, -1.0, MODE_DBL, nullptr));
@@ -298,6 +492,7 @@ int kpm_bench_copy(kpm_ctx* ctx, kpm_block* dst, const kpm_block* src, int64_t r
   if (dst->NH != src->NH || dst->NRp != src->NRp || dst->prec != src->prec)
     return fail(KPM_EINVAL, "block shapes differ");
   KPM_HIP_TRY(hipSetDevice(ctx->device));
+  ++ctx->epoch;
   const int64_t nwords = dst->NH * dst->NRp * elem_bytes(dst->prec) / 16;
   KPM_HIP_TRY(hipEventRecord(ctx->events[14], ctx->stream));
   for (int64_t r = 0; r < reps; ++r) KPM_TRY(launch_copy16(ctx, src->d, dst->d, nwords));

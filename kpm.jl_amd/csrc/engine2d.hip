@@ -165,8 +165,15 @@ static int dual_operator(kpm_ctx* ctx, const kpm_csr* A, const kpm_csr* B, int64
   char* base = nullptr;
   if (ctx->merged_key[0] != A->serial || ctx->merged_key[1] != B->serial) {
     ctx->merged_key[0] = ctx->merged_key[1] = 0;
-    // no room for the merged operator: the two passes run separately
-    if (ensure(ctx, ctx->merged, brp + bci + 2 * bv + bcnt + bblk + 256) != KPM_OK) return KPM_OK;
+    // no room for the merged operator (over the context's limit, or the
+    // allocation fails; ensure clears HIP's latched status): the two passes
+    // run separately
+    const size_t need = brp + bci + 2 * bv + bcnt + bblk + 256;
+    if (ctx->merged_limit >= 0 && need > (size_t)ctx->merged_limit) return KPM_OK;
+    if (ensure(ctx, ctx->merged, need) != KPM_OK) {
+      set_error("");
+      return KPM_OK;
+    }
     base = (char*)ctx->merged.ptr;
     int* rpu = (int*)base;
     int* cnt = (int*)(base + brp + bci + 2 * bv);

@@ -26,6 +26,15 @@
 
 namespace kpm {
 
+struct PendingCombine;
+struct Combine {
+  int nd;
+  int tcol[2];
+  double alpha[2];
+  int bcol[2];
+  double bscale[2];
+};
+
 // ---- error plumbing -------------------------------------------------------
 void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
@@ -59,6 +68,31 @@ struct Buffer {
 
 }  // namespace kpm
 
+// State of the primitive recurrence path (kpm_cheb_step / kpm_spmm followed
+// by columnwise_dot! calls): every primitive step also reduces <Vp|Vp> and
+// <Vp|Vout> into per-workgroup partials (two halves, alternating), keyed on
+// the step's blocks and the context's write epoch; a matching dot combines
+// those partials instead of re-reading the blocks, and the combine is
+// deferred into the next primitive step's launch when it can be.
+struct kpm_prim_state {
+  kpm::Buffer part;             // 2 halves of per-workgroup dot partials
+  int k = 0;                    // primitive steps so far (selects the half)
+  // dots of the last primitive step
+  bool cached = false;
+  const void* vp = nullptr;     // device data of Vp (d0 = <Vp|Vp>)
+  const void* vout = nullptr;   // device data of Vout (d1 = <Vp|Vout>)
+  uint64_t epoch = 0;
+  void* part_ptr = nullptr;     // the half holding them
+  int G = 0, NRp = 0, NR = 0;
+  // combine waiting for the next primitive step (or a flush)
+  bool pending = false;
+  void* pend_mu = nullptr;      // accumulator data
+  void* pend_part = nullptr;    // partials it reduces
+  int pend_G = 0, pend_NRp = 0, pend_NR = 0;
+  kpm::Combine pend_cb{2, {0, 0}, {0.0, 0.0}, {-1, -1}, {0.0, 0.0}};
+  bool pend_used[2] = {false, false};
+};
+
 struct kpm_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -85,6 +119,12 @@ struct kpm_ctx {
   int merged_max_row = 0;
   int64_t merged_nnz = 0;
   int dual_pass = 1;      // fuse the kpm_2d / kpm_3d right chain with the Jb pass
+  int64_t merged_limit = -1;  // bytes the merged dual-pass operator may take (< 0: no limit)
+  // launch / synchronisation counters (kpm_ctx_counters)
+  int64_t counters[KPM_COUNT_N] = {};
+  // incremented by every entry point that writes a caller's block
+  uint64_t epoch = 1;
+  kpm_prim_state prim;
 };
 
 struct kpm_csr {
@@ -191,13 +231,7 @@ int launch_step_seg(kpm_ctx* ctx, const StepLaunch& s, const StepGeom& g, int G,
 int launch_dot(kpm_ctx* ctx, const kpm_block* a, const kpm_block* b, int* grid_used);
 // Reduce partials [grid][nd][NRp] in a fixed order and write moment columns:
 //   mu[:, tcol[d]] = alpha[d]*sum_d + bscale[d]*mu[:, bcol[d]]  (bcol < 0: none)
-struct Combine {
-  int nd;
-  int tcol[2];
-  double alpha[2];
-  int bcol[2];
-  double bscale[2];
-};
+// (struct Combine is declared at the top of this header)
 int launch_combine(kpm_ctx* ctx, int grid, int64_t NR, int64_t NRp, const Combine& c,
                    void* mu_dev, int64_t NCmu);
 // the same reduction over an explicit partials buffer

@@ -90,6 +90,7 @@ int kpm_block_random_phase(kpm_block* b, uint64_t seed, int64_t lanes) {
   if (lanes < 1 || lanes > (int64_t)1 << 24) return fail(KPM_EINVAL, "lanes must be in [1, 2^24]");
   kpm_ctx* ctx = b->ctx;
   KPM_HIP_TRY(hipSetDevice(ctx->device));
+  ++ctx->epoch;
   // padded columns stay zero
   KPM_HIP_TRY(hipMemsetAsync(b->d, 0, (size_t)(b->NH * b->NRp) * elem_bytes(b->prec), ctx->stream));
   const unsigned g = (unsigned)((lanes + 255) / 256);

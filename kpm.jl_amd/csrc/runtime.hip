@@ -36,6 +36,10 @@ int ensure(kpm_ctx* ctx, Buffer& b, size_t bytes) {
   hipError_t e = hipMalloc(&b.ptr, bytes);
   if (e != hipSuccess) {
     b.ptr = nullptr;
+    // HIP keeps a failed call's status until hipGetLastError reads it; a
+    // caller that tolerates the failure (and falls back) must not see it
+    // again at its next launch check
+    (void)hipGetLastError();
     return fail(KPM_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
                                 hipGetErrorString(e));
   }
@@ -114,6 +118,7 @@ int upload_csr(kpm_ctx* ctx, int64_t n, const std::vector<int64_t>& rp,
     if (A->colind) (void)hipFree(A->colind);
     if (A->vals) (void)hipFree(A->vals);
     delete A;
+    (void)hipGetLastError();  // clear the latched status (see ensure)
     return fail(KPM_ENOMEM, std::string("hipMalloc for CSR failed: ") + hipGetErrorString(e));
   }
   std::vector<double> vr;
@@ -224,6 +229,7 @@ int csr_from_csc_staged(kpm_ctx* ctx, int64_t n, const int64_t* colptr, const in
     if (A->colind) (void)hipFree(A->colind);
     if (A->vals) (void)hipFree(A->vals);
     delete A;
+    (void)hipGetLastError();  // clear the latched status (see ensure)
     return fail(KPM_ENOMEM, std::string("hipMalloc for CSR failed: ") + hipGetErrorString(e));
   }
   e = hipMemcpyAsync(A->rowptr, rp, brp, hipMemcpyHostToDevice, ctx->stream);
@@ -314,6 +320,7 @@ int kpm_ctx_destroy(kpm_ctx* ctx) {
   free_buffer(ctx->adjoint);
   free_buffer(ctx->adj_grad);
   free_buffer(ctx->merged);
+  free_buffer(ctx->prim.part);
   if (ctx->pinned.ptr) (void)hipHostFree(ctx->pinned.ptr);
   for (auto& ev : ctx->events)
     if (ev) (void)hipEventDestroy(ev);
@@ -331,6 +338,21 @@ int kpm_ctx_synchronize(kpm_ctx* ctx) {
 int kpm_ctx_set_dual_pass(kpm_ctx* ctx, int enable) {
   if (!ctx) return fail(KPM_EINVAL, "null context");
   ctx->dual_pass = enable ? 1 : 0;
+  return KPM_OK;
+}
+
+int kpm_ctx_set_merged_limit(kpm_ctx* ctx, int64_t bytes) {
+  if (!ctx) return fail(KPM_EINVAL, "null context");
+  ctx->merged_limit = bytes;
+  ctx->merged_key[0] = ctx->merged_key[1] = 0;  // re-plan the merged operator
+  return KPM_OK;
+}
+
+int kpm_ctx_counters(kpm_ctx* ctx, int64_t* counts, int n, int reset) {
+  if (!ctx || (n > 0 && !counts)) return fail(KPM_EINVAL, "bad arguments");
+  for (int i = 0; i < n && i < KPM_COUNT_N; ++i) counts[i] = ctx->counters[i];
+  if (reset)
+    for (int i = 0; i < KPM_COUNT_N; ++i) ctx->counters[i] = 0;
   return KPM_OK;
 }
 
@@ -428,6 +450,7 @@ int kpm_block_create(kpm_ctx* ctx, int64_t NH, int64_t NR, int vec_prec, kpm_blo
   const size_t bytes = (size_t)(NH * b->NRp) * elem_bytes(vec_prec);
   hipError_t e = hipMalloc(&b->d, bytes);
   if (e != hipSuccess) {
+    (void)hipGetLastError();  // clear the latched status (see ensure)
     delete b;
     return fail(KPM_ENOMEM, "hipMalloc of block failed: " + std::string(hipGetErrorString(e)));
   }
@@ -444,6 +467,7 @@ int kpm_block_create(kpm_ctx* ctx, int64_t NH, int64_t NR, int vec_prec, kpm_blo
 
 int kpm_block_destroy(kpm_block* b) {
   if (!b) return KPM_OK;
+  if (b->ctx) ++b->ctx->epoch;  // its address may come back for another block
   if (b->owned && b->d) {
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
@@ -457,6 +481,7 @@ int kpm_block_upload(kpm_block* b, const double* host) {
   if (!b || !host) return fail(KPM_EINVAL, "null argument");
   kpm_ctx* ctx = b->ctx;
   KPM_HIP_TRY(hipSetDevice(ctx->device));
+  ++ctx->epoch;
   const size_t hbytes = (size_t)(b->NH * b->NR) * 16;
   KPM_TRY(ensure(ctx, ctx->staging, hbytes));
   KPM_HIP_TRY(hipMemcpyAsync(ctx->staging.ptr, host, hbytes, hipMemcpyHostToDevice, ctx->stream));
@@ -483,6 +508,7 @@ int kpm_block_copy(kpm_block* dst, const kpm_block* src) {
     return fail(KPM_EINVAL, "block shapes or precisions differ");
   if (dst == src || dst->d == src->d) return KPM_OK;
   kpm_ctx* ctx = dst->ctx;
+  ++ctx->epoch;
   KPM_HIP_TRY(hipMemcpyAsync(dst->d, src->d, (size_t)(dst->NH * dst->NRp) * elem_bytes(dst->prec),
                              hipMemcpyDeviceToDevice, ctx->stream));
   return KPM_OK;
@@ -490,6 +516,7 @@ int kpm_block_copy(kpm_block* dst, const kpm_block* src) {
 
 int kpm_block_zero(kpm_block* b) {
   if (!b) return fail(KPM_EINVAL, "null argument");
+  ++b->ctx->epoch;
   KPM_HIP_TRY(hipMemsetAsync(b->d, 0, (size_t)(b->NH * b->NRp) * elem_bytes(b->prec), b->ctx->stream));
   return KPM_OK;
 }

@@ -202,6 +202,8 @@ int launch_step(kpm_ctx* ctx, const StepLaunch& s, const StepConfig& cfg, int* g
      (int)ntiles, s.alpha, s.beta, g.has_tail, nullptr, nullptr, nullptr, s.gamma, pc, s.vals2,
      s.out2 ? s.out2->d : nullptr, o2);
   KPM_HIP_TRY(hipGetLastError());
+  ++ctx->counters[KPM_COUNT_STEP];
+  if (s.mode == MODE_DUAL) ++ctx->counters[KPM_COUNT_DUAL];
   if (grid_used) *grid_used = G;
   return KPM_OK;
 }
@@ -214,6 +216,7 @@ int launch_step_seg(kpm_ctx* ctx, const StepLaunch& s, const StepGeom& g, int G,
      s.out->d, nullptr, partials, (int)x->NH, g.RW, (int)x->NRp, (int)ntiles, s.alpha, s.beta,
      g.has_tail, tile_lo, treal, seg0, 0.0, PendingCombine{}, nullptr, nullptr, SplitOut{});
   KPM_HIP_TRY(hipGetLastError());
+  ++ctx->counters[KPM_COUNT_STEP];
   return KPM_OK;
 }
 
@@ -253,6 +256,7 @@ int launch_dot(kpm_ctx* ctx, const kpm_block* a, const kpm_block* b, int* grid_u
   fn(dim3(G, chunks), ctx->stream, a->d, b->d, (double2*)ctx->partials.ptr, (int)a->NH, RW,
      (int)a->NRp, (int)ntiles);
   KPM_HIP_TRY(hipGetLastError());
+  ++ctx->counters[KPM_COUNT_DOT];
   if (grid_used) *grid_used = G;
   return KPM_OK;
 }
@@ -267,6 +271,7 @@ int launch_combine_from(kpm_ctx* ctx, const double2* partials, int grid, int64_t
   hipLaunchKernelGGL(k_combine, dim3((unsigned)(c.nd * NR)), dim3(kBlock), 0, ctx->stream, partials,
                      grid, c.nd, (int)NRp, (int)NR, (double2*)mu_dev, c);
   KPM_HIP_TRY(hipGetLastError());
+  ++ctx->counters[KPM_COUNT_COMBINE];
   return KPM_OK;
 }
 

@@ -56,6 +56,7 @@ int kpm_autotune_step(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z
   if (X->NH != H->n || Z->NH != H->n || X->NR != Z->NR || X->prec != Z->prec)
     return fail(KPM_EINVAL, "block shapes do not match");
   KPM_HIP_TRY(hipSetDevice(ctx->device));
+  ++ctx->epoch;
   const int unrolls[2] = {1, 2};
   const int per_cu[4] = {2, 4, 8, 16};
   float best = 1e30f;

@@ -29,7 +29,7 @@ from .stiffness import (diamagnetic_term, nambu_current_q, nambu_diamagnetic, su
                         two_energy_response)
 from .cpge import Table3, cpge, d_cpge
 from . import ensemble
-from .device import (ACTIVE_DEVICE, CPUDevice, DeviceBlock, DeviceCSR, HIPDevice, activate,
+from .device import (ACTIVE_DEVICE, CPUDevice, DeviceAccumulator, DeviceBlock, DeviceCSR, HIPDevice, activate,
                      device_free_memory, get_context, maybe_on_device_zeros, maybe_to_host,
                      to_device, whichcore)
 from .frontend import (ConductivityMoments, CurrentMoments, DosMoments, GreenMoments,
@@ -41,7 +41,8 @@ from .kernels import (JacksonKernel, LorentzKernels, hn, mu2D_apply_kernel_and_h
 from .moment import (engine_blocking, kpm_1d, kpm_1d_, kpm_1d_current, kpm_1d_current_,
                      kpm_2d, kpm_2d_, kpm_2d_blocking, kpm_3d, kpm_3d_)
 from .primitives import (chebyshev_iter, chebyshev_iter_single, chebyshev_iter_wrap,
-                         columnwise_dot, moment_accumulator, normalize_by_col, r2_i, r2_ip,
+                         columnwise_dot, copy_moment_accumulator, moment_accumulator,
+                         normalize_by_col, r2_i, r2_ip,
                          r2_ipp, r_i, r_ip, r_ipp, spmm)
 from .rescale import normalizeH
 from .rng import Xoshiro256pp, default_rng

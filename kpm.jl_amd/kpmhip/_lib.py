@@ -28,6 +28,9 @@ GEMM_F64 = 0
 GEMM_F32 = 1
 GEMM_BF16X3 = 2
 PARITY = {"NONE": 0, "ODD": 1, "EVEN": 2}
+# kpm_ctx_counters indices
+COUNT_STEP, COUNT_DUAL, COUNT_DOT, COUNT_COMBINE, COUNT_SYNC = 0, 1, 2, 3, 4
+COUNT_N = 8
 
 
 class ArgumentError(ValueError):
@@ -84,6 +87,13 @@ _SIGS = {
     "kpm_cheb_step": (C.c_int, [_p, _p, _p, _p, _p]),
     "kpm_spmm": (C.c_int, [_p, _p, C.c_double, _p, C.c_double, _p]),
     "kpm_colwise_dot": (C.c_int, [_p, _pd, _i64, _i64, _p, _p, C.c_double, _i64, C.c_double]),
+    "kpm_accum_create": (C.c_int, [_p, _i64, _i64, C.POINTER(_p)]),
+    "kpm_accum_destroy": (C.c_int, [_p]),
+    "kpm_accum_dot": (C.c_int, [_p, _p, _i64, _p, _p, C.c_double, _i64, C.c_double]),
+    "kpm_accum_download": (C.c_int, [_p, _p, _pd]),
+    "kpm_ctx_set_merged_limit": (C.c_int, [_p, _i64]),
+    "kpm_ctx_counters": (C.c_int, [_p, _pi64, C.c_int, C.c_int]),
+    "kpm_batch_graph_builds": (C.c_int, [_p, _pi64]),
     "kpm_1d": (C.c_int, [_p, _p, _i64, _i64, _i64, _pd, _pd, C.c_int]),
     "kpm_1d_lr": (C.c_int, [_p, _p, _i64, _i64, _i64, _pd, _pd, _pd, C.c_int]),
     "kpm_1d_current": (C.c_int, [_p, _p, _p, _i64, _i64, _i64, _pd, _pd, C.c_int]),

@@ -79,6 +79,12 @@ class BatchOperator:
               "kpm_batch_info")
         return {"padded_rows": pr.value, "nnz": nnz.value, "segments": ns.value}
 
+    def graph_builds(self) -> int:
+        """How many times run(use_graph=True) has recorded its step graph."""
+        n = C.c_int64()
+        check(self.ctx.lib.kpm_batch_graph_builds(self.handle, C.byref(n)), "graph builds")
+        return int(n.value)
+
     def __del__(self):
         from .device import _shutdown
         try:

@@ -96,6 +96,21 @@ class Context:
         previous order (default) or run them as two passes; same tables."""
         check(self.lib.kpm_ctx_set_dual_pass(self.handle, 1 if enable else 0), "kpm_ctx_set_dual_pass")
 
+    def set_merged_limit(self, nbytes: int):
+        """Cap the merged operator of the dual pass (bytes; negative: no cap).
+        Over the cap kpm_2d / kpm_3d run the two passes separately."""
+        check(self.lib.kpm_ctx_set_merged_limit(self.handle, int(nbytes)), "merged limit")
+
+    def counters(self, reset: bool = False) -> dict:
+        """Launch counters of this context: recurrence passes, dual passes,
+        dot-only passes, separate moment reductions, host synchronisations."""
+        buf = (C.c_int64 * _lib.COUNT_N)()
+        check(self.lib.kpm_ctx_counters(self.handle, buf, _lib.COUNT_N, 1 if reset else 0),
+              "counters")
+        return {"step": int(buf[_lib.COUNT_STEP]), "dual": int(buf[_lib.COUNT_DUAL]),
+                "dot": int(buf[_lib.COUNT_DOT]), "combine": int(buf[_lib.COUNT_COMBINE]),
+                "sync": int(buf[_lib.COUNT_SYNC])}
+
     def free_memory(self) -> int:
         f = C.c_size_t(0)
         t = C.c_size_t(0)
@@ -279,6 +294,51 @@ class DeviceBlock:
         try:
             if getattr(self, "handle", None) and not _shutdown[0]:
                 self.ctx.lib.kpm_block_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class DeviceAccumulator:
+    """The device moment accumulator (kpm_accum): a zeroed NR x NC
+    ComplexF64 matrix in device memory, the value of
+    ``moment_accumulator(::CuArray, target)`` (ext/KPMCUDAExt.jl:125-126)."""
+
+    def __init__(self, NR: int, NC: int, ctx: Context | None = None):
+        self.ctx = ctx or get_context()
+        self.NR = int(NR)
+        self.NC = int(NC)
+        h = C.c_void_p()
+        check(self.ctx.lib.kpm_accum_create(self.ctx.handle, self.NR, self.NC, C.byref(h)),
+              "moment_accumulator")
+        self.handle = h
+
+    @property
+    def shape(self):
+        return (self.NR, self.NC)
+
+    def dot(self, n: int, A: DeviceBlock, B: DeviceBlock, alpha=1.0, beta_col: int = 0,
+            beta_scale=0.0):
+        """columnwise_dot!(self, n, A, B; alpha, beta_col, beta_scale), 1-based
+        n and beta_col (src/utils/vectors.jl:103-117; ext/KPMCUDAExt.jl:128-145)."""
+        check(self.ctx.lib.kpm_accum_dot(self.ctx.handle, self.handle, int(n), A.handle, B.handle,
+                                         float(alpha), int(beta_col), float(beta_scale)),
+              "columnwise_dot!")
+
+    def download(self, out: np.ndarray | None = None) -> np.ndarray:
+        """copy_moment_accumulator!(out, self) (ext/KPMCUDAExt.jl:147-150)."""
+        if out is None:
+            out = np.zeros((self.NR, self.NC), dtype=np.complex128, order="F")
+        if out.shape != (self.NR, self.NC) or out.dtype != np.complex128 or not out.flags.f_contiguous:
+            raise _lib.ArgumentError("target must be a Fortran-ordered complex128 NR x NC matrix")
+        check(self.ctx.lib.kpm_accum_download(self.ctx.handle, self.handle, dptr(out)),
+              "copy_moment_accumulator!")
+        return out
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None) and not _shutdown[0]:
+                self.ctx.lib.kpm_accum_destroy(self.handle)
                 self.handle = None
         except Exception:
             pass

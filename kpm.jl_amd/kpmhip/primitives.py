@@ -8,7 +8,7 @@ from __future__ import annotations
 import numpy as np
 
 from ._lib import ArgumentError, check, dptr
-from .device import DeviceBlock, DeviceCSR, get_context, to_device
+from .device import DeviceAccumulator, DeviceBlock, DeviceCSR, get_context, to_device
 
 
 def _H(H, ctx) -> DeviceCSR:
@@ -51,16 +51,43 @@ def spmm(A, X: DeviceBlock, Y: DeviceBlock, alpha: float = 1.0, beta: float = 0.
           "mul!")
 
 
-def moment_accumulator(NR: int, NC: int) -> np.ndarray:
-    """The NR x NC ComplexF64 moment target (src/utils/vectors.jl:101)."""
-    return np.zeros((NR, NC), dtype=np.complex128, order="F")
+def moment_accumulator(alpha_all, target):
+    """moment_accumulator(α_all, target) (src/utils/vectors.jl:101,
+    ext/KPMCUDAExt.jl:125-126): the host target itself when the workspace is
+    on the host, a zeroed device accumulator of the target's shape when it is
+    a device block (or a list of device blocks)."""
+    blk = alpha_all[0] if isinstance(alpha_all, (list, tuple)) else alpha_all
+    if isinstance(blk, DeviceBlock):
+        NR, NC = np.shape(target)
+        return DeviceAccumulator(NR, NC, blk.ctx)
+    return target
 
 
-def columnwise_dot(target: np.ndarray, n: int, A: DeviceBlock, B: DeviceBlock, *, alpha=1.0,
+def copy_moment_accumulator(target: np.ndarray, source):
+    """copy_moment_accumulator!(target, source) (src/utils/vectors.jl:119-122,
+    ext/KPMCUDAExt.jl:147-150)."""
+    if isinstance(source, DeviceAccumulator):
+        source.download(target)
+    elif source is not target:
+        np.copyto(target, source)
+
+
+def columnwise_dot(target, n: int, A: DeviceBlock, B: DeviceBlock, *, alpha=1.0,
                    beta_col: int = 0, beta_scale=0.0):
     """columnwise_dot!(target, n, A, B; alpha, beta_col, beta_scale)
     (src/utils/vectors.jl:103-117): target[i,n] = alpha <A_i|B_i> +
-    beta_scale target[i,beta_col], 1-based n and beta_col."""
+    beta_scale target[i,beta_col], 1-based n and beta_col.  A device
+    accumulator takes the dots on the device without a host round trip;
+    a host target is written through one synchronising call.  Complex alpha
+    or beta_scale raise ArgumentError (the device reduction takes reals)."""
+    for name, v in (("alpha", alpha), ("beta_scale", beta_scale)):
+        if np.iscomplexobj(v) and np.imag(v) != 0:
+            raise ArgumentError(f"{name} must be real (got {v!r})")
+    if isinstance(target, DeviceAccumulator):
+        if target.NR != A.NR:
+            raise ArgumentError("target rows must equal the number of probe columns")
+        target.dot(n, A, B, float(np.real(alpha)), int(beta_col), float(np.real(beta_scale)))
+        return
     if target.dtype != np.complex128 or not target.flags.f_contiguous:
         raise ArgumentError("target must be a Fortran-ordered complex128 NR x NC matrix")
     NR, NC = target.shape
@@ -111,6 +138,6 @@ def r2_ipp(n: int) -> int:
 
 __all__ = [
     "chebyshev_iter_single", "chebyshev_iter", "chebyshev_iter_wrap", "spmm", "moment_accumulator",
-    "columnwise_dot", "normalize_by_col", "r_i", "r_ip", "r_ipp", "r2_i", "r2_ip", "r2_ipp",
+    "copy_moment_accumulator", "columnwise_dot", "normalize_by_col", "r_i", "r_ip", "r_ipp", "r2_i", "r2_ip", "r2_ipp",
     "get_context",
 ]

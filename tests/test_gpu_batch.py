@@ -62,6 +62,40 @@ def test_batch_graph_replays_match_eager():
         np.testing.assert_array_equal(e2, eager)
 
 
+def test_batch_graph_rerecorded_when_probe_width_changes():
+    """A graph recorded at NR = 4 copies from the resident probe block.  An
+    upload at NR = 16 reallocates that block, and an upload back at NR = 4
+    restores the original shape key: the graph must be recorded again (it
+    would otherwise copy from the freed block), and every replay must equal
+    the eager run at that width."""
+    Hs = ensemble(3)
+    B = K.BatchOperator(Hs)
+    NC = 64
+    ps4 = [K.random_phase_vectors(Xoshiro256pp(90 + i), H.shape[0], 4) for i, H in enumerate(Hs)]
+    ps16 = [K.random_phase_vectors(Xoshiro256pp(120 + i), H.shape[0], 16) for i, H in enumerate(Hs)]
+    e4 = K.kpm_batch_1d(B, NC, 4, ps4, avg_output=False)
+    e16 = K.kpm_batch_1d(B, NC, 16, ps16, avg_output=False)
+    builds = B.graph_builds()
+    g4 = K.kpm_batch_1d(B, NC, 4, ps4, avg_output=False, use_graph=True)
+    np.testing.assert_array_equal(g4, e4)
+    assert B.graph_builds() == builds + 1
+    g16 = K.kpm_batch_1d(B, NC, 16, ps16, avg_output=False, use_graph=True)
+    np.testing.assert_array_equal(g16, e16)
+    assert B.graph_builds() == builds + 2
+    g4b = K.kpm_batch_1d(B, NC, 4, ps4, avg_output=False, use_graph=True)
+    np.testing.assert_array_equal(g4b, e4)
+    assert B.graph_builds() >= builds + 2
+    # same width, same buffers: the recorded graph is replayed as it is
+    n = B.graph_builds()
+    g4c = K.kpm_batch_1d(B, NC, 4, ps4, avg_output=False, use_graph=True)
+    np.testing.assert_array_equal(g4c, e4)
+    assert B.graph_builds() == n
+    # the autotuner may rebuild the launch schedule: the graph follows it
+    B.autotune(reps=2)
+    g4d = K.kpm_batch_1d(B, NC, 4, ps4, avg_output=False, use_graph=True)
+    np.testing.assert_allclose(g4d, e4, rtol=0, atol=1e-12)
+
+
 def test_batch_rejects_coupled_blocks():
     import scipy.sparse as sp
     H = models.chain(10)

@@ -17,7 +17,10 @@ Cases:
 - kpm_3d with the outer orders batched as column blocks, and nested;
 - an empty Jb or Ja (no stored entries);
 - a Jb with a duplicated stored entry, whose repeated column switches the
-  fused pass off.
+  fused pass off;
+- a merged operator over the context's size limit, which falls back to the
+  separate passes as when its allocation fails.
+The context's dual-pass counter shows which form actually ran.
 """
 import numpy as np
 import pytest
@@ -38,14 +41,30 @@ def phases(seed, n, nr):
     return K.random_phase_vectors(Xoshiro256pp(seed), n, nr)
 
 
-def fused_and_separate(fn):
+LAST_DUAL = {}
+
+
+def fused_and_separate(fn, expect_fused=True):
+    """Run fn with the dual pass off, then on; check with the context's
+    dual-pass counter that the separate run used none and that the fused run
+    used it exactly when expected."""
     ctx = K.get_context()
     ctx.set_dual_pass(False)
     try:
+        ctx.counters(reset=True)
         sep = fn()
+        assert ctx.counters()["dual"] == 0
     finally:
         ctx.set_dual_pass(True)
-    return fn(), sep
+    ctx.counters(reset=True)
+    got = fn()
+    dual = ctx.counters()["dual"]
+    LAST_DUAL["n"] = dual
+    if expect_fused:
+        assert dual > 0, "the fused pass did not run"
+    else:
+        assert dual == 0, f"the fused pass ran {dual} times where it must not apply"
+    return got, sep
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
@@ -57,8 +76,10 @@ def test_kpm2d_haldane_fused_equals_separate(prec):
     ref = oracle.kpm_2d(Hn, Jx, Jy, NC, pl, pr, NC, NC)
     for KL, KR in ((NC, NC), (5, 3), (2, 1), (7, 24), (NC, 2)):
         print(f"haldane kpm_2d {prec} KL={KL} KR={KR}", flush=True)
+        # a right block of one order has no previous order to fuse with
         got, sep = fused_and_separate(lambda: K.kpm_2d(Hn, Jx, Jy, NC, NR, psi_in_l=pl, psi_in_r=pr,
-                                                       arr_size=KL, right_block=KR, precision=prec))
+                                                       arr_size=KL, right_block=KR, precision=prec),
+                                      expect_fused=KR >= 2)
         np.testing.assert_array_equal(got, sep)
         if prec == "f64":
             np.testing.assert_allclose(got, ref, **TOL)
@@ -97,7 +118,7 @@ def test_kpm2d_one_lane_rows():
     NC = 12
     pl, pr = phases(11, H.shape[0], 1), phases(12, H.shape[0], 1)
     got, sep = fused_and_separate(lambda: K.kpm_2d(Hn, Jx, Jy, NC, 1, psi_in_l=pl, psi_in_r=pr,
-                                                   arr_size=5, right_block=3))
+                                                   arr_size=5, right_block=3), expect_fused=False)
     np.testing.assert_array_equal(got, sep)
     np.testing.assert_allclose(got, oracle.kpm_2d(Hn, Jx, Jy, NC, pl, pr, 5, 3), **TOL)
 
@@ -154,6 +175,44 @@ def test_kpm2d_duplicate_entries_take_separate_passes():
     assert Jdup.nnz == Jb.nnz + 1 and not Jdup.has_canonical_format
     pl, pr = phases(34, n, NR), phases(35, n, NR)
     ref = oracle.kpm_2d(H, Ja, Jb, NC, pl, pr, NC, NC)
-    got, sep = fused_and_separate(lambda: K.kpm_2d(H, Ja, Jdup, NC, NR, psi_in_l=pl, psi_in_r=pr))
+    got, sep = fused_and_separate(lambda: K.kpm_2d(H, Ja, Jdup, NC, NR, psi_in_l=pl, psi_in_r=pr),
+                                  expect_fused=False)
     np.testing.assert_array_equal(got, sep)
     np.testing.assert_allclose(got, ref, **TOL)
+
+
+def test_kpm2d_merged_operator_over_limit_falls_back():
+    """No room for the merged operator (here: a context limit of 0 bytes, the
+    same path as a failed allocation): the passes run separately, the table
+    still equals the oracle's, and lifting the limit brings the fused pass
+    back on the same operators."""
+    H, Jx, Jy, area = models.haldane_model(5, 5, t2=0.2)
+    Hn = (H / 4.0).tocsc()
+    NR, NC = 4, 16
+    pl, pr = phases(51, H.shape[0], NR), phases(52, H.shape[0], NR)
+    ref = oracle.kpm_2d(Hn, Jx, Jy, NC, pl, pr, NC, NC)
+    ctx = K.get_context()
+    ctx.set_merged_limit(0)
+    try:
+        ctx.counters(reset=True)
+        got = K.kpm_2d(Hn, Jx, Jy, NC, NR, psi_in_l=pl, psi_in_r=pr)
+        assert ctx.counters()["dual"] == 0
+    finally:
+        ctx.set_merged_limit(-1)
+    np.testing.assert_allclose(got, ref, **TOL)
+    ctx.counters(reset=True)
+    again = K.kpm_2d(Hn, Jx, Jy, NC, NR, psi_in_l=pl, psi_in_r=pr)
+    assert ctx.counters()["dual"] > 0
+    np.testing.assert_array_equal(again, got)
+
+
+def test_failed_allocation_does_not_poison_the_next_call():
+    """A device allocation that fails (a block far beyond HBM) raises
+    MemoryError and leaves no latched HIP status behind: the next engine call
+    on the same context runs and matches the oracle."""
+    with pytest.raises(MemoryError):
+        K.DeviceBlock(1 << 40, 64)
+    H = ragged_hermitian(40, 3, seed=61)
+    psi = phases(62, 40, 3)
+    got = K.kpm_1d(H, 16, 3, psi_in=psi, avg_output=False)
+    np.testing.assert_allclose(got, oracle.kpm_1d(H, 16, psi), **TOL)

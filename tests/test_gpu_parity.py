@@ -215,7 +215,7 @@ def test_chebyshev_iter_and_colwise_dot():
     v1 = 2 * Hd @ v0 - s1
     np.testing.assert_allclose(slots[0].download(), v0, **TOL)
     np.testing.assert_allclose(slots[1].download(), v1, **TOL)
-    tgt = K.moment_accumulator(NR, 4)
+    tgt = np.zeros((NR, 4), dtype=np.complex128, order="F")
     K.columnwise_dot(tgt, 1, slots[2], slots[3])
     K.columnwise_dot(tgt, 2, slots[2], slots[3], alpha=2.0, beta_col=1, beta_scale=-1.0)
     d = np.einsum("ir,ir->r", ref[:, :, 2].conj(), ref[:, :, 3])
