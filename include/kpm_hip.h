@@ -316,6 +316,20 @@ int kpm_bench_trilinear(kpm_table3* t, int64_t Q, int64_t reps, float* ms);
 int kpm_1d_adjoint(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t NR, int64_t NH,
                    const double* psi, const double* g, double* grad, int vec_prec,
                    int64_t ckpt_stride);
+/* No counterpart in the reference.  Reverse mode through the kpm_2d table
+ * mu[n,m] = (1/NR) sum_r <T_m(H) psi_l_r | Jb T_n(H) Ja psi_r_r> (n, m < NC;
+ * the table of _kpm_2d_blocked!, src/moment.jl:987-1088): for the cotangent
+ * gbar (NC x NC complex column-major, dL = Re sum_nm conj(gbar[n,m])
+ * dmu[n,m]) writes the derivatives on the stored entries of H, Ja and Jb in
+ * the order of the CSC arrays each was built from (interleaved complex,
+ * dL = Re sum_k grad[k] dA_k).  Reverse Clenshaw over both chains; keeps all
+ * 2 NC chain vectors and two mixes of them (4 NC blocks, FP64), cached in
+ * the context.  H, Ja, Jb from kpm_csr_from_csc; Jb_adj = adjoint(Jb) from
+ * kpm_csr_adjoint_from_csc.  psi_l, psi_r: NH x NR column-major complex. */
+int kpm_2d_adjoint(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, const kpm_csr* Jb,
+                   const kpm_csr* Jb_adj, int64_t NC, int64_t NR, int64_t NH, const double* psi_l,
+                   const double* psi_r, const double* gbar, double* grad_H, double* grad_Ja,
+                   double* grad_Jb);
 /* Host bookkeeping for per-bond gradients (no reference counterpart): for a
  * CSC pattern with sorted rows holding both (i, j) and (j, i), and grad on that
  * pattern (the output of kpm_1d_adjoint), write every pair i < j with

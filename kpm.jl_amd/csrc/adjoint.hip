@@ -33,7 +33,6 @@
 
 namespace kpm {
 
-constexpr int kWin = 8;
 
 template <int PREC>
 struct SddmmWindow {
@@ -281,16 +280,15 @@ static int launch_sddmm_t(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* const
   return KPM_OK;
 }
 
-static int launch_sddmm(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* const* beta,
-                        const kpm_block* const* alpha, const double* c, int W, double2* M) {
+int launch_sddmm(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* const* beta,
+                 const kpm_block* const* alpha, const double* c, int W, double2* M) {
   if (W < 1 || W > kWin) return fail(KPM_EINVAL, "sddmm window out of range");
   return beta[0]->prec == KPM_VEC_F32 ? launch_sddmm_t<KPM_VEC_F32>(ctx, A, beta, alpha, c, W, M)
                                       : launch_sddmm_t<KPM_VEC_F64>(ctx, A, beta, alpha, c, W, M);
 }
 
-static int run_step(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* x, const kpm_block* z,
-                    kpm_block* out, const kpm_block* src, double alpha, double beta, double gamma,
-                    int mode) {
+int adj_step(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* x, const kpm_block* z,
+             kpm_block* out, const kpm_block* src, double alpha, double beta, double gamma, int mode) {
   StepLaunch s{A, x, z, out, src, alpha, beta, mode, gamma};
   return launch_step(ctx, s, tuned_config(ctx, A, x->NR, x->prec, MODE_NONE), nullptr);
 }
@@ -381,7 +379,7 @@ int kpm_1d_adjoint(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t NR, int64
     // its term with alpha_{j-1}
     auto reverse = [&](int64_t j, const kpm_block& a_prev) -> int {
       kpm_block bx = beta(j + 1), bz = beta(j + 2), bo = beta(j);
-      KPM_TRY(run_step(ctx, H, &bx, &bz, &bo, &v, 2.0, -1.0, g[j], MODE_SRC));
+      KPM_TRY(adj_step(ctx, H, &bx, &bz, &bo, &v, 2.0, -1.0, g[j], MODE_SRC));
       pb[np] = bo;
       pa[np] = a_prev;
       pc[np] = j == 1 ? 1.0 : 2.0;
@@ -394,10 +392,10 @@ int kpm_1d_adjoint(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t NR, int64
       for (int64_t k = 1; k < nalpha && rc == KPM_OK; ++k) {
         kpm_block x = alpha_full(k - 1), o = alpha_full(k);
         if (k == 1) {
-          rc = run_step(ctx, H, &x, nullptr, &o, nullptr, 1.0, 0.0, 0.0, MODE_NONE);
+          rc = adj_step(ctx, H, &x, nullptr, &o, nullptr, 1.0, 0.0, 0.0, MODE_NONE);
         } else {
           kpm_block z = alpha_full(k - 2);
-          rc = run_step(ctx, H, &x, &z, &o, nullptr, 2.0, -1.0, 0.0, MODE_NONE);
+          rc = adj_step(ctx, H, &x, &z, &o, nullptr, 2.0, -1.0, 0.0, MODE_NONE);
         }
       }
       for (int64_t j = NC - 1; j >= 1 && rc == KPM_OK; --j) rc = reverse(j, alpha_full(j - 1));
@@ -406,11 +404,11 @@ int kpm_1d_adjoint(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t NR, int64
       // forward with checkpoint pairs (alpha_{sK}, alpha_{sK+1})
       kpm_block ring[2] = {blk(1 + nbr), blk(2 + nbr)};
       if ((rc = kpm_block_copy(&ring[0], &v)) != KPM_OK) break;
-      if ((rc = run_step(ctx, H, &ring[0], nullptr, &ring[1], nullptr, 1.0, 0.0, 0.0, MODE_NONE)) != KPM_OK) break;
+      if ((rc = adj_step(ctx, H, &ring[0], nullptr, &ring[1], nullptr, 1.0, 0.0, 0.0, MODE_NONE)) != KPM_OK) break;
       const int64_t klast = (nseg - 1) * K + 1;
       for (int64_t k = 1; k <= klast && rc == KPM_OK; ++k) {
         if (k >= 2)
-          rc = run_step(ctx, H, &ring[(k - 1) % 2], &ring[k % 2], &ring[k % 2], nullptr, 2.0, -1.0, 0.0, MODE_NONE);
+          rc = adj_step(ctx, H, &ring[(k - 1) % 2], &ring[k % 2], &ring[k % 2], nullptr, 2.0, -1.0, 0.0, MODE_NONE);
         if (rc == KPM_OK && (k - 1) % K == 0) {  // pair (alpha_{k-1}, alpha_k) starts segment (k-1)/K
           const int64_t sg = (k - 1) / K;
           kpm_block a = ckA(sg), b = ckB(sg);
@@ -430,7 +428,7 @@ int kpm_1d_adjoint(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t NR, int64
         }
         for (int64_t i = lo + 2; i <= hi && rc == KPM_OK; ++i) {
           kpm_block si = seg(i - lo), sm1 = seg(i - lo - 1), sm2 = seg(i - lo - 2);
-          rc = run_step(ctx, H, &sm1, &sm2, &si, nullptr, 2.0, -1.0, 0.0, MODE_NONE);
+          rc = adj_step(ctx, H, &sm1, &sm2, &si, nullptr, 2.0, -1.0, 0.0, MODE_NONE);
         }
         for (int64_t i = hi; i >= lo && rc == KPM_OK; --i) rc = reverse(i + 1, seg(i - lo));
         if (rc == KPM_OK) rc = flush();

@@ -111,6 +111,8 @@ struct kpm_ctx {
   kpm::Buffer pinned;     // page-locked host staging for operator uploads
   kpm::Buffer adjoint;    // adjoint workspace: v | beta ring | alpha store
   kpm::Buffer adj_grad;   // adjoint gradient accumulator (nnz complex)
+  kpm::Buffer adj2;       // kpm_2d adjoint: chain slots, mixes, Clenshaw ring
+  kpm::Buffer adj2_aux;   // kpm_2d adjoint: gradient accumulators, mix coefficients
   // merged operator of the dual pass: the union pattern of two operators
   // (rowptr | colind | first values | second values), rebuilt when the pair
   // of operator serials changes
@@ -267,6 +269,17 @@ int launch_copy16(kpm_ctx* ctx, const void* src, void* dst, int64_t nwords);
 int launch_contract(kpm_ctx* ctx, const void* Lslots, int nL, const void* Rslots, int nR,
                     int64_t K, int64_t pitch, int prec, int gemm_mode, double scale, void* mu_dev,
                     int64_t NC, int64_t n0, int64_t m0, int parity);
+
+// ---- reverse mode (adjoint.hip) ---------------------------------------------------
+// window of reverse steps whose gradient terms one SDDMM pass adds
+constexpr int kWin = 8;
+// M[k] += sum_{w < W} c[w] sum_r conj(beta_w[a, r]) alpha_w[colind[k], r] for
+// every stored entry k (row a) of A; W <= kWin
+int launch_sddmm(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* const* beta,
+                 const kpm_block* const* alpha, const double* c, int W, double2* M);
+// out = alpha*A*x + beta*z (+ gamma*src in MODE_SRC), no dots
+int adj_step(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* x, const kpm_block* z,
+             kpm_block* out, const kpm_block* src, double alpha, double beta, double gamma, int mode);
 
 // tuner table (tune.hip): best StepConfig for a shape, or defaults; record one
 StepConfig tuned_config(kpm_ctx* ctx, const kpm_csr* A, int64_t NR, int prec, int mode);

@@ -319,6 +319,8 @@ int kpm_ctx_destroy(kpm_ctx* ctx) {
   for (auto& b : ctx->ring) free_buffer(b);
   free_buffer(ctx->adjoint);
   free_buffer(ctx->adj_grad);
+  free_buffer(ctx->adj2);
+  free_buffer(ctx->adj2_aux);
   free_buffer(ctx->merged);
   free_buffer(ctx->prim.part);
   if (ctx->pinned.ptr) (void)hipHostFree(ctx->pinned.ptr);

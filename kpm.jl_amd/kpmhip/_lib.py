@@ -118,6 +118,8 @@ _SIGS = {
     "kpm_bond_gradient": (C.c_int, [_i64, _pi64, _pi64, _pd, _pd, C.c_double, _pi64, _pi64, _pd,
                                     _pi64]),
     "kpm_1d_adjoint": (C.c_int, [_p, _p, _i64, _i64, _i64, _pd, _pd, _pd, C.c_int, _i64]),
+    "kpm_2d_adjoint": (C.c_int, [_p, _p, _p, _p, _p, _i64, _i64, _i64, _pd, _pd, _pd, _pd, _pd,
+                                 _pd]),
     "kpm_batch_create": (C.c_int, [_p, C.c_int, _pi64, _pi64, _pi64, _pd, C.c_int, C.POINTER(_p)]),
     "kpm_batch_destroy": (C.c_int, [_p]),
     "kpm_batch_1d": (C.c_int, [_p, _p, _i64, _i64, _pd, _pd, C.c_int, C.c_int]),

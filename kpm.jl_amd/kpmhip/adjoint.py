@@ -118,3 +118,118 @@ def bond_gradient(H, grad_Hnorm, a: float) -> sp.coo_matrix:
                                            C.byref(nb)), "bond_gradient")
     m = nb.value
     return sp.coo_matrix((val[:m], (bi[:m], bj[:m])), shape=A.shape)
+
+
+# ---- reverse mode through kpm_2d and the Kubo-Bastin head ---------------------
+
+
+def kpm_2d_adjoint(H, Ja, Jb, NC: int, psi_l, psi_r, gbar):
+    """Derivatives of a real loss through the kpm_2d table
+    mu[n,m] = (1/NR) sum_r <T_m(H) psi_l_r | Jb T_n(H) Ja psi_r_r>
+    (src/moment.jl:987-1088) for the cotangent gbar (NC x NC complex,
+    dL = Re sum conj(gbar) dmu).  Returns (gH, gJa, gJb): CSC matrices on the
+    patterns of H, Ja and Jb with dL = Re sum_k g[k] dA[k] (FP64, GPU)."""
+    A, Aa, Ab = _csc(H), _csc(Ja), _csc(Jb)
+    NC = int(NC)
+    n = A.shape[0]
+    pl = np.asfortranarray(np.asarray(psi_l).reshape(n, -1), dtype=np.complex128)
+    pr = np.asfortranarray(np.asarray(psi_r).reshape(n, -1), dtype=np.complex128)
+    if pl.shape != pr.shape:
+        raise ArgumentError(f"psi_l and psi_r must have equal size (got {pl.shape} and {pr.shape})")
+    gb = np.asfortranarray(np.asarray(gbar, dtype=np.complex128))
+    if gb.shape != (NC, NC):
+        raise ArgumentError(f"gbar must be {NC} x {NC} (got {gb.shape})")
+    from .device import to_device_adjoint
+
+    ctx = get_context()
+    Hd, Jad, Jbd = to_device(A, ctx), to_device(Aa, ctx), to_device(Ab, ctx)
+    Jbh = to_device_adjoint(Ab, ctx)
+    outs = [np.zeros(X.nnz, dtype=np.complex128) for X in (A, Aa, Ab)]
+    check(ctx.lib.kpm_2d_adjoint(ctx.handle, Hd.handle, Jad.handle, Jbd.handle, Jbh.handle, NC,
+                                 pl.shape[1], n, dptr(pl), dptr(pr), dptr(gb), dptr(outs[0]),
+                                 dptr(outs[1]), dptr(outs[2])), "kpm_2d_adjoint")
+    return tuple(sp.csc_matrix((o, X.indices.copy(), X.indptr.copy()), shape=X.shape)
+                 for o, X in zip(outs, (A, Aa, Ab)))
+
+
+def kubo_bastin_weights(NC: int, a: float, Ef: float, *, b: float = 0.0, NH: int, area: float,
+                        beta: float = math.inf, kernel=JacksonKernel, quad_N: int | None = None,
+                        edge_cutoff: float = 1e-3) -> np.ndarray:
+    """The NC x NC complex K with kubo_bastin_cond(mu, ...) = Re sum K * mu
+    (src/applications/conductivity.jl:101-157 is linear in the moments):
+    K[n,m] = -2 NH/(area a^2) w_n w_m sum_k phi_k Gamma_nm(x_k), with the same
+    Gauss-Chebyshev nodes, Fermi weights and edge cutoff as the head and the
+    kernel weights w_n = g_n h_n.  Its conjugate is the cotangent of mu for
+    the loss sigma."""
+    from .applications import fermiFunctions, gausschebyshevt
+
+    NC = int(NC)
+    quad_N = 8 * NC if quad_N is None else int(quad_N)
+    ns = np.arange(NC)
+    w = np.asarray(kernel(ns, NC), dtype=np.float64) * hn(ns)
+    nodes, weights = gausschebyshevt(quad_N)
+    fv = fermiFunctions(Ef, beta)(a * nodes + b)
+    keep = (np.abs(nodes) < 1 - edge_cutoff) & (fv != 0)
+    if not np.any(keep):
+        return np.zeros((NC, NC), dtype=np.complex128)
+    x = nodes[keep]
+    phi = weights[keep] * fv[keep] / (1 - x ** 2) ** 1.5
+    th = np.arccos(x)[None, :]
+    s = np.sqrt(1 - x ** 2)[None, :]
+    Cn = np.cos(ns[:, None] * th)
+    Em = np.exp(1j * ns[:, None] * th)
+    WM = (x[None, :] - 1j * ns[:, None] * s) * Em        # Gamma's m-factor of the T_n term
+    WN = (x[None, :] + 1j * ns[:, None] * s) * np.conj(Em)  # n-factor of the T_m term
+    G = (Cn * phi[None, :]) @ WM.T + (WN * phi[None, :]) @ Cn.T
+    return (-2.0 * NH / (area * a * a)) * (w[:, None] * w[None, :]) * G
+
+
+def sigma_rrule(H, Ja, Jb, NC: int, psi, a: float, Ef: float, *, b: float = 0.0, area: float,
+                psi_r=None, **head_kwargs):
+    """(sigma, pullback): sigma = kubo_bastin_cond of the FP64 kpm_2d table of
+    the rescaled H (src/frontend.jl:270-327 + src/applications/
+    conductivity.jl:101-157), and pullback(sbar) = (gH, gJa, gJb), the
+    derivatives on the stored entries of H, Ja and Jb (dsigma = Re sum g dA)."""
+    from .moment import kpm_2d
+
+    psi = np.asarray(psi)
+    psi_r = psi if psi_r is None else np.asarray(psi_r)
+    NH = H.shape[0]
+    mu = kpm_2d(H, Ja, Jb, int(NC), psi.shape[1], psi_in_l=psi, psi_in_r=psi_r, gemm="f64")
+    K = kubo_bastin_weights(NC, a, Ef, b=b, NH=NH, area=area, **head_kwargs)
+    sigma = float(np.real(np.sum(K * mu)))
+
+    def pullback(sbar=1.0):
+        return kpm_2d_adjoint(H, Ja, Jb, NC, psi, psi_r, float(sbar) * np.conj(K))
+
+    return sigma, pullback
+
+
+def hopping_gradient(H, Ja, Jb, gH, gJa, gJb, a: float):
+    """Per-site and per-bond derivatives from the entry derivatives of
+    kpm_2d_adjoint, for the parametrisation of a tight-binding model whose
+    current operators are built from its hoppings: H = diag(eps) + sum_b t_b
+    (h_b + h_b^H) and J_x = sum_b t_b (j_b + j_b^H), so H_ij / t_b and
+    J_ij / t_b are fixed (t_b = |H_ij|, J on H's pattern or a subset).  The
+    table is of H_norm = (H - b I)/a with a, b fixed, so gH is per H_norm
+    entry.  Returns (d/deps_i for every site, COO of d/dt_b for every stored
+    pair i < j)."""
+    A = _csc(H)
+    inv = _inv_magnitude(A)
+    unit = sp.csc_matrix(A.multiply(inv))               # H_ij / |H_ij|
+    contrib = sp.csc_matrix(sp.csc_matrix(gH).multiply(unit)).real / a
+    for J, gJ in ((Ja, gJa), (Jb, gJb)):
+        Jhat = sp.csc_matrix(_csc(J).multiply(inv))     # J_ij / |H_ij|
+        contrib = contrib + sp.csc_matrix(sp.csc_matrix(gJ).multiply(Jhat)).real
+    site = np.real(sp.csc_matrix(gH).diagonal()) / a
+    C = sp.csc_matrix(contrib)
+    bonds = sp.triu(C + C.T, k=1).tocoo()
+    return site, bonds
+
+
+def _inv_magnitude(A):
+    """1/|A_ij| on A's pattern (0 where A_ij = 0)."""
+    M = _csc(A)
+    m = np.abs(M.data)
+    M.data = np.where(m > 0, 1.0 / np.where(m > 0, m, 1.0), 0.0).astype(np.complex128)
+    return M

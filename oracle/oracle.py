@@ -182,6 +182,39 @@ def kpm_2d(H, Ja, Jb, NC: int, psi_l, psi_r, K_left: int, K_right: int, parity: 
     return mu
 
 
+def kpm_2d_tangent(H, Ja, Jb, dH, dJa, dJb, NC: int, psi_l, psi_r):
+    """Forward-mode tangent of the kpm_2d table (numpy, small sizes):
+    mu[n,m] = (1/NR) sum_r <l_m|Jb r_n> with the left chain l_m = T_m(H) psi_l
+    and the right chain r_n = T_n(H) Ja psi_r of _kpm_2d_blocked!
+    (src/moment.jl:1013-1061; conj placement of :1055-1070), differentiated
+    along (dH, dJa, dJb) through the recurrences
+    dl_1 = dH l_0, dl_k = 2 dH l_{k-1} + 2 H dl_{k-1} - dl_{k-2} (and the same
+    for r with r_0 = Ja psi_r, dr_0 = dJa psi_r).  Returns (mu, dmu), NC x NC."""
+    H, Ja, Jb = (sp.csr_matrix(X, dtype=np.complex128) for X in (H, Ja, Jb))
+    dH, dJa, dJb = (sp.csr_matrix(X, dtype=np.complex128) for X in (dH, dJa, dJb))
+    pl = np.asarray(psi_l, dtype=np.complex128).reshape(H.shape[0], -1)
+    pr = np.asarray(psi_r, dtype=np.complex128).reshape(H.shape[0], -1)
+    NR = pl.shape[1]
+
+    def chain(v, dv):
+        X, dX = [v], [dv]
+        if NC > 1:
+            X.append(H @ v)
+            dX.append(dH @ v + H @ dv)
+        for _ in range(2, NC):
+            X.append(2 * (H @ X[-1]) - X[-2])
+            dX.append(2 * (dH @ X[-2]) + 2 * (H @ dX[-1]) - dX[-2])
+        return np.array(X), np.array(dX)
+
+    L, dL = chain(pl, np.zeros_like(pl))
+    R, dR = chain(Ja @ pr, dJa @ pr)
+    JR = np.array([Jb @ r for r in R])
+    dJR = np.array([dJb @ r + Jb @ d for r, d in zip(R, dR)])
+    mu = np.einsum("mir,nir->nm", L.conj(), JR) / NR
+    dmu = (np.einsum("mir,nir->nm", dL.conj(), JR) + np.einsum("mir,nir->nm", L.conj(), dJR)) / NR
+    return mu, dmu
+
+
 def kpm_3d(H, Ja, Jb, Jg, NC: int, psi_l, psi_r, K_left: int = 3, K_right: int | None = None):
     K_right = min(NC, 16) if K_right is None else K_right
     A, Ac, Bc, Gc = CSC(H), CSC(Ja), CSC(Jb), CSC(Jg)

@@ -19,13 +19,18 @@ SRC = os.path.join(ROOT, "oracle", "kpm_oracle.c")
 GOLD = os.path.join(ROOT, "tests", "golden")
 
 MUTANTS = {
-    "dot_without_conj": ("acc += conj(a[i]) * b[i];", "acc += a[i] * b[i];"),
+    "dot_without_conj": ("for (int64_t i = 0; i < n; ++i) acc += conj(a[i]) * b[i];",
+                         "for (int64_t i = 0; i < n; ++i) acc += a[i] * b[i];"),
     "doubling_wrong_base": ("colwise_dot(mu, NR, 2 * n - 1, views[ip], views[ip], NH, 2.0, 1, -1.0);",
                             "colwise_dot(mu, NR, 2 * n - 1, views[ip], views[ip], NH, 2.0, 2, -1.0);"),
     "doubling_factor": ("colwise_dot(mu, NR, 2 * n, views[ip], views[ipp], NH, 2.0, 2, -1.0);",
                         "colwise_dot(mu, NR, 2 * n, views[ip], views[ipp], NH, 1.0, 2, -1.0);"),
     "recurrence_sign": ("block_mul5(H, NH, NR, Vp, Vpp, 2.0, -1.0);",
                         "block_mul5(H, NH, NR, Vp, Vpp, 2.0, 1.0);"),
+    "contract_without_conj": ("sr += ar * br + ai * bi;\n                        si += ar * bi - ai * br;",
+                              "sr += ar * br - ai * bi;\n                        si += ar * bi + ai * br;"),
+    "contract_chunk_skipped": ("for (size_t c0 = 0; c0 < K; c0 += kChunk) {",
+                               "for (size_t c0 = kChunk; c0 < K; c0 += kChunk) {"),
     "kpm2d_missing_conj": ("mu[(n - 1) + NC * (m - 1)] += conj(mub[(jn - 1) + K_right * (jm - 1)]);",
                            "mu[(n - 1) + NC * (m - 1)] += mub[(jn - 1) + K_right * (jm - 1)];"),
     "kpm2d_transposed": ("mu[(n - 1) + NC * (m - 1)] += conj(mub[(jn - 1) + K_right * (jm - 1)]);",

@@ -246,3 +246,53 @@ def test_ed_hall_and_broadened_kubo_bastin_pins():
     Ht, Jxt, Jyt, at = models.haldane_model(6, 6, t=1.0, t2=0.2, m=1.6)
     st = oracle.ed_hall_conductivity_T0(Ht, Jxt, Jyt, at, Ef=0.0)
     assert abs(st) < 0.2 and abs(st + 0.11) < 0.02
+
+
+def phases_nr(seed, n, nr):
+    return phases(Xoshiro256pp(seed), n, nr)
+
+
+def test_kpm2d_tangent_matches_table_and_finite_differences():
+    """oracle.kpm_2d_tangent: its table equals the C restatement's kpm_2d and
+    its tangent equals central differences of that kpm_2d along random
+    complex directions of H, Ja and Jb (the checker of the kpm_2d adjoint)."""
+    from kpmhip import models
+    H, Jx, Jy, area = models.haldane_model(4, 4, t2=0.2)
+    Hn = (H / 4.4).tocsc()
+    NC, NR = 16, 3
+    pl = phases_nr(1, H.shape[0], NR)
+    pr = phases_nr(2, H.shape[0], NR)
+    rng = Xoshiro256pp(9)
+
+    def rnd(A):
+        D = sp.csc_matrix(A, dtype=complex).copy()
+        D.data = (rng.rand(D.nnz) - 0.5) + 1j * (rng.rand(D.nnz) - 0.5)
+        return D
+    dH, dJa, dJb = rnd(Hn), rnd(Jx), rnd(Jy)
+    mu, dmu = oracle.kpm_2d_tangent(Hn, Jx, Jy, dH, dJa, dJb, NC, pl, pr)
+    np.testing.assert_allclose(mu, oracle.kpm_2d(Hn, Jx, Jy, NC, pl, pr, NC, NC), rtol=0, atol=1e-13)
+    h = 1e-6
+    fd = (oracle.kpm_2d(Hn + h * dH, Jx + h * dJa, Jy + h * dJb, NC, pl, pr, NC, NC) -
+          oracle.kpm_2d(Hn - h * dH, Jx - h * dJa, Jy - h * dJb, NC, pl, pr, NC, NC)) / (2 * h)
+    assert np.max(np.abs(fd - dmu)) <= 1e-8 * np.max(np.abs(dmu))
+    # each operator's part alone (a dropped term would show here)
+    for d in ((dH, 0 * dJa, 0 * dJb), (0 * dH, dJa, 0 * dJb), (0 * dH, 0 * dJa, dJb)):
+        _, t = oracle.kpm_2d_tangent(Hn, Jx, Jy, *d, NC, pl, pr)
+        f = (oracle.kpm_2d(Hn + h * d[0], Jx + h * d[1], Jy + h * d[2], NC, pl, pr, NC, NC) -
+             oracle.kpm_2d(Hn - h * d[0], Jx - h * d[1], Jy - h * d[2], NC, pl, pr, NC, NC)) / (2 * h)
+        assert np.max(np.abs(f - t)) <= 1e-8 * max(np.max(np.abs(t)), 1e-12)
+
+
+@pytest.mark.parametrize("Ef,beta", [(0.0, np.inf), (1.3, np.inf), (-0.7, 40.0)])
+def test_kubo_bastin_weights_reproduce_the_head(Ef, beta):
+    """sigma(mu) = Re sum K * mu with K = kpmhip.kubo_bastin_weights, against
+    the oracle's restatement of kubo_bastin_cond on random complex tables."""
+    import kpmhip as K
+    NC, a, NH, area = 40, 4.2, 72, 37.5
+    rng = Xoshiro256pp(31)
+    W = K.kubo_bastin_weights(NC, a, Ef, b=0.1, NH=NH, area=area, beta=beta)
+    for _ in range(3):
+        mu = (rng.rand(NC, NC) - 0.5) + 1j * (rng.rand(NC, NC) - 0.5)
+        ref = oracle.kubo_bastin_cond(mu, a, Ef, b=0.1, NH=NH, area=area, beta=beta)
+        got = float(np.real(np.sum(W * mu)))
+        assert abs(got - ref) <= 1e-12 * max(1.0, abs(ref))
