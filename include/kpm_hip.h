@@ -196,6 +196,18 @@ int kpm_1d_block(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, const kpm_block* ps
                  double* mu_all);
 int kpm_1d_lr_block(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, const kpm_block* psi_l,
                     const kpm_block* psi_r, double* mu_all);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory while they run (code under HIP graph capture, serving loops).
+ * kpm_1d_block with every buffer supplied: psi (read), the two recurrence
+ * workspaces w0, w1 (overwritten; the alpha_all of src/moment.jl:428-429),
+ * same shape and precision as psi, and mu, a device accumulator of at least
+ * NR x NC (kpm_accum_create) that receives the moments.  The dot partials
+ * live in the library's static 8 MiB scratch array on each device, so NRp
+ * may be at most 16,384 (a clear KPM_EINVAL beyond).  Calls that overlap on
+ * one device share that array and race.  Launches only: no allocation, no
+ * host synchronisation, no memset or memcpy. */
+int kpm_1d_block_noalloc(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, const kpm_block* psi,
+                         kpm_block* w0, kpm_block* w1, kpm_accum* mu);
 
 /* kpm_2d_blocking (src/moment.jl:759-810).  budget < 0 means "auto" (50% of
  * free device memory, src/moment.jl:741); arr_size/right_block <= 0 mean :auto. */

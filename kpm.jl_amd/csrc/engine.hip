@@ -70,6 +70,7 @@ struct DotSteps {
   bool has_pend = false;
   int k = 0;
 
+  int max_grid = 0;  // 0: kMaxGridPerCU * CUs
   int init(int nd) {
     const size_t per = (size_t)kMaxGridPerCU * std::max(ctx->num_cus, 1) * nd * NRp;
     KPM_TRY(ensure(ctx, ctx->partials, 2 * per * sizeof(double2)));
@@ -77,16 +78,24 @@ struct DotSteps {
     half[1] = half[0] + per;
     return KPM_OK;
   }
+  // caller-owned halves of max_grid workgroups each (no allocation)
+  void init_external(double2* h0, double2* h1, int grid_cap) {
+    half[0] = h0;
+    half[1] = h1;
+    max_grid = grid_cap;
+  }
+  double2* mu_dev = nullptr;  // null: ctx->mu
   // one fused pass whose moment columns are described by c
   int step(const kpm_block* x, const kpm_block* z, kpm_block* out, const kpm_block* l, double alpha,
            double beta, int mode, const Combine& c) {
     StepLaunch s{H, x, z, out, l, alpha, beta, mode};
     s.part_out = half[k & 1];
+    s.max_grid = max_grid;
     s.pend = has_pend ? &pend : nullptr;
     int G = 0;
     KPM_TRY(launch_step(ctx, s, tuned_config(ctx, H, x->NR, x->prec, mode), &G));
     pend.part = half[k & 1];
-    pend.mu = (double2*)ctx->mu.ptr;
+    pend.mu = mu_dev ? mu_dev : (double2*)ctx->mu.ptr;
     pend.cb = c;
     pend.G = G;
     pend.NRp = (int)NRp;
@@ -98,18 +107,22 @@ struct DotSteps {
   int finish() {
     if (!has_pend) return KPM_OK;
     has_pend = false;
-    return launch_combine_from(ctx, pend.part, pend.G, NR, NRp, pend.cb, ctx->mu.ptr);
+    return launch_combine_from(ctx, pend.part, pend.G, NR, NRp, pend.cb, pend.mu);
   }
 };
 
 // ---- kpm_1d! equal-vector, on a device probe block -------------------------
-// w0 holds psi on entry; w1 is scratch.  mu (device) receives NR x NC.
-static int run_1d_doubling(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, kpm_block* w0,
-                           kpm_block* w1) {
+// w0 holds psi on entry; w1 is scratch.  mu_dev receives NR x NC.
+}  // namespace kpm
+
+int kpm::run_1d_doubling_into(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, kpm_block* w0,
+                              kpm_block* w1, void* mu_dev, double2* part0, double2* part1,
+                              int max_grid) {
   const int64_t NR = w0->NR, NRp = w0->NRp;
-  KPM_TRY(mu_buffer(ctx, NR, NC));
   DotSteps ds{ctx, H, NR, NRp, {nullptr, nullptr}};
-  KPM_TRY(ds.init(2));
+  if (part0 && part1) ds.init_external(part0, part1, max_grid);
+  else KPM_TRY(ds.init(2));
+  ds.mu_dev = (double2*)mu_dev;
   kpm_block* v[2] = {w0, w1};
   // alpha_1 = H alpha_0 with mu0 = <a0|a0>, mu1 = <a0|a1> (:442-449)
   Combine c0{2, {0, 1}, {1.0, 1.0}, {-1, -1}, {0.0, 0.0}};
@@ -123,6 +136,14 @@ static int run_1d_doubling(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, kpm_block
     ipp = 1 - ipp;
   }
   return ds.finish();
+}
+
+namespace kpm {
+
+static int run_1d_doubling(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, kpm_block* w0,
+                           kpm_block* w1) {
+  KPM_TRY(mu_buffer(ctx, w0->NR, NC));
+  return run_1d_doubling_into(ctx, H, NC, w0, w1, ctx->mu.ptr, nullptr, nullptr, 0);
 }
 
 // ---- bra/ket and current: one dot per step against l ------------------------
@@ -316,11 +337,7 @@ static int check_dot_args(const kpm_block* A, const kpm_block* B, int64_t NC, in
 
 }  // namespace kpm
 
-struct kpm_accum {
-  kpm_ctx* ctx = nullptr;
-  int64_t NR = 0, NC = 0;
-  void* d = nullptr;  // NR x (NC + 1) double2, column-major; column NC is scratch
-};
+
 
 extern "C" {
 

@@ -126,7 +126,19 @@ struct kpm_ctx {
   int64_t counters[KPM_COUNT_N] = {};
   // incremented by every entry point that writes a caller's block
   uint64_t epoch = 1;
+  // device address and size of the allocation-free path's scratch array
+  // (noalloc.hip), looked up once per device at context creation
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
   kpm_prim_state prim;
+};
+
+// device moment accumulator (kpm_accum_*): NR x (NC + 1) double2,
+// column-major; column NC is scratch for unused combine entries
+struct kpm_accum {
+  kpm_ctx* ctx = nullptr;
+  int64_t NR = 0, NC = 0;
+  void* d = nullptr;
 };
 
 struct kpm_csr {
@@ -197,6 +209,7 @@ struct StepLaunch {
   int mode;
   double gamma = 0.0;                     // MODE_SRC, MODE_DUAL only
   double2* part_out = nullptr;            // dot partials (null: ctx->partials)
+  int max_grid = 0;                       // with part_out: workgroups it holds (0: kMaxGridPerCU * CUs)
   const PendingCombine* pend = nullptr;   // previous step's combine, run in this launch
   const void* vals2 = nullptr;            // MODE_DUAL: second operator's values (complex)
   kpm_block* out2 = nullptr;              // MODE_DUAL: second output
@@ -280,6 +293,18 @@ int launch_sddmm(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* const* beta,
 // out = alpha*A*x + beta*z (+ gamma*src in MODE_SRC), no dots
 int adj_step(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* x, const kpm_block* z,
              kpm_block* out, const kpm_block* src, double alpha, double beta, double gamma, int mode);
+
+// looks up the scratch array's device address for ctx->device (noalloc.hip)
+int noalloc_scratch_init(kpm_ctx* ctx);
+
+// ---- kpm_1d! doubling engine (engine.hip) --------------------------------------------
+// w0 holds psi on entry (overwritten), w1 is scratch; mu_dev (NR x NC
+// double2, every column written) receives the moments.  part0 / part1: two
+// dot-partial buffers of max_grid * 2 * NRp double2 each (null: the
+// context's own, grown as needed).  Launches only, no allocation or
+// synchronisation when both partial buffers are given.
+int run_1d_doubling_into(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, kpm_block* w0, kpm_block* w1,
+                         void* mu_dev, double2* part0, double2* part1, int max_grid);
 
 // tuner table (tune.hip): best StepConfig for a shape, or defaults; record one
 StepConfig tuned_config(kpm_ctx* ctx, const kpm_csr* A, int64_t NR, int prec, int mode);

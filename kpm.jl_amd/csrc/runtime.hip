@@ -300,6 +300,13 @@ int kpm_ctx_create(int device, kpm_ctx** out) {
     delete ctx;
     return fail(KPM_EHIP, std::string("context creation failed: ") + hipGetErrorString(e));
   }
+  // the allocation-free entry point's scratch address on this device; without
+  // it only that entry point is unavailable
+  if (noalloc_scratch_init(ctx) != KPM_OK) {
+    (void)hipGetLastError();
+    ctx->scratch = nullptr;
+    ctx->scratch_bytes = 0;
+  }
   *out = ctx;
   return KPM_OK;
 }

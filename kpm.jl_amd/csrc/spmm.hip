@@ -177,8 +177,9 @@ int launch_step(kpm_ctx* ctx, const StepLaunch& s, const StepConfig& cfg, int* g
   int G = cfg.grid > 0 ? (cfg.grid + 7) / 8 * 8 : default_grid(ctx, ntiles);
   const int nd = s.mode == MODE_DBL ? 2 : (s.mode == MODE_LR ? 1 : 0);
   double2* part = s.part_out;
-  if (part) {  // caller-owned buffer sized for kMaxGridPerCU * CUs workgroups
-    G = std::min(G, kMaxGridPerCU * std::max(ctx->num_cus, 1) / 8 * 8);
+  if (part) {  // caller-owned buffer sized for max_grid (or kMaxGridPerCU * CUs) workgroups
+    const int cap = s.max_grid > 0 ? s.max_grid : kMaxGridPerCU * std::max(ctx->num_cus, 1);
+    G = std::max(8, std::min(G, cap / 8 * 8));
   } else {
     if (nd > 0) KPM_TRY(ensure(ctx, ctx->partials, (size_t)G * nd * x->NRp * sizeof(double2)));
     part = (double2*)ctx->partials.ptr;

@@ -98,6 +98,7 @@ _SIGS = {
     "kpm_1d_lr": (C.c_int, [_p, _p, _i64, _i64, _i64, _pd, _pd, _pd, C.c_int]),
     "kpm_1d_current": (C.c_int, [_p, _p, _p, _i64, _i64, _i64, _pd, _pd, C.c_int]),
     "kpm_1d_block": (C.c_int, [_p, _p, _i64, _p, _pd]),
+    "kpm_1d_block_noalloc": (C.c_int, [_p, _p, _i64, _p, _p, _p, _p]),
     "kpm_1d_lr_block": (C.c_int, [_p, _p, _i64, _p, _p, _pd]),
     "kpm_2d_blocking": (C.c_int, [_p, _i64, _i64, _i64, _i64, _i64, _i64, _pi64, _pi64, _pi64]),
     "kpm_2d": (C.c_int, [_p, _p, _p, _p, _i64, _i64, _i64, _pd, _pd, _i64, _i64, C.c_int,
