@@ -411,13 +411,24 @@ int kpm_event_elapsed(kpm_ctx* ctx, int slot_a, int slot_b, float* ms);
 
 /* ---- launch-shape tuner ---------------------------------------------------
  * No counterpart in the reference (it has no tuning knob).  Times every
- * candidate geometry of the fused pass (rows in flight per lane x workgroups
- * per CU) on the caller's operator and blocks X, Z (Z is overwritten),
- * records the fastest for this shape and returns it. */
+ * candidate geometry of the fused pass (rows in flight per lane x either a
+ * persistent grid of 2..16 workgroups per CU or a one-shot grid of 2..64
+ * row tiles per workgroup) on the caller's operator and blocks X, Z (Z is
+ * overwritten), records the fastest for this shape and returns it
+ * (best_grid: the workgroups it launches). */
 int kpm_autotune_step(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z, int reps,
                       int* best_unroll, int* best_grid, float* best_ms_per_step);
 /* Set the table entry for a shape directly (unroll 1 or 2; grid 0 = auto). */
 int kpm_tune_set(kpm_ctx* ctx, const kpm_csr* H, int64_t NR, int vec_prec, int unroll, int grid);
+/* Set a one-shot schedule for a shape: one workgroup per `chunk` consecutive
+ * row tiles, chunks in dispatch order (xcd_grouped = 0) or each XCD sweeping
+ * its own contiguous eighth (xcd_grouped = 1).  The engines fall back to the
+ * persistent schedule when the grid would exceed their partials buffers. */
+int kpm_tune_set_chunked(kpm_ctx* ctx, const kpm_csr* H, int64_t NR, int vec_prec, int unroll,
+                         int chunk, int xcd_grouped);
+/* The launch shape the engines use for a shape (chunk 0: persistent grid). */
+int kpm_tune_get(kpm_ctx* ctx, const kpm_csr* H, int64_t NR, int vec_prec, int* unroll, int* grid,
+                 int* chunk, int* xcd_grouped);
 
 #ifdef __cplusplus
 }

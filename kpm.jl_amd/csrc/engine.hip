@@ -66,50 +66,130 @@ struct DotSteps {
   const kpm_csr* H;
   int64_t NR, NRp;
   double2* half[2];
-  PendingCombine pend;
-  bool has_pend = false;
   int k = 0;
+  int max_grid = 0;          // workgroups each half holds (0: kMaxGridPerCU * CUs)
+  double2* l2buf[2] = {nullptr, nullptr};  // slice sums of the two-level reduction
+  double2* mu_dev = nullptr;  // null: ctx->mu
+  // reductions still owed: a final one (into the moment columns) and a
+  // first level (partials -> slice sums) whose final follows a launch later
+  struct Fin {
+    bool on = false;
+    const double2* part = nullptr;
+    int G = 0;
+    Combine cb{0, {0, 0}, {0.0, 0.0}, {-1, -1}, {0.0, 0.0}};
+  } fin;
+  struct L1 {
+    bool on = false;
+    const double2* part = nullptr;
+    double2* out = nullptr;
+    int G = 0;
+    Combine cb{0, {0, 0}, {0.0, 0.0}, {-1, -1}, {0.0, 0.0}};
+  } l1;
 
-  int max_grid = 0;  // 0: kMaxGridPerCU * CUs
-  int init(int nd) {
-    const size_t per = (size_t)kMaxGridPerCU * std::max(ctx->num_cus, 1) * nd * NRp;
-    KPM_TRY(ensure(ctx, ctx->partials, 2 * per * sizeof(double2)));
+  const StepConfig* cfg = nullptr;  // launch shape (null: the tuned one)
+
+  double2* mu() const { return mu_dev ? mu_dev : (double2*)ctx->mu.ptr; }
+  // halves (and slice-sum buffers) in ctx->partials, sized for the grid of
+  // the first block's shape
+  int init(int nd, const kpm_block* x, int mode) {
+    max_grid = cfg ? partials_grid_cfg(ctx, H, x->NH, x->NR, x->prec, *cfg)
+                   : partials_grid(ctx, H, x->NH, x->NR, x->prec, mode);
+    const size_t per = (size_t)max_grid * nd * NRp;
+    const size_t l2per = (size_t)((max_grid + kSlice - 1) / kSlice) * nd * NRp;
+    KPM_TRY(ensure(ctx, ctx->partials, (2 * per + 2 * l2per) * sizeof(double2)));
     half[0] = (double2*)ctx->partials.ptr;
     half[1] = half[0] + per;
+    l2buf[0] = half[1] + per;
+    l2buf[1] = l2buf[0] + l2per;
     return KPM_OK;
   }
-  // caller-owned halves of max_grid workgroups each (no allocation)
+  // caller-owned halves of max_grid (<= kTwoLevelMin) workgroups each
   void init_external(double2* h0, double2* h1, int grid_cap) {
     half[0] = h0;
     half[1] = h1;
-    max_grid = grid_cap;
+    max_grid = std::min(grid_cap, kTwoLevelMin);
   }
-  double2* mu_dev = nullptr;  // null: ctx->mu
   // one fused pass whose moment columns are described by c
   int step(const kpm_block* x, const kpm_block* z, kpm_block* out, const kpm_block* l, double alpha,
            double beta, int mode, const Combine& c) {
     StepLaunch s{H, x, z, out, l, alpha, beta, mode};
     s.part_out = half[k & 1];
     s.max_grid = max_grid;
-    s.pend = has_pend ? &pend : nullptr;
+    PendingCombine pc;
+    pc.mu = mu();
+    pc.NRp = (int)NRp;
+    pc.NR = (int)NR;
+    if (fin.on) {
+      pc.part = fin.part;
+      pc.G = fin.G;
+      pc.cb = fin.cb;
+    }
+    if (l1.on) {
+      pc.l1_part = l1.part;
+      pc.l1_out = l1.out;
+      pc.l1_G = l1.G;
+      pc.l1_nd = l1.cb.nd;
+    }
+    s.pend = (fin.on || l1.on) ? &pc : nullptr;
     int G = 0;
-    KPM_TRY(launch_step(ctx, s, tuned_config(ctx, H, x->NR, x->prec, mode), &G));
-    pend.part = half[k & 1];
-    pend.mu = mu_dev ? mu_dev : (double2*)ctx->mu.ptr;
-    pend.cb = c;
-    pend.G = G;
-    pend.NRp = (int)NRp;
-    pend.NR = (int)NR;
-    has_pend = true;
+    KPM_TRY(launch_step(ctx, s, cfg ? *cfg : tuned_config(ctx, H, x->NR, x->prec, mode), &G));
+    fin.on = false;
+    if (l1.on) {  // its first level ran in this launch; the final is next
+      fin.on = true;
+      fin.part = l1.out;
+      fin.G = (l1.G + kSlice - 1) / kSlice;
+      fin.cb = l1.cb;
+      l1.on = false;
+    }
+    if (G > kTwoLevelMin && l2buf[0]) {
+      l1.on = true;
+      l1.part = half[k & 1];
+      l1.out = l2buf[k & 1];
+      l1.G = G;
+      l1.cb = c;
+    } else {
+      if (fin.on) {  // both would be final next launch: finish the older now
+        KPM_TRY(launch_combine_from(ctx, fin.part, fin.G, NR, NRp, fin.cb, mu()));
+      }
+      fin.on = true;
+      fin.part = half[k & 1];
+      fin.G = G;
+      fin.cb = c;
+    }
     ++k;
     return KPM_OK;
   }
+  // the owed reductions as launches of their own, older step first
   int finish() {
-    if (!has_pend) return KPM_OK;
-    has_pend = false;
-    return launch_combine_from(ctx, pend.part, pend.G, NR, NRp, pend.cb, pend.mu);
+    if (fin.on) {
+      fin.on = false;
+      KPM_TRY(launch_combine_from(ctx, fin.part, fin.G, NR, NRp, fin.cb, mu()));
+    }
+    if (l1.on) {
+      l1.on = false;
+      KPM_TRY(launch_reduce1(ctx, l1.part, l1.G, l1.cb.nd, NR, NRp, l1.out));
+      KPM_TRY(launch_combine_from(ctx, l1.out, (l1.G + kSlice - 1) / kSlice, NR, NRp, l1.cb, mu()));
+    }
+    return KPM_OK;
   }
 };
+
+int time_engine_steps(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z,
+                      const StepConfig& cfg, int reps, float* ms) {
+  KPM_TRY(ensure(ctx, ctx->mu, (size_t)(X->NR * 2) * sizeof(double2)));
+  DotSteps ds{ctx, H, X->NR, X->NRp, {nullptr, nullptr}};
+  ds.cfg = &cfg;
+  KPM_TRY(ds.init(2, X, MODE_DBL));
+  const Combine c{2, {0, 1}, {2.0, 2.0}, {-1, -1}, {0.0, 0.0}};
+  for (int r = 0; r < 2; ++r) KPM_TRY(ds.step(X, Z, Z, nullptr, 2.0, -1.0, MODE_DBL, c));
+  KPM_HIP_TRY(hipEventRecord(ctx->events[12], ctx->stream));
+  for (int r = 0; r < reps; ++r) KPM_TRY(ds.step(X, Z, Z, nullptr, 2.0, -1.0, MODE_DBL, c));
+  KPM_TRY(ds.finish());
+  KPM_HIP_TRY(hipEventRecord(ctx->events[13], ctx->stream));
+  KPM_HIP_TRY(hipEventSynchronize(ctx->events[13]));
+  KPM_HIP_TRY(hipEventElapsedTime(ms, ctx->events[12], ctx->events[13]));
+  return KPM_OK;
+}
 
 // ---- kpm_1d! equal-vector, on a device probe block -------------------------
 // w0 holds psi on entry; w1 is scratch.  mu_dev receives NR x NC.
@@ -121,7 +201,7 @@ int kpm::run_1d_doubling_into(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, kpm_bl
   const int64_t NR = w0->NR, NRp = w0->NRp;
   DotSteps ds{ctx, H, NR, NRp, {nullptr, nullptr}};
   if (part0 && part1) ds.init_external(part0, part1, max_grid);
-  else KPM_TRY(ds.init(2));
+  else KPM_TRY(ds.init(2, w0, MODE_DBL));
   ds.mu_dev = (double2*)mu_dev;
   kpm_block* v[2] = {w0, w1};
   // alpha_1 = H alpha_0 with mu0 = <a0|a0>, mu1 = <a0|a1> (:442-449)
@@ -156,7 +236,7 @@ static int run_1d_lr(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, const kpm_block
   Combine c0{1, {0, 0}, {1.0, 0.0}, {-1, -1}, {0.0, 0.0}};
   KPM_TRY(launch_combine(ctx, G, NR, NRp, c0, ctx->mu.ptr, NC));
   DotSteps ds{ctx, H, NR, NRp, {nullptr, nullptr}};
-  KPM_TRY(ds.init(1));
+  KPM_TRY(ds.init(1, w0, MODE_LR));
   kpm_block* v[2] = {w0, w1};
   Combine c1{1, {1, 0}, {1.0, 0.0}, {-1, -1}, {0.0, 0.0}};
   KPM_TRY(ds.step(v[0], nullptr, v[1], l, 1.0, 0.0, MODE_LR, c1));  // a1 = H a0, mu1
@@ -280,8 +360,8 @@ static int prim_flush(kpm_ctx* ctx) {
 static int prim_step(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* x, const kpm_block* z,
                      kpm_block* out, double alpha, double beta) {
   kpm_prim_state& P = ctx->prim;
-  const size_t half_bytes =
-      (size_t)kMaxGridPerCU * std::max(ctx->num_cus, 1) * 2 * x->NRp * sizeof(double2);
+  const int cap = partials_grid(ctx, A, x->NH, x->NR, x->prec, MODE_DBL);
+  const size_t half_bytes = (size_t)cap * 2 * x->NRp * sizeof(double2);
   if (P.part.bytes < 2 * half_bytes) {  // growing frees the halves the pending combine reads
     KPM_TRY(prim_flush(ctx));
     P.cached = false;
@@ -291,6 +371,7 @@ static int prim_step(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* x, const k
   double2* part = (double2*)((char*)P.part.ptr + h * (P.part.bytes / 2));
   StepLaunch s{A, x, z, out, nullptr, alpha, beta, MODE_DBL};
   s.part_out = part;
+  s.max_grid = (int)(P.part.bytes / 2 / (2 * x->NRp * sizeof(double2)));
   PendingCombine pc;
   if (P.pending) {
     pc.part = (const double2*)P.pend_part;

@@ -226,7 +226,9 @@ struct SplitOut {
 // Launch geometry picked by the tuner for (LPR, rows per lane, grid size).
 struct StepConfig {
   int unroll = 2;     // rows per lane per tile
-  int grid = 0;       // workgroups (multiple of 8); 0 = auto
+  int grid = 0;       // persistent workgroups (multiple of 8); 0 = auto
+  int chunk = 0;      // > 0: one-shot schedule, this many tiles per workgroup
+  int xmap = 0;       // one-shot schedule: 1 = chunks grouped per XCD
 };
 int launch_step(kpm_ctx* ctx, const StepLaunch& s, const StepConfig& cfg, int* grid_used);
 // geometry of the fused pass for a block shape: lanes per row, row-chunks,
@@ -257,13 +259,38 @@ int launch_combine_from(kpm_ctx* ctx, const double2* partials, int grid, int64_t
 // previous launch, complete at the kernel boundary) before their own tiles,
 // which saves one launch per step.  Dot steps then alternate between two
 // partials buffers of at most kMaxGridPerCU * CUs workgroups each.
+// Large one-shot grids reduce in two levels: the launch after a step sums
+// its partials in slices of kSlice workgroups (first level, l1_*), and the
+// launch after that reduces the slice sums into the moment columns (part,
+// G = slices).  Both levels add in a fixed order.
+constexpr int kSlice = 1024;
+constexpr int kTwoLevelMin = 4096;  // grids above this reduce in two levels
 struct PendingCombine {
   const double2* part = nullptr;
   double2* mu = nullptr;
   Combine cb{0, {0, 0}, {0.0, 0.0}, {-1, -1}, {0.0, 0.0}};
   int G = 0, NRp = 0, NR = 0;
+  const double2* l1_part = nullptr;  // first level: partials of l1_G workgroups
+  double2* l1_out = nullptr;         // -> slice sums [slice][d][col]
+  int l1_G = 0, l1_nd = 0;
 };
+// first-level reduction as a launch of its own (the engines' last steps)
+int launch_reduce1(kpm_ctx* ctx, const double2* partials, int G, int nd, int64_t NR, int64_t NRp,
+                   double2* out);
 constexpr int kMaxGridPerCU = 16;
+// one-shot (chunked) grids may use up to this many workgroups per CU; the
+// engines size their partials buffers for the tuned grid of the shape
+constexpr int kMaxChunkGridPerCU = 128;
+// workgroups the partials buffers of the engines must hold for a shape:
+// kMaxGridPerCU * CUs, or the tuned one-shot grid when it is larger
+int partials_grid(kpm_ctx* ctx, const kpm_csr* A, int64_t NH, int64_t NR, int prec, int mode);
+int partials_grid_cfg(kpm_ctx* ctx, const kpm_csr* A, int64_t NH, int64_t NR, int prec,
+                      const StepConfig& cfg);
+// ms of `reps` doubling steps of the kpm_1d engine (fused passes and their
+// deferred reductions) under launch shape cfg on blocks X (v_n) and Z
+// (v_{n-1}, overwritten), after two untimed steps: what the tuner compares
+int time_engine_steps(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z,
+                      const StepConfig& cfg, int reps, float* ms);
 // layout conversions between the host column-major ComplexF64 block (in the
 // staging buffer, already on the device) and a device recurrence block
 int launch_upload(kpm_ctx* ctx, const void* staging, kpm_block* b);

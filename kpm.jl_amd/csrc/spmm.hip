@@ -68,7 +68,23 @@ __global__ void __launch_bounds__(kBlock)
     k_combine(const double2* __restrict__ partials, int G, int nd, int NRp, int NR,
               double2* __restrict__ mu, Combine cb) {
   __shared__ double2 sm[kBlock];
-  combine_pair(partials, G, nd, NRp, NR, mu, cb, blockIdx.x / NR, blockIdx.x % NR, sm);
+  combine_unit(partials, G, nd, NRp, NR, mu, cb, blockIdx.x, sm);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_reduce1(const double2* __restrict__ partials, int G, int nd, int NRp, int NR,
+              double2* __restrict__ out) {
+  __shared__ double2 sm[kBlock];
+  reduce1_unit(partials, G, nd, NRp, NR, out, blockIdx.x, sm);
+}
+
+int launch_reduce1(kpm_ctx* ctx, const double2* partials, int G, int nd, int64_t NR, int64_t NRp,
+                   double2* out) {
+  hipLaunchKernelGGL(k_reduce1, dim3((unsigned)reduce1_units(nd, (int)NR, G)), dim3(kBlock), 0,
+                     ctx->stream, partials, G, nd, (int)NRp, (int)NR, out);
+  KPM_HIP_TRY(hipGetLastError());
+  ++ctx->counters[KPM_COUNT_COMBINE];
+  return KPM_OK;
 }
 
 // host column-major ComplexF64 (NH x NR) -> row-interleaved block (NH x NRp)
@@ -168,6 +184,21 @@ static StepLauncher pick(int prec, const StepGeom& g, bool realh, int mode) {
                              : pick_step_f64(g.lpr, g.unr, g.kb, realh, mode);
 }
 
+int partials_grid(kpm_ctx* ctx, const kpm_csr* A, int64_t NH, int64_t NR, int prec, int mode) {
+  return partials_grid_cfg(ctx, A, NH, NR, prec, tuned_config(ctx, A, NR, prec, mode));
+}
+
+int partials_grid_cfg(kpm_ctx* ctx, const kpm_csr* A, int64_t NH, int64_t NR, int prec,
+                      const StepConfig& cfg) {
+  const int64_t base = (int64_t)kMaxGridPerCU * std::max(ctx->num_cus, 1);
+  if (cfg.chunk <= 0) return (int)base;
+  const StepGeom g = step_geom(A, NR, prec, cfg);
+  const int64_t ntiles = (NH + g.TR - 1) / g.TR;
+  const int64_t gc = ((ntiles + cfg.chunk - 1) / cfg.chunk + 7) / 8 * 8;
+  const int64_t lim = (int64_t)kMaxChunkGridPerCU * std::max(ctx->num_cus, 1);
+  return (int)std::max(base, std::min(gc, lim));
+}
+
 int launch_step(kpm_ctx* ctx, const StepLaunch& s, const StepConfig& cfg, int* grid_used) {
   const kpm_block* x = s.x;
   const int prec = x->prec;
@@ -177,8 +208,18 @@ int launch_step(kpm_ctx* ctx, const StepLaunch& s, const StepConfig& cfg, int* g
   int G = cfg.grid > 0 ? (cfg.grid + 7) / 8 * 8 : default_grid(ctx, ntiles);
   const int nd = s.mode == MODE_DBL ? 2 : (s.mode == MODE_LR ? 1 : 0);
   double2* part = s.part_out;
-  if (part) {  // caller-owned buffer sized for max_grid (or kMaxGridPerCU * CUs) workgroups
-    const int cap = s.max_grid > 0 ? s.max_grid : kMaxGridPerCU * std::max(ctx->num_cus, 1);
+  const int cap = s.max_grid > 0 ? s.max_grid : kMaxGridPerCU * std::max(ctx->num_cus, 1);
+  // one-shot schedule: one workgroup per chunk of tiles (a multiple of 8
+  // workgroups), unless the caller's partials buffer cannot hold them
+  int chunk = 0;
+  if (cfg.chunk > 0) {
+    const int64_t gc = ((ntiles + cfg.chunk - 1) / cfg.chunk + 7) / 8 * 8;
+    if (gc <= INT32_MAX && (!part || gc <= cap)) {
+      G = (int)gc;
+      chunk = cfg.chunk;
+    }
+  }
+  if (part && chunk == 0) {  // caller-owned buffer sized for `cap` workgroups
     G = std::max(8, std::min(G, cap / 8 * 8));
   } else {
     if (nd > 0) KPM_TRY(ensure(ctx, ctx->partials, (size_t)G * nd * x->NRp * sizeof(double2)));
@@ -201,7 +242,7 @@ int launch_step(kpm_ctx* ctx, const StepLaunch& s, const StepConfig& cfg, int* g
   fn(dim3(G, g.chunks), ctx->stream, s.A->rowptr, s.A->colind, s.A->vals, x->d, s.z ? s.z->d : x->d,
      s.out->d, s.l ? s.l->d : nullptr, part, (int)x->NH, g.RW, (int)x->NRp,
      (int)ntiles, s.alpha, s.beta, g.has_tail, nullptr, nullptr, nullptr, s.gamma, pc, s.vals2,
-     s.out2 ? s.out2->d : nullptr, o2);
+     s.out2 ? s.out2->d : nullptr, o2, chunk, chunk > 0 ? cfg.xmap : 0);
   KPM_HIP_TRY(hipGetLastError());
   ++ctx->counters[KPM_COUNT_STEP];
   if (s.mode == MODE_DUAL) ++ctx->counters[KPM_COUNT_DUAL];
@@ -215,7 +256,7 @@ int launch_step_seg(kpm_ctx* ctx, const StepLaunch& s, const StepGeom& g, int G,
   StepLauncher fn = pick(x->prec, g, s.A->is_real != 0, MODE_DBL_SEG);
   fn(dim3(G, g.chunks), ctx->stream, s.A->rowptr, s.A->colind, s.A->vals, x->d, s.z ? s.z->d : x->d,
      s.out->d, nullptr, partials, (int)x->NH, g.RW, (int)x->NRp, (int)ntiles, s.alpha, s.beta,
-     g.has_tail, tile_lo, treal, seg0, 0.0, PendingCombine{}, nullptr, nullptr, SplitOut{});
+     g.has_tail, tile_lo, treal, seg0, 0.0, PendingCombine{}, nullptr, nullptr, SplitOut{}, 0, 0);
   KPM_HIP_TRY(hipGetLastError());
   ++ctx->counters[KPM_COUNT_STEP];
   return KPM_OK;
@@ -269,7 +310,7 @@ int launch_combine(kpm_ctx* ctx, int grid, int64_t NR, int64_t NRp, const Combin
 
 int launch_combine_from(kpm_ctx* ctx, const double2* partials, int grid, int64_t NR, int64_t NRp,
                         const Combine& c, void* mu_dev) {
-  hipLaunchKernelGGL(k_combine, dim3((unsigned)(c.nd * NR)), dim3(kBlock), 0, ctx->stream, partials,
+  hipLaunchKernelGGL(k_combine, dim3((unsigned)combine_units(c.nd, (int)NR)), dim3(kBlock), 0, ctx->stream, partials,
                      grid, c.nd, (int)NRp, (int)NR, (double2*)mu_dev, c);
   KPM_HIP_TRY(hipGetLastError());
   ++ctx->counters[KPM_COUNT_COMBINE];

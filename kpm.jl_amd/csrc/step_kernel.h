@@ -36,8 +36,10 @@
 //   already loads the column indices of tile t+1 and the row pointers of
 //   tile t+2, so a tile costs one memory latency instead of three dependent
 //   ones (row pointer -> column index -> gather).
-//   Arithmetic is FP64 for both storage precisions; FP32 blocks are rounded
-//   on store and the dots use the stored (rounded) values.
+//   Arithmetic follows the storage precision: the sparse product and the
+//   combination with z run in FP64 for FP64 blocks and in FP32 for FP32
+//   blocks (operator values rounded to FP32 as they are loaded); the moment
+//   dots always convert the stored values to FP64 and accumulate in FP64.
 #pragma once
 
 #include <type_traits>
@@ -49,10 +51,18 @@ namespace kpm {
 template <int PREC>
 struct Word;
 
+// A: the arithmetic type of the recurrence (the sparse product and the
+// combination with z).  The dots always take the stored values to FP64.
 template <>
 struct Word<KPM_VEC_F64> {
   using T = double2;
+  using A = double;
   static constexpr int CPL = 1;
+  __device__ static inline void unpack_a(const T& w, A* re, A* im) {
+    re[0] = w.x;
+    im[0] = w.y;
+  }
+  __device__ static inline T pack_a(const A* re, const A* im) { return make_double2(re[0], im[0]); }
   __device__ static inline void unpack(const T& w, double* re, double* im) {
     re[0] = w.x;
     im[0] = w.y;
@@ -66,7 +76,17 @@ struct Word<KPM_VEC_F64> {
 template <>
 struct Word<KPM_VEC_F32> {
   using T = float4;
+  using A = float;
   static constexpr int CPL = 2;
+  __device__ static inline void unpack_a(const T& w, A* re, A* im) {
+    re[0] = w.x;
+    im[0] = w.y;
+    re[1] = w.z;
+    im[1] = w.w;
+  }
+  __device__ static inline T pack_a(const A* re, const A* im) {
+    return make_float4(re[0], im[0], re[1], im[1]);
+  }
   __device__ static inline void unpack(const T& w, double* re, double* im) {
     re[0] = (double)w.x;
     im[0] = (double)w.y;
@@ -110,6 +130,10 @@ __device__ __forceinline__ int group_bcast(int v) {
   } else {
     return __builtin_amdgcn_update_dpp(0, v, 0x150 + J, 0xF, 0xF, false);
   }
+}
+template <int G, int J>
+__device__ __forceinline__ float group_bcast(float v) {
+  return __int_as_float(group_bcast<G, J>(__float_as_int(v)));
 }
 template <int G, int J>
 __device__ __forceinline__ double group_bcast(double v) {
@@ -160,31 +184,60 @@ __device__ inline void st_stream(float4* p, const float4& v) {
 #endif
 }
 
-// Fixed-order reduction of one (dot d, probe column col) moment over G
-// per-workgroup partials, then columnwise_dot!'s alpha * dot + beta_scale *
-// target[:, beta_col] (src/utils/vectors.jl:103-117).  sm: kBlock double2 of
-// LDS; every thread of the workgroup must call it.
-__device__ inline void combine_pair(const double2* __restrict__ partials, int G, int nd, int NRp,
-                                    int NR, double2* __restrict__ mu, const Combine& cb, int d,
-                                    int col, double2* sm) {
+// Fixed-order reduction of the moments of one dot d and a group of kCombCols
+// consecutive probe columns over G per-workgroup partials stored [g][d][col],
+// then columnwise_dot!'s alpha * dot + beta_scale * target[:, beta_col]
+// (src/utils/vectors.jl:103-117).  Thread t takes column c0 + t % kCombCols
+// and the partials of workgroups t / kCombCols, + kCombRows, ... in that
+// order (loads issued 16 at a time), so the kCombCols columns of one
+// workgroup's row, one 128-byte line, are read together by one unit; a
+// fixed tree over the kCombRows lanes of each column finishes the sum.
+// sm: kBlock double2 of LDS; every thread of the workgroup must call it.
+constexpr int kCombCols = 8;
+constexpr int kCombRows = kBlock / kCombCols;
+__host__ __device__ inline int combine_units(int nd, int NR) {
+  return nd * ((NR + kCombCols - 1) / kCombCols);
+}
+__device__ inline void combine_unit(const double2* __restrict__ partials, int G, int nd, int NRp,
+                                    int NR, double2* __restrict__ mu, const Combine& cb, int unit,
+                                    double2* sm) {
+  const int ngrp = (NR + kCombCols - 1) / kCombCols;
+  const int d = unit / ngrp;
+  const int col = (unit % ngrp) * kCombCols + (int)threadIdx.x % kCombCols;
+  const int lane = (int)threadIdx.x / kCombCols;
+  const bool live = col < NR;
+  const size_t gs = (size_t)nd * NRp;  // between consecutive workgroups
+  const double2* __restrict__ p = partials + (size_t)d * NRp + (live ? col : 0);
   double sr = 0.0, si = 0.0;
-  for (int g = threadIdx.x; g < G; g += kBlock) {
-    const double2 v = partials[((size_t)g * nd + d) * NRp + col];
+  constexpr int kBatch = 16;
+  int g = lane;
+  for (; g + (kBatch - 1) * kCombRows < G; g += kBatch * kCombRows) {
+    double2 v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) v[u] = p[(size_t)(g + u * kCombRows) * gs];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      sr += v[u].x;
+      si += v[u].y;
+    }
+  }
+  for (; g < G; g += kCombRows) {
+    const double2 v = p[(size_t)g * gs];
     sr += v.x;
     si += v.y;
   }
   sm[threadIdx.x] = make_double2(sr, si);
   __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      const double2 o = sm[threadIdx.x + w];
+  for (int w = kCombRows / 2; w > 0; w >>= 1) {  // lanes of a column are kCombCols apart
+    if (lane < w) {
+      const double2 o = sm[threadIdx.x + w * kCombCols];
       sm[threadIdx.x].x += o.x;
       sm[threadIdx.x].y += o.y;
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    const double2 tot = sm[0];
+  if (lane == 0 && live) {
+    const double2 tot = sm[threadIdx.x];
     double rr = cb.alpha[d] * tot.x;
     double ri = cb.alpha[d] * tot.y;
     if (cb.bcol[d] >= 0) {
@@ -194,6 +247,58 @@ __device__ inline void combine_pair(const double2* __restrict__ partials, int G,
     }
     mu[col + (size_t)NR * cb.tcol[d]] = make_double2(rr, ri);
   }
+  __syncthreads();
+}
+
+// First level of the two-level reduction: one unit sums, for one dot d and
+// kCombCols consecutive columns, the partials of the kSlice workgroups of one
+// slice (each lane its kSlice / kCombRows workgroups in order, then a fixed
+// tree) and writes the slice sum to out[slice][d][col].
+__host__ __device__ inline int reduce1_units(int nd, int NR, int G) {
+  return combine_units(nd, NR) * ((G + kSlice - 1) / kSlice);
+}
+__device__ inline void reduce1_unit(const double2* __restrict__ partials, int G, int nd, int NRp,
+                                    int NR, double2* __restrict__ out, int unit, double2* sm) {
+  const int nu = combine_units(nd, NR);
+  const int slice = unit / nu;
+  const int cu = unit % nu;
+  const int ngrp = (NR + kCombCols - 1) / kCombCols;
+  const int d = cu / ngrp;
+  const int col = (cu % ngrp) * kCombCols + (int)threadIdx.x % kCombCols;
+  const int lane = (int)threadIdx.x / kCombCols;
+  const bool live = col < NR;
+  const size_t gs = (size_t)nd * NRp;
+  const double2* __restrict__ p = partials + (size_t)d * NRp + (live ? col : 0);
+  const int g1 = min(G, (slice + 1) * kSlice);
+  double sr = 0.0, si = 0.0;
+  constexpr int kBatch = 8;
+  int g = slice * kSlice + lane;
+  for (; g + (kBatch - 1) * kCombRows < g1; g += kBatch * kCombRows) {
+    double2 v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) v[u] = p[(size_t)(g + u * kCombRows) * gs];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      sr += v[u].x;
+      si += v[u].y;
+    }
+  }
+  for (; g < g1; g += kCombRows) {
+    const double2 v = p[(size_t)g * gs];
+    sr += v.x;
+    si += v.y;
+  }
+  sm[threadIdx.x] = make_double2(sr, si);
+  __syncthreads();
+  for (int w = kCombRows / 2; w > 0; w >>= 1) {
+    if (lane < w) {
+      const double2 o = sm[threadIdx.x + w * kCombCols];
+      sm[threadIdx.x].x += o.x;
+      sm[threadIdx.x].y += o.y;
+    }
+    __syncthreads();
+  }
+  if (lane == 0 && live) out[(size_t)slice * gs + (size_t)d * NRp + col] = sm[threadIdx.x];
   __syncthreads();
 }
 
@@ -211,11 +316,11 @@ struct ColMeta {
 // KB same-address loads per lane); group_bcast hands slot jj to every lane of
 // the row.  Rows wider than 16 lanes load the same nonzeros in each DPP row.
 // DUAL adds the values of a second operator on the same pattern.
-template <int UNR, int E, bool DUAL = false>
+template <int UNR, int E, bool DUAL = false, class AT = double>
 struct LaneMeta {
   int c[UNR][E];
-  double hr[UNR][E], hi[UNR][E];
-  double gr[DUAL ? UNR : 1][DUAL ? E : 1], gi[DUAL ? UNR : 1][DUAL ? E : 1];
+  AT hr[UNR][E], hi[UNR][E];
+  AT gr[DUAL ? UNR : 1][DUAL ? E : 1], gi[DUAL ? UNR : 1][DUAL ? E : 1];
 };
 
 // Segmented schedule (SEG = true, batched realisations): workgroup b owns the
@@ -244,7 +349,7 @@ __global__ void __launch_bounds__(kBlock)
            const int* __restrict__ tile_lo, const int* __restrict__ treal,
            const int* __restrict__ seg0, double gamma, PendingCombine pc,
            const void* __restrict__ vals2_, typename Word<PREC>::T* __restrict__ out2,
-           SplitOut o2) {
+           SplitOut o2, int chunk, int xmap) {
   using WT = Word<PREC>;
   using T = typename WT::T;
   constexpr int CPL = WT::CPL;
@@ -258,7 +363,8 @@ __global__ void __launch_bounds__(kBlock)
   constexpr int ME = kLaneMeta ? (KB + MG - 1) / MG : 1;
   constexpr bool kDual = MODE == MODE_DUAL;
   static_assert(!kDual || (kLaneMeta && !REALH && !SEG), "dual pass: complex values, spread metadata");
-  using CMeta = std::conditional_t<kLaneMeta, LaneMeta<UNR, ME, kDual>, ColMeta<UNR, KB>>;
+  using AT = typename WT::A;
+  using CMeta = std::conditional_t<kLaneMeta, LaneMeta<UNR, ME, kDual, AT>, ColMeta<UNR, KB>>;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -290,6 +396,18 @@ __global__ void __launch_bounds__(kBlock)
   if constexpr (SEG) {
     tstart = tile_lo[b];
     t1 = tile_lo[b + 1];
+    Bx = 1;
+  } else if (chunk > 0) {
+    // one-shot schedule: workgroup b sweeps chunk c of `chunk` consecutive
+    // tiles.  xmap = 0: c = b, so the dispatcher's in-order hand-out keeps
+    // the whole grid on one front; xmap = 1: XCD k (= b % 8) takes the
+    // chunks of its own contiguous eighth in order (one front per XCD, the
+    // neighbour rows of a front in that XCD's L2).  Chunks past the last
+    // tile are empty and only write zero partials.
+    int c = b;
+    if (xmap) c = (b & 7) * (G >> 3) + (b >> 3);
+    tstart = (int)min<int64_t>((int64_t)c * chunk, ntiles);
+    t1 = (int)min<int64_t>((int64_t)tstart + chunk, ntiles);
     Bx = 1;
   } else {
     const int xcd = b & 7;
@@ -356,17 +474,17 @@ __global__ void __launch_bounds__(kBlock)
           Cm.c[u][e] = v ? c : 0;
           if constexpr (REALH) {
             const double h = rvals[k];
-            Cm.hr[u][e] = v ? h : 0.0;
-            Cm.hi[u][e] = 0.0;
+            Cm.hr[u][e] = v ? (AT)h : (AT)0;
+            Cm.hi[u][e] = (AT)0;
           } else {
             const double2 h = cvals[k];
-            Cm.hr[u][e] = v ? h.x : 0.0;
-            Cm.hi[u][e] = v ? h.y : 0.0;
+            Cm.hr[u][e] = v ? (AT)h.x : (AT)0;
+            Cm.hi[u][e] = v ? (AT)h.y : (AT)0;
           }
           if constexpr (kDual) {
             const double2 g = cvals2[k];
-            Cm.gr[u][e] = v ? g.x : 0.0;
-            Cm.gi[u][e] = v ? g.y : 0.0;
+            Cm.gr[u][e] = v ? (AT)g.x : (AT)0;
+            Cm.gi[u][e] = v ? (AT)g.y : (AT)0;
           }
         }
     } else {
@@ -380,9 +498,9 @@ __global__ void __launch_bounds__(kBlock)
         }
     }
   };
-  auto mac = [&](double hr, double hi, const T& xw, double* yr, double* yi) {
-    double xr[CPL], xi[CPL];
-    WT::unpack(xw, xr, xi);
+  auto mac = [&](AT hr, AT hi, const T& xw, AT* yr, AT* yi) {
+    AT xr[CPL], xi[CPL];
+    WT::unpack_a(xw, xr, xi);
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
       if constexpr (REALH) {
@@ -397,7 +515,8 @@ __global__ void __launch_bounds__(kBlock)
 
   // fixed-order workgroup reduction of the per-lane dot sums into dst
   __shared__ double2 red[ND > 0 ? kBlock * NDA * CPL : 1];
-  auto flush = [&](double2* dst) {
+  // (dot d, column col) goes to dst[(d * NRp + col) * stride]
+  auto flush = [&](double2* dst, size_t stride) {
 #pragma unroll
     for (int d = 0; d < NDA; ++d)
 #pragma unroll
@@ -417,15 +536,22 @@ __global__ void __launch_bounds__(kBlock)
           si += v.y;
         }
       const int col = (blockIdx.y * LPR + sb) * CPL + c;
-      dst[(size_t)d * NRp + col] = make_double2(sr, si);
+      dst[((size_t)d * NRp + col) * stride] = make_double2(sr, si);
     }
     __syncthreads();
   };
   // the previous step's moment columns (uniform per workgroup)
   if constexpr (!SEG && ND > 0) {
-    const int npairs = pc.cb.nd * pc.NR;
-    for (int q = blockIdx.y * gridDim.x + blockIdx.x; q < npairs; q += gridDim.x * gridDim.y)
-      combine_pair(pc.part, pc.G, pc.cb.nd, pc.NRp, pc.NR, pc.mu, pc.cb, q / pc.NR, q % pc.NR, red);
+    // the previous steps' reductions, uniform per workgroup: final units
+    // first, then first-level units
+    const int nfin = pc.part ? combine_units(pc.cb.nd, pc.NR) : 0;
+    const int nl1 = pc.l1_part ? reduce1_units(pc.l1_nd, pc.NR, pc.l1_G) : 0;
+    for (int q = blockIdx.y * gridDim.x + blockIdx.x; q < nfin + nl1; q += gridDim.x * gridDim.y) {
+      if (q < nfin)
+        combine_unit(pc.part, pc.G, pc.cb.nd, pc.NRp, pc.NR, pc.mu, pc.cb, q, red);
+      else
+        reduce1_unit(pc.l1_part, pc.l1_G, pc.l1_nd, pc.NRp, pc.NR, pc.l1_out, q - nfin, red);
+    }
   }
   int seg = 0, cur = 0;
   if constexpr (SEG) {
@@ -442,8 +568,8 @@ __global__ void __launch_bounds__(kBlock)
 
   for (; t < t1; t += Bx) {
     // 1. this tile's values, gathers and row-local operands, all in flight
-    double hr[UNR][KB], hi[UNR][KB];
-    double gr[kDual ? UNR : 1][kDual ? KB : 1], gi[kDual ? UNR : 1][kDual ? KB : 1];
+    AT hr[UNR][KB], hi[UNR][KB];
+    AT gr[kDual ? UNR : 1][kDual ? KB : 1], gi[kDual ? UNR : 1][kDual ? KB : 1];
     T xv[UNR][KB];
     if constexpr (kLaneMeta) {  // every lane is active here (uniform loop)
 #pragma unroll
@@ -453,7 +579,7 @@ __global__ void __launch_bounds__(kBlock)
           constexpr int e = jj / MG, src = jj % MG;
           const int c = group_bcast<MG, src>(C0.c[u][e]);
           hr[u][jj] = group_bcast<MG, src>(C0.hr[u][e]);
-          hi[u][jj] = REALH ? 0.0 : group_bcast<MG, src>(C0.hi[u][e]);
+          hi[u][jj] = REALH ? (AT)0 : group_bcast<MG, src>(C0.hi[u][e]);
           if constexpr (kDual) {
             gr[u][jj] = group_bcast<MG, src>(C0.gr[u][e]);
             gi[u][jj] = group_bcast<MG, src>(C0.gi[u][e]);
@@ -469,12 +595,12 @@ __global__ void __launch_bounds__(kBlock)
           const int k = v ? R0.s[u] + jj : 0;
           if constexpr (REALH) {
             const double h = rvals[k];
-            hr[u][jj] = v ? h : 0.0;
-            hi[u][jj] = 0.0;
+            hr[u][jj] = v ? (AT)h : (AT)0;
+            hi[u][jj] = (AT)0;
           } else {
             const double2 h = cvals[k];
-            hr[u][jj] = v ? h.x : 0.0;
-            hi[u][jj] = v ? h.y : 0.0;
+            hr[u][jj] = v ? (AT)h.x : (AT)0;
+            hi[u][jj] = v ? (AT)h.y : (AT)0;
           }
           xv[u][jj] = x[(size_t)C0.c[u][jj] * RW + wcol];
         }
@@ -495,17 +621,17 @@ __global__ void __launch_bounds__(kBlock)
     load_rows(t + 2 * Bx, R2);
 
     // 3. accumulate
-    double yr[UNR][CPL], yi[UNR][CPL];
-    double y2r[kDual ? UNR : 1][CPL], y2i[kDual ? UNR : 1][CPL];
+    AT yr[UNR][CPL], yi[UNR][CPL];
+    AT y2r[kDual ? UNR : 1][CPL], y2i[kDual ? UNR : 1][CPL];
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
 #pragma unroll
-      for (int c = 0; c < CPL; ++c) yr[u][c] = yi[u][c] = 0.0;
+      for (int c = 0; c < CPL; ++c) yr[u][c] = yi[u][c] = (AT)0;
 #pragma unroll
       for (int jj = 0; jj < KB; ++jj) mac(hr[u][jj], hi[u][jj], xv[u][jj], yr[u], yi[u]);
       if constexpr (kDual) {
 #pragma unroll
-        for (int c = 0; c < CPL; ++c) y2r[u][c] = y2i[u][c] = 0.0;
+        for (int c = 0; c < CPL; ++c) y2r[u][c] = y2i[u][c] = (AT)0;
 #pragma unroll
         for (int jj = 0; jj < KB; ++jj) mac(gr[u][jj], gi[u][jj], xv[u][jj], y2r[u], y2i[u]);
       }
@@ -515,20 +641,20 @@ __global__ void __launch_bounds__(kBlock)
       for (int u = 0; u < UNR; ++u)
         for (int j = KB; j < R0.cnt[u]; ++j) {
           const int k = R0.s[u] + j;
-          double h_r, h_i;
+          AT h_r, h_i;
           if constexpr (REALH) {
-            h_r = rvals[k];
-            h_i = 0.0;
+            h_r = (AT)rvals[k];
+            h_i = (AT)0;
           } else {
             const double2 h = cvals[k];
-            h_r = h.x;
-            h_i = h.y;
+            h_r = (AT)h.x;
+            h_i = (AT)h.y;
           }
           const T xw = x[(size_t)colind[k] * RW + wcol];
           mac(h_r, h_i, xw, yr[u], yi[u]);
           if constexpr (kDual) {
             const double2 g = cvals2[k];
-            mac(g.x, g.y, xw, y2r[u], y2i[u]);
+            mac((AT)g.x, (AT)g.y, xw, y2r[u], y2i[u]);
           }
         }
     }
@@ -537,32 +663,33 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
       const bool live = R0.row[u] < NH;
-      double orr[CPL], oii[CPL], zr[CPL], zi[CPL];
-      WT::unpack(zv[u], zr, zi);
+      const AT al = (AT)alpha, be = (AT)beta, ga = (AT)gamma;
+      AT orr[CPL], oii[CPL], zr[CPL], zi[CPL];
+      WT::unpack_a(zv[u], zr, zi);
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
-        orr[c] = has_z ? fma(alpha, yr[u][c], beta * zr[c]) : alpha * yr[u][c];
-        oii[c] = has_z ? fma(alpha, yi[u][c], beta * zi[c]) : alpha * yi[u][c];
+        orr[c] = has_z ? fma(al, yr[u][c], be * zr[c]) : al * yr[u][c];
+        oii[c] = has_z ? fma(al, yi[u][c], be * zi[c]) : al * yi[u][c];
       }
       if constexpr (MODE == MODE_SRC) {  // + gamma * s (reverse Clenshaw source)
-        double qr[CPL], qi[CPL];
-        WT::unpack(sv[u], qr, qi);
+        AT qr[CPL], qi[CPL];
+        WT::unpack_a(sv[u], qr, qi);
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
-          orr[c] = fma(gamma, qr[c], orr[c]);
-          oii[c] = fma(gamma, qi[c], oii[c]);
+          orr[c] = fma(ga, qr[c], orr[c]);
+          oii[c] = fma(ga, qi[c], oii[c]);
         }
       }
-      const T ow = WT::pack(orr, oii);
+      const T ow = WT::pack_a(orr, oii);
       if (live) st_stream(out + (size_t)R0.row[u] * RW + wcol, ow);
       if constexpr (kDual) {
-        double pr[CPL], pi[CPL];
+        AT pr[CPL], pi[CPL];
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
-          pr[c] = gamma * y2r[u][c];
-          pi[c] = gamma * y2i[u][c];
+          pr[c] = ga * y2r[u][c];
+          pi[c] = ga * y2i[u][c];
         }
-        if (live) st_stream(out2 + (size_t)R0.row[u] * o2.rw + o2col, WT::pack(pr, pi));
+        if (live) st_stream(out2 + (size_t)R0.row[u] * o2.rw + o2col, WT::pack_a(pr, pi));
       }
       if constexpr (MODE == MODE_DBL || MODE == MODE_LR) {
         double sr[CPL], si[CPL], ar[CPL], ai[CPL];
@@ -596,7 +723,7 @@ __global__ void __launch_bounds__(kBlock)
     if constexpr (SEG && ND > 0) {
       const int nxt = t + 1 < t1 ? treal[t + 1] : -1;  // uniform over the workgroup
       if (nxt != cur) {
-        flush(partials + (size_t)seg * ND * NRp);
+        flush(partials + (size_t)seg * ND * NRp, 1);
 #pragma unroll
         for (int d = 0; d < NDA; ++d)
 #pragma unroll
@@ -607,7 +734,8 @@ __global__ void __launch_bounds__(kBlock)
     }
   }
 
-  if constexpr (!SEG && ND > 0) flush(partials + (size_t)b * ND * NRp);
+  // per-workgroup partials [b][d][col]: each workgroup writes whole lines
+  if constexpr (!SEG && ND > 0) flush(partials + (size_t)b * ND * NRp, 1);
 }
 
 // ---- launcher tables -------------------------------------------------------------
@@ -615,7 +743,7 @@ __global__ void __launch_bounds__(kBlock)
 using StepLauncher = void (*)(dim3, hipStream_t, const int*, const int*, const void*, const void*,
                               const void*, void*, const void*, double2*, int, int, int, int,
                               double, double, int, const int*, const int*, const int*, double,
-                              const PendingCombine&, const void*, void*, const SplitOut&);
+                              const PendingCombine&, const void*, void*, const SplitOut&, int, int);
 
 template <int PREC, int LPR, int UNR, int KB, bool REALH, int MODE, bool SEG>
 static void launch_step_t(dim3 grid, hipStream_t st, const int* rp, const int* ci, const void* v,
@@ -623,11 +751,12 @@ static void launch_step_t(dim3 grid, hipStream_t st, const int* rp, const int* c
                           int NH, int RW, int NRp, int ntiles, double alpha, double beta,
                           int has_tail, const int* tile_lo, const int* treal, const int* seg0,
                           double gamma, const PendingCombine& pc, const void* v2, void* out2,
-                          const SplitOut& o2) {
+                          const SplitOut& o2, int chunk, int xmap) {
   using T = typename Word<PREC>::T;
   hipLaunchKernelGGL((k_step<PREC, LPR, UNR, KB, REALH, MODE, SEG>), grid, dim3(kBlock), 0, st, rp,
                      ci, v, (const T*)x, (const T*)z, (T*)out, (const T*)l, part, NH, RW, NRp,
-                     ntiles, alpha, beta, has_tail, tile_lo, treal, seg0, gamma, pc, v2, (T*)out2, o2);
+                     ntiles, alpha, beta, has_tail, tile_lo, treal, seg0, gamma, pc, v2, (T*)out2, o2,
+                     chunk, xmap);
 }
 
 template <int PREC, int LPR, int UNR, int KB, bool REALH>

@@ -59,31 +59,80 @@ int kpm_autotune_step(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z
   ++ctx->epoch;
   const int unrolls[2] = {1, 2};
   const int per_cu[4] = {2, 4, 8, 16};
-  float best = 1e30f;
-  StepConfig bestc;
-  for (int u : unrolls)
+  // candidates: persistent grids of 2..16 workgroups per CU, and one-shot
+  // grids of 2..64 tiles per workgroup (in-order or per-XCD chunk order) of
+  // at most kMaxChunkGridPerCU workgroups per CU
+  std::vector<StepConfig> cand;
+  for (int u : unrolls) {
     for (int pc : per_cu) {
       StepConfig c;
       c.unroll = u;
       c.grid = std::max(ctx->num_cus, 8) * pc;
-      StepLaunch s{H, X, Z, Z, nullptr, 2.0, -1.0, MODE_DBL};
-      KPM_TRY(launch_step(ctx, s, c, nullptr));  // warm-up
-      KPM_HIP_TRY(hipEventRecord(ctx->events[12], ctx->stream));
-      for (int r = 0; r < reps; ++r) KPM_TRY(launch_step(ctx, s, c, nullptr));
-      KPM_HIP_TRY(hipEventRecord(ctx->events[13], ctx->stream));
-      KPM_HIP_TRY(hipEventSynchronize(ctx->events[13]));
-      float ms = 0.f;
-      KPM_HIP_TRY(hipEventElapsedTime(&ms, ctx->events[12], ctx->events[13]));
-      ms /= (float)reps;
-      if (ms < best) {
-        best = ms;
-        bestc = c;
+      cand.push_back(c);
+    }
+    const StepGeom g = step_geom(H, X->NR, X->prec, StepConfig{u, 0, 0, 0});
+    const int64_t ntiles = (X->NH + g.TR - 1) / g.TR;
+    const int64_t cap = (int64_t)kMaxChunkGridPerCU * std::max(ctx->num_cus, 1);
+    for (int ch : {2, 4, 8, 16, 32, 64}) {
+      const int64_t gc = ((ntiles + ch - 1) / ch + 7) / 8 * 8;
+      if (gc > cap || gc < 2LL * std::max(ctx->num_cus, 8)) continue;  // too many, or too few to fill the chip
+      for (int xm : {0, 1}) {
+        StepConfig c;
+        c.unroll = u;
+        c.chunk = ch;
+        c.xmap = xm;
+        cand.push_back(c);
       }
     }
+  }
+  float best = 1e30f;
+  StepConfig bestc;
+  // each candidate runs as the engine runs it: fused passes with the
+  // deferred moment reductions their grid implies
+  for (const StepConfig& c : cand) {
+    float ms = 0.f;
+    KPM_TRY(time_engine_steps(ctx, H, X, Z, c, reps, &ms));
+    ms /= (float)reps;
+    if (ms < best) {
+      best = ms;
+      bestc = c;
+    }
+  }
   tune_record(ctx, H, X->NR, X->prec, bestc);
   if (best_unroll) *best_unroll = bestc.unroll;
-  if (best_grid) *best_grid = bestc.grid;
+  if (best_grid) {  // the workgroup count the winner launches with
+    const StepGeom g = step_geom(H, X->NR, X->prec, bestc);
+    const int64_t ntiles = (X->NH + g.TR - 1) / g.TR;
+    *best_grid = bestc.chunk > 0 ? (int)(((ntiles + bestc.chunk - 1) / bestc.chunk + 7) / 8 * 8)
+                                 : (bestc.grid > 0 ? bestc.grid : default_grid(ctx, ntiles));
+  }
   if (best_ms_per_step) *best_ms_per_step = best;
+  return KPM_OK;
+}
+
+// Set a one-shot (chunked) table entry directly.
+int kpm_tune_set_chunked(kpm_ctx* ctx, const kpm_csr* H, int64_t NR, int vec_prec, int unroll,
+                         int chunk, int xcd_grouped) {
+  if (!ctx || !H || NR <= 0) return fail(KPM_EINVAL, "bad arguments");
+  if (unroll != 1 && unroll != 2) return fail(KPM_EINVAL, "unroll must be 1 or 2");
+  if (chunk < 1) return fail(KPM_EINVAL, "chunk must be >= 1 tile");
+  StepConfig c;
+  c.unroll = unroll;
+  c.chunk = chunk;
+  c.xmap = xcd_grouped ? 1 : 0;
+  tune_record(ctx, H, NR, vec_prec, c);
+  return KPM_OK;
+}
+
+// The table entry the engines use for a shape (defaults when untuned).
+int kpm_tune_get(kpm_ctx* ctx, const kpm_csr* H, int64_t NR, int vec_prec, int* unroll, int* grid,
+                 int* chunk, int* xcd_grouped) {
+  if (!ctx || !H || NR <= 0) return fail(KPM_EINVAL, "bad arguments");
+  const StepConfig c = tuned_config(ctx, H, NR, vec_prec, MODE_DBL);
+  if (unroll) *unroll = c.unroll;
+  if (grid) *grid = c.grid;
+  if (chunk) *chunk = c.chunk;
+  if (xcd_grouped) *xcd_grouped = c.xmap;
   return KPM_OK;
 }
 

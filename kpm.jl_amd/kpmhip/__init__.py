@@ -47,6 +47,6 @@ from .primitives import (chebyshev_iter, chebyshev_iter_single, chebyshev_iter_w
                          r2_ipp, r_i, r_ip, r_ipp, spmm)
 from .rescale import normalizeH
 from .rng import Xoshiro256pp, default_rng
-from .tune import autotune, tune_set
+from .tune import autotune, tune_get, tune_set, tune_set_chunked
 
 __all__ = [n for n in dir() if not n.startswith("_")]

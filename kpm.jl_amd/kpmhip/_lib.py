@@ -138,6 +138,9 @@ _SIGS = {
     "kpm_autotune_step": (C.c_int, [_p, _p, _p, _p, C.c_int, C.POINTER(C.c_int),
                                     C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "kpm_tune_set": (C.c_int, [_p, _p, _i64, C.c_int, C.c_int, C.c_int]),
+    "kpm_tune_set_chunked": (C.c_int, [_p, _p, _i64, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "kpm_tune_get": (C.c_int, [_p, _p, _i64, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                               C.POINTER(C.c_int), C.POINTER(C.c_int)]),
 }
 
 EXPORTED = tuple(_SIGS)

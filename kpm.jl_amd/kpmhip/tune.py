@@ -1,10 +1,12 @@
 """Launch-shape autotuner of the fused recurrence pass (no reference
 counterpart; the reference has no tuning knob).
 
-``autotune(H, NR, precision)`` times every candidate geometry (rows in flight
-per lane x workgroups per CU) of the kpm_1d hot-loop pass on the caller's
-operator and a probe-sized block, and records the fastest for that shape in
-the library's table; every later engine call on the same shape uses it.
+``autotune(H, NR, precision)`` times every candidate geometry of the kpm_1d
+hot-loop pass (rows in flight per lane x a persistent grid of 2..16
+workgroups per CU or a one-shot grid of 2..64 row tiles per workgroup) on
+the caller's operator and a probe-sized block, and records the fastest for
+that shape in the library's table; every later engine call on the same shape
+uses it.
 """
 from __future__ import annotations
 
@@ -23,8 +25,31 @@ def autotune(H, NR: int, precision="f64", reps: int = 5) -> dict:
     unroll, grid, ms = C.c_int(0), C.c_int(0), C.c_float(0.0)
     check(ctx.lib.kpm_autotune_step(ctx.handle, Hd.handle, x.handle, z.handle, int(reps),
                                     C.byref(unroll), C.byref(grid), C.byref(ms)), "autotune")
-    return {"unroll": unroll.value, "grid": grid.value, "ms_per_pass": ms.value,
-            "precision": prec_code(precision)}
+    out = {"unroll": unroll.value, "grid": grid.value, "ms_per_pass": ms.value,
+           "precision": prec_code(precision)}
+    out.update({k: v for k, v in tune_get(Hd, NR, precision).items() if k in ("chunk", "xmap")})
+    return out
+
+
+def tune_get(H, NR: int, precision="f64") -> dict:
+    """The launch shape the engines use for (H, NR, precision); chunk 0 is
+    the persistent grid."""
+    ctx = get_context()
+    Hd = to_device(H, ctx)
+    u, g, ch, xm = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
+    check(ctx.lib.kpm_tune_get(ctx.handle, Hd.handle, int(NR), prec_code(precision), C.byref(u),
+                               C.byref(g), C.byref(ch), C.byref(xm)), "tune_get")
+    return {"unroll": u.value, "grid": g.value, "chunk": ch.value, "xmap": xm.value}
+
+
+def tune_set_chunked(H, NR: int, precision="f64", unroll: int = 1, chunk: int = 64,
+                     xcd_grouped: bool = False) -> None:
+    """Record the one-shot schedule (chunk row tiles per workgroup) for a shape."""
+    ctx = get_context()
+    Hd = to_device(H, ctx)
+    check(ctx.lib.kpm_tune_set_chunked(ctx.handle, Hd.handle, int(NR), prec_code(precision),
+                                       int(unroll), int(chunk), 1 if xcd_grouped else 0),
+          "tune_set_chunked")
 
 
 def tune_set(H, NR: int, precision="f64", unroll: int = 2, grid: int = 0) -> None:
