@@ -59,6 +59,10 @@ def main() -> int:
     ap.add_argument("--skip-fp64", action="store_true", help="omit the FP64 comparison leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--kernel-launches", type=int, default=100)
+    ap.add_argument("--shape", default="",
+                    help="fix the fused pass's launch shape instead of tuning it: "
+                         "'unroll:chunk' (chunk 0: the persistent grid), e.g. 2:8; used to "
+                         "profile the timed calls alone")
     ap.add_argument("--workload", choices=["graphene", "ensemble", "conductivity", "core", "adjoint",
                                            "cpge", "cond_ensemble", "spmm"],
                     default="graphene",
@@ -118,8 +122,16 @@ def main() -> int:
     Hd = K.to_device(H)
 
     def run_leg(precision: str):
-        tuned = K.autotune(Hd, NR, precision)
-        print(f"[bench] autotune {precision}: {tuned}", file=sys.stderr, flush=True)
+        if args.shape:
+            u, ch = (int(v) for v in args.shape.split(":"))
+            if ch > 0:
+                K.tune_set_chunked(Hd, NR, precision, unroll=u, chunk=ch)
+            else:
+                K.tune_set(Hd, NR, precision, unroll=u, grid=0)
+            tuned = dict(K.tune_get(Hd, NR, precision), precision=precision, fixed=True)
+        else:
+            tuned = K.autotune(Hd, NR, precision)
+        print(f"[bench] launch shape {precision}: {tuned}", file=sys.stderr, flush=True)
         blk = K.DeviceBlock.from_host(psi, precision)
         mu = np.zeros((NR, NC), dtype=np.complex128, order="F")
         lib = ctx.lib
