@@ -431,13 +431,14 @@ def bench_adjoint(args) -> int:
     t_ij ~ 1 + U[-0.1, 0.1] (all trainable), fixed a = 5.5 (Gershgorin bound
     4 * 1.1 + 1 = 5.4), NC = 1024, NR = 16, Jackson kernel; loss = sum over 256
     energies of (rho(E) - rho*(E))^2 with rho* the DOS of a second disorder
-    seed.  One step = forward moments + loss + reverse Chebyshev gradient
-    (every forward vector kept in HBM, 69 GB here) + per-site / per-bond
-    gradients, host H in.  Algorithmic bytes (FP64, per step): forward
-    moments NC/2 fused passes; adjoint NC-2 forward passes that keep alpha,
-    NC-1 reverse passes (one extra source read), and the gradient terms:
-    beta and alpha read once each per reverse step, the nnz gradient read and
-    written once per 8-step window."""
+    seed.  One step = one forward pass that keeps every vector in HBM (69 GB
+    here) and dots each order with the probes (the moments), the loss, the
+    reverse Chebyshev gradient on the kept vectors, and the per-site /
+    per-bond gradients, host H in.  Algorithmic bytes (FP64, per step):
+    NC-1 forward passes (x, z and the probe block read, out written), NC-1
+    reverse passes (x, z and the source read, out written) and the gradient
+    terms: beta and alpha read once each per reverse step, the nnz gradient
+    read and written once per 8-step window."""
     import numpy as np
 
     import kpmhip as K
@@ -460,10 +461,12 @@ def bench_adjoint(args) -> int:
     T = np.cos(ns[None, :] * np.arccos(x)[:, None])
     rho_t = (T * w[None, :] / (a * np.pi * np.sqrt(1 - x * x))[:, None]) @ mu_t
 
+    Hd = K.to_device(Hn)  # the operator stays on the device across steps
+
     def step():
-        mu = K.kpm_1d(Hn, NC, NR, psi_in=psi)
-        loss, g = K.dos_loss(mu, a, 0.0, E, rho_t)
-        grad = K.kpm_1d_adjoint(Hn, NC, psi, g)
+        tape = K.kpm_1d_adjoint_forward(Hn, NC, psi, operator=Hd)
+        loss, g = K.dos_loss(tape.mu, a, 0.0, E, rho_t)
+        grad = tape.pullback(g)
         return loss, K.site_gradient(grad, a), K.bond_gradient(H, grad, a)
 
     for _ in range(max(args.warmup, 1)):
@@ -481,11 +484,9 @@ def bench_adjoint(args) -> int:
     nrp = K.moment._padded_nr(NR, 0)
     vec = NH * nrp * 16
     hb = nnz * (8 + 4) + (NH + 1) * 4
-    fwd = (NC // 2) * (hb + 3 * vec)
     windows = -(-(NC - 1) // 8)
-    adj = ((NC - 2) * (hb + 3 * vec) + (NC - 1) * (hb + 4 * vec) + (NC - 1) * 2 * vec
-           + windows * (nnz * (4 + 32) + (NH + 1) * 4))
-    total = fwd + adj
+    total = ((NC - 1) * (hb + 4 * vec) + (NC - 1) * (hb + 4 * vec) + (NC - 1) * 2 * vec
+             + windows * (nnz * (4 + 32) + (NH + 1) * 4))
     out = {"metric": "DOS-loss gradient step time (forward + reverse Chebyshev)",
            "value": round(wall * 1e3, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps,
            "warmup": max(args.warmup, 1), "ms_per_step": round(wall * 1e3, 3),

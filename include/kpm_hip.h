@@ -342,6 +342,20 @@ int kpm_2d_adjoint(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, const kpm_
                    const kpm_csr* Jb_adj, int64_t NC, int64_t NR, int64_t NH, const double* psi_l,
                    const double* psi_r, const double* gbar, double* grad_H, double* grad_Ja,
                    double* grad_Jb);
+/* The two halves of kpm_1d_adjoint, so a loss can be formed from the
+ * moments of the same forward pass (no reference counterpart).
+ * kpm_1d_adjoint_forward runs the forward recurrence once, keeps its
+ * vectors (or checkpoints, as ckpt_stride says) in the context and writes
+ * the moments mu_all[r, n] = <psi_r|T_n(H)|psi_r> (NR x NC column-major
+ * complex, the equal-vector moments of kpm_1d! as bra/ket dots of every
+ * order).  kpm_1d_adjoint_reverse then takes g = dL/dmu_n (NC reals,
+ * probe-averaged moments) and writes grad as kpm_1d_adjoint does; it needs
+ * the same operator, and any later forward pass replaces the kept one.
+ * Gradients are deterministic: rows wider than 64 words sum their row
+ * chunks in a fixed order. */
+int kpm_1d_adjoint_forward(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t NR, int64_t NH,
+                           const double* psi, double* mu_all, int vec_prec, int64_t ckpt_stride);
+int kpm_1d_adjoint_reverse(kpm_ctx* ctx, const kpm_csr* H, const double* g, double* grad);
 /* Host bookkeeping for per-bond gradients (no reference counterpart): for a
  * CSC pattern with sorted rows holding both (i, j) and (j, i), and grad on that
  * pattern (the output of kpm_1d_adjoint), write every pair i < j with

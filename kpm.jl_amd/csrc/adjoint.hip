@@ -29,6 +29,7 @@
 #include <cmath>
 #include <vector>
 
+#include "dot_steps.h"
 #include "step_kernel.h"
 
 namespace kpm {
@@ -44,12 +45,13 @@ struct SddmmWindow {
 
 // M[k] += sum_{w < W} c_w sum_r conj(beta_w[a,r]) alpha_w[colind[k], r] for
 // every stored entry k of row a; LPR lanes per row, reduced across the row's
-// lanes (fixed xor order), then across row chunks with atomics only when a
-// row is wider than 64 words.
+// lanes (group_sum, a fixed order).  Rows wider than 64 words run as row chunks
+// (gridDim.y > 1), each adding into its own accumulator M + chunk * mstride;
+// the caller sums the chunks in order, so every entry is deterministic.
 template <int PREC, int LPR>
 __global__ void __launch_bounds__(kBlock)
     k_sddmm(const int* __restrict__ rowptr, const int* __restrict__ colind, SddmmWindow<PREC> win,
-            double2* __restrict__ M, int NH, int RW) {
+            double2* __restrict__ M, int NH, int RW, size_t mstride) {
   using WT = Word<PREC>;
   constexpr int CPL = WT::CPL;
   constexpr int RPW = 64 / LPR;
@@ -98,39 +100,35 @@ __global__ void __launch_bounds__(kBlock)
           pi = fma(win.c[w], qi, pi);
         }
       }
-#pragma unroll
-      for (int o = 1; o < LPR; o <<= 1) {
-        pr += __shfl_xor(pr, o);
-        pi += __shfl_xor(pi, o);
-      }
-      if (v && sub == 0 && gridDim.y == 1) {
-        double2 m = M[k];
+      pr = group_sum<LPR>(pr);
+      pi = group_sum<LPR>(pi);
+      if (v && sub == 0) {
+        double2* Mc = M + (size_t)blockIdx.y * mstride;
+        double2 m = Mc[k];
         m.x += pr;
         m.y += pi;
-        M[k] = m;
-      }
-      if (gridDim.y > 1 && v && sub == 0) {
-        // rows wider than 64 words: each row chunk adds its own columns
-        atomicAdd(&M[k].x, pr);
-        atomicAdd(&M[k].y, pi);
+        Mc[k] = m;
       }
     }
   }
 }
 
 // The same sums for rows of LPR >= 2 lanes, at most KB <= 16 nonzeros and a
-// single row chunk, with every memory access of a row issued together: with
-// G = min(LPR, 16), lane j of a row holds column index and accumulator entry
-// of nonzeros j, j + G, ... (one coalesced load each), group_bcast hands slot
-// jj's column to the row, the window's KB gathers per step go out at once,
-// and after the row reduction (the xor order above) each lane adds its own
-// entries and stores them coalesced.  Per entry the terms are summed in the
-// order of k_sddmm, so both kernels give bitwise the same M.
+// single row chunk, arranged for loads in flight: with G = min(LPR, 16),
+// lane j of a row holds column index and accumulator entry of nonzeros
+// j, j + G, ... (one coalesced load each); the window's beta words are
+// loaded once and scaled by c_w (1 or 2: exact); then for each nonzero slot
+// up to the wave's longest row, group_bcast hands the column to the row and
+// the W gathers of alpha_w at that column go out together, are summed over
+// the windows in order and reduced over the row's lanes (group_sum); each
+// lane finally adds its own entries and stores them coalesced.  Slots past a
+// row's end issue no load.
 template <int PREC, int LPR, int KB>
 __global__ void __launch_bounds__(kBlock)
     k_sddmm_rows(const int* __restrict__ rowptr, const int* __restrict__ colind,
                  SddmmWindow<PREC> win, double2* __restrict__ M, int NH, int RW) {
   using WT = Word<PREC>;
+  using T = typename WT::T;
   constexpr int CPL = WT::CPL;
   constexpr int RPW = 64 / LPR;
   constexpr int MG = LPR >= 16 ? 16 : LPR;
@@ -147,8 +145,14 @@ __global__ void __launch_bounds__(kBlock)
     const int row = base + wave * RPW + rslot;
     const bool live = row < NH;
     const int rc = live ? row : 0;
+    T bw[kWin];
+#pragma unroll
+    for (int w = 0; w < kWin; ++w) bw[w] = w < win.W ? win.beta[w][(size_t)rc * RW + sub] : T{};
     const int s0 = rowptr[rc];
     const int n = live ? rowptr[rc + 1] - s0 : 0;
+    int nmax = n;
+#pragma unroll
+    for (int o = LPR; o < 64; o <<= 1) nmax = max(nmax, __shfl_xor(nmax, o));
     int cl[ME];
     double2 m[ME];
 #pragma unroll
@@ -156,61 +160,45 @@ __global__ void __launch_bounds__(kBlock)
       const int idx = e * MG + j;
       const bool v = idx < n;
       const int k = v ? s0 + idx : 0;
-      const int c = colind[k];
-      const double2 mv = M[k];
-      cl[e] = v ? c : 0;
-      m[e] = mv;
+      cl[e] = v ? colind[k] : 0;
+      m[e] = v ? M[k] : make_double2(0.0, 0.0);
     }
     double br[kWin][CPL], bi[kWin][CPL];
 #pragma unroll
     for (int w = 0; w < kWin; ++w) {
-      if (w < win.W) {
-        WT::unpack(win.beta[w][(size_t)rc * RW + sub], br[w], bi[w]);
-      } else {
+      WT::unpack(bw[w], br[w], bi[w]);
 #pragma unroll
-        for (int q = 0; q < CPL; ++q) br[w][q] = bi[w][q] = 0.0;
+      for (int q = 0; q < CPL; ++q) {
+        br[w][q] *= win.c[w];
+        bi[w][q] *= win.c[w];
       }
     }
-    int col[KB];
     static_for<0, KB>([&](auto J) {
       constexpr int jj = decltype(J)::value;
-      col[jj] = group_bcast<MG, jj % MG>(cl[jj / MG]);
-    });
-    double pr[KB], pi[KB];
+      if (jj < nmax) {  // uniform
+        const int col = group_bcast<MG, jj % MG>(cl[jj / MG]);
+        const bool v = jj < n;
+        const size_t off = (size_t)col * RW + sub;
+        T av[kWin];
 #pragma unroll
-    for (int jj = 0; jj < KB; ++jj) pr[jj] = pi[jj] = 0.0;
+        for (int w = 0; w < kWin; ++w) av[w] = (w < win.W && v) ? win.alpha[w][off] : T{};
+        double pr = 0.0, pi = 0.0;
 #pragma unroll
-    for (int w = 0; w < kWin; ++w) {
-      if (w < win.W) {  // uniform
-        typename WT::T av[KB];
-#pragma unroll
-        for (int jj = 0; jj < KB; ++jj) av[jj] = win.alpha[w][(size_t)col[jj] * RW + sub];
-#pragma unroll
-        for (int jj = 0; jj < KB; ++jj) {
+        for (int w = 0; w < kWin; ++w) {
           double ar[CPL], ai[CPL];
-          WT::unpack(av[jj], ar, ai);
-          double qr = 0.0, qi = 0.0;
+          WT::unpack(av[w], ar, ai);
 #pragma unroll
           for (int q = 0; q < CPL; ++q) {
-            qr = fma(br[w][q], ar[q], fma(bi[w][q], ai[q], qr));
-            qi = fma(br[w][q], ai[q], fma(-bi[w][q], ar[q], qi));
+            pr = fma(br[w][q], ar[q], fma(bi[w][q], ai[q], pr));
+            pi = fma(br[w][q], ai[q], fma(-bi[w][q], ar[q], pi));
           }
-          pr[jj] = fma(win.c[w], qr, pr[jj]);
-          pi[jj] = fma(win.c[w], qi, pi[jj]);
         }
-      }
-    }
-    static_for<0, KB>([&](auto J) {
-      constexpr int jj = decltype(J)::value;
-      double tr = pr[jj], ti = pi[jj];
-#pragma unroll
-      for (int o = 1; o < LPR; o <<= 1) {
-        tr += __shfl_xor(tr, o);
-        ti += __shfl_xor(ti, o);
-      }
-      if (j == jj % MG) {
-        m[jj / MG].x += tr;
-        m[jj / MG].y += ti;
+        pr = group_sum<LPR>(pr);
+        pi = group_sum<LPR>(pi);
+        if (j == jj % MG) {
+          m[jj / MG].x += pr;
+          m[jj / MG].y += pi;
+        }
       }
     });
 #pragma unroll
@@ -221,9 +209,15 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// resident workgroups per CU of the gradient-term launches (grid stride)
+#ifndef KPM_SDDMM_WG_PER_CU
+#define KPM_SDDMM_WG_PER_CU 8
+#endif
+
 template <int PREC>
 static int launch_sddmm_t(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* const* beta,
-                          const kpm_block* const* alpha, const double* c, int W, double2* M) {
+                          const kpm_block* const* alpha, const double* c, int W, double2* M,
+                          size_t mstride) {
   using T = typename Word<PREC>::T;
   SddmmWindow<PREC> win{};
   for (int w = 0; w < W; ++w) {
@@ -237,7 +231,8 @@ static int launch_sddmm_t(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* const
   const int chunks = RW / lpr;
   const int64_t rows_per_block = 4LL * (64 / lpr);
   const int64_t nb = (beta[0]->NH + rows_per_block - 1) / rows_per_block;
-  const unsigned G = (unsigned)std::min<int64_t>(nb, (int64_t)std::max(ctx->num_cus, 8) * 8);
+  const unsigned G =
+      (unsigned)std::min<int64_t>(nb, (int64_t)std::max(ctx->num_cus, 8) * KPM_SDDMM_WG_PER_CU);
   const int w = A->max_row_nnz;
   if (lpr >= 2 && chunks == 1 && w <= 16) {
     const int kb = w <= 4 ? 4 : (w <= 8 ? 8 : (w <= 12 ? 12 : 16));
@@ -262,7 +257,7 @@ static int launch_sddmm_t(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* const
 #define KPM_SDDMM(L)                                                                              \
   case L:                                                                                         \
     hipLaunchKernelGGL((k_sddmm<PREC, L>), dim3(G, chunks), dim3(kBlock), 0, ctx->stream,         \
-                       A->rowptr, A->colind, win, M, (int)beta[0]->NH, RW);                       \
+                       A->rowptr, A->colind, win, M, (int)beta[0]->NH, RW, mstride);              \
     break;
   switch (lpr) {
     KPM_SDDMM(1)
@@ -273,18 +268,27 @@ static int launch_sddmm_t(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* const
     KPM_SDDMM(32)
     default:
       hipLaunchKernelGGL((k_sddmm<PREC, 64>), dim3(G, chunks), dim3(kBlock), 0, ctx->stream,
-                         A->rowptr, A->colind, win, M, (int)beta[0]->NH, RW);
+                         A->rowptr, A->colind, win, M, (int)beta[0]->NH, RW, mstride);
   }
 #undef KPM_SDDMM
   KPM_HIP_TRY(hipGetLastError());
   return KPM_OK;
 }
 
+int sddmm_chunks(int64_t NR, int prec) {
+  const int RW = row_words(NR, prec);
+  return RW > 64 ? RW / 64 : 1;
+}
+
 int launch_sddmm(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* const* beta,
-                 const kpm_block* const* alpha, const double* c, int W, double2* M) {
+                 const kpm_block* const* alpha, const double* c, int W, double2* M,
+                 size_t mstride) {
   if (W < 1 || W > kWin) return fail(KPM_EINVAL, "sddmm window out of range");
-  return beta[0]->prec == KPM_VEC_F32 ? launch_sddmm_t<KPM_VEC_F32>(ctx, A, beta, alpha, c, W, M)
-                                      : launch_sddmm_t<KPM_VEC_F64>(ctx, A, beta, alpha, c, W, M);
+  if (sddmm_chunks(beta[0]->NR, beta[0]->prec) > 1 && mstride < (size_t)A->nnz)
+    return fail(KPM_EINVAL, "sddmm: row chunks need one accumulator each");
+  return beta[0]->prec == KPM_VEC_F32
+             ? launch_sddmm_t<KPM_VEC_F32>(ctx, A, beta, alpha, c, W, M, mstride)
+             : launch_sddmm_t<KPM_VEC_F64>(ctx, A, beta, alpha, c, W, M, mstride);
 }
 
 int adj_step(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* x, const kpm_block* z,
@@ -295,192 +299,297 @@ int adj_step(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* x, const kpm_block
 
 }  // namespace kpm
 
-using namespace kpm;
+namespace kpm {
 
-extern "C" {
+// Workspace of the split adjoint: v | beta ring (nbr) | then either every
+// alpha_1..alpha_{nalpha-1} (full) or the forward ring (2), 2 * nseg
+// checkpoint blocks and one segment of K blocks.
+struct AdjLayout {
+  int64_t NC, NR, NRp, NH, nalpha, nbr, K, nseg;
+  int prec;
+  bool full;
+  size_t bbytes;
+  int64_t nblk() const { return full ? 1 + nbr + nalpha : 1 + nbr + 2 + 2 * nseg + K; }
+};
 
-int kpm_1d_adjoint(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t NR, int64_t NH,
-                   const double* psi, const double* g, double* grad, int vec_prec,
-                   int64_t ckpt_stride) {
-  if (!ctx || !H || !psi || !g || !grad) return fail(KPM_EINVAL, "null argument");
+static AdjLayout adj_layout(kpm_ctx* ctx, int64_t NC, int64_t NR, int64_t NH, int prec,
+                            int64_t ckpt_stride, size_t avail) {
+  AdjLayout L;
+  L.NC = NC;
+  L.NR = NR;
+  L.NH = NH;
+  L.prec = prec;
+  L.NRp = padded_nr(NR, prec);
+  L.bbytes = (size_t)(NH * L.NRp) * elem_bytes(prec);
+  L.nalpha = NC - 1;  // alpha_0 .. alpha_{NC-2} enter the gradient
+  L.nbr = kWin + 2;   // beta ring: the recurrence's two plus a window
+  L.full = ckpt_stride == 1 ||
+           (ckpt_stride == 0 && L.bbytes * (size_t)(1 + L.nbr + L.nalpha) <= avail / 10 * 6);
+  L.K = L.full ? 1 : (ckpt_stride > 1 ? ckpt_stride : (int64_t)std::ceil(std::sqrt((double)NC)));
+  L.nseg = L.full ? 0 : (L.nalpha + L.K - 1) / L.K;
+  (void)ctx;
+  return L;
+}
+
+static kpm_block adj_blk(kpm_ctx* ctx, const AdjLayout& L, int64_t i) {
+  kpm_block b;
+  b.ctx = ctx;
+  b.NH = L.NH;
+  b.NR = L.NR;
+  b.NRp = L.NRp;
+  b.prec = L.prec;
+  b.d = (char*)ctx->adjoint.ptr + L.bbytes * (size_t)i;
+  b.owned = false;
+  return b;
+}
+
+// forward pass: keeps alpha (or checkpoint pairs) and writes the moments
+// mu_n = <v|alpha_n> (n < NC, device NR x NC) from the same passes.  The
+// moments come from the doubling identities of kpm_1d! (src/moment.jl:442-487):
+// pass k <= ceil(NC/2) dots <alpha_{k-1}|alpha_{k-1}> and
+// <alpha_{k-1}|alpha_k>, which give mu_{2k-2} and mu_{2k-1}; later passes
+// carry no dots.  No pass reads more than the recurrence's own vectors.
+static int adj_forward(kpm_ctx* ctx, const kpm_csr* H, const AdjLayout& L, const double* psi) {
+  KPM_TRY(ensure(ctx, ctx->adjoint, L.bbytes * (size_t)L.nblk()));
+  kpm_block v = adj_blk(ctx, L, 0);
+  KPM_HIP_TRY(hipMemsetAsync(v.d, 0, L.bbytes, ctx->stream));
+  KPM_TRY(ensure(ctx, ctx->staging, (size_t)(L.NH * L.NR) * 16));
+  KPM_HIP_TRY(hipMemcpyAsync(ctx->staging.ptr, psi, (size_t)(L.NH * L.NR) * 16,
+                             hipMemcpyHostToDevice, ctx->stream));
+  KPM_TRY(launch_upload(ctx, ctx->staging.ptr, &v));
+  // one spare column: for odd NC the last dot pass also yields mu_NC
+  KPM_TRY(ensure(ctx, ctx->mu, (size_t)(L.NR * (L.NC + 1)) * sizeof(double2)));
+  const int64_t ndot = (L.NC + 1) / 2;  // passes that carry dots
+  DotSteps ds{ctx, H, L.NR, L.NRp, {nullptr, nullptr}};
+  KPM_TRY(ds.init(2, &v, MODE_DBL));
+  auto dbl = [](int64_t k) {
+    if (k == 1) return Combine{2, {0, 1}, {1.0, 1.0}, {-1, -1}, {0.0, 0.0}};
+    return Combine{2, {(int)(2 * k - 2), (int)(2 * k - 1)}, {2.0, 2.0}, {0, 1}, {-1.0, -1.0}};
+  };
+  // pass k: out = alpha_k from x = alpha_{k-1}, z = alpha_{k-2}
+  auto pass = [&](int64_t k, kpm_block* x, kpm_block* z, kpm_block* o) -> int {
+    const double a = k == 1 ? 1.0 : 2.0, b = k == 1 ? 0.0 : -1.0;
+    if (k <= ndot) {
+      KPM_TRY(ds.step(x, k == 1 ? nullptr : z, o, nullptr, a, b, MODE_DBL, dbl(k)));
+      if (k == ndot) KPM_TRY(ds.finish());
+      return KPM_OK;
+    }
+    return adj_step(ctx, H, x, k == 1 ? nullptr : z, o, nullptr, a, b, 0.0, MODE_NONE);
+  };
+  kpm_block scratch = adj_blk(ctx, L, 1);  // a beta-ring block, cleared by the reverse pass
+  if (L.full) {
+    auto alpha = [&](int64_t i) { return i == 0 ? v : adj_blk(ctx, L, 1 + L.nbr + i); };
+    for (int64_t k = 1; k < L.NC; ++k) {
+      kpm_block x = alpha(k - 1), z = k >= 2 ? alpha(k - 2) : v, o = k < L.nalpha ? alpha(k) : scratch;
+      KPM_TRY(pass(k, &x, &z, &o));
+    }
+  } else {
+    // a two-block ring with checkpoint pairs (alpha_{sK}, alpha_{sK+1})
+    kpm_block ring[2] = {adj_blk(ctx, L, 1 + L.nbr), adj_blk(ctx, L, 2 + L.nbr)};
+    KPM_TRY(kpm_block_copy(&ring[0], &v));
+    auto ckA = [&](int64_t sg) { return adj_blk(ctx, L, 1 + L.nbr + 2 + 2 * sg); };
+    auto ckB = [&](int64_t sg) { return adj_blk(ctx, L, 1 + L.nbr + 3 + 2 * sg); };
+    for (int64_t k = 1; k < L.NC; ++k) {
+      // alpha_k overwrites alpha_{k-2} in place
+      KPM_TRY(pass(k, &ring[(k - 1) % 2], &ring[k % 2], &ring[k % 2]));
+      if ((k - 1) % L.K == 0 && (k - 1) / L.K < L.nseg) {  // pair starting segment (k-1)/K
+        const int64_t sg = (k - 1) / L.K;
+        kpm_block a = ckA(sg), b = ckB(sg);
+        KPM_TRY(kpm_block_copy(&a, &ring[(k - 1) % 2]));
+        KPM_TRY(kpm_block_copy(&b, &ring[k % 2]));
+      }
+    }
+  }
+  return KPM_OK;
+}
+
+// reverse pass from g = dL/dmu on the kept forward vectors; grad in the CSC
+// order of H (scaled by 1/NR)
+static int adj_reverse(kpm_ctx* ctx, const kpm_csr* H, const AdjLayout& L, const double* g,
+                       double* grad) {
+  kpm_block v = adj_blk(ctx, L, 0);
+  auto beta = [&](int64_t j) { return adj_blk(ctx, L, 1 + j % L.nbr); };
+  auto alpha_full = [&](int64_t i) { return i == 0 ? v : adj_blk(ctx, L, 1 + L.nbr + i); };
+  auto ckA = [&](int64_t sg) { return adj_blk(ctx, L, 1 + L.nbr + 2 + 2 * sg); };
+  auto ckB = [&](int64_t sg) { return adj_blk(ctx, L, 1 + L.nbr + 3 + 2 * sg); };
+  auto seg = [&](int64_t i) { return adj_blk(ctx, L, 1 + L.nbr + 2 + 2 * L.nseg + i); };
+  const int chunks = sddmm_chunks(L.NR, L.prec);
+  const size_t nnz = (size_t)std::max<int64_t>(H->nnz, 1);
+  const size_t mbytes = nnz * chunks * sizeof(double2);
+  KPM_TRY(ensure(ctx, ctx->adj_grad, mbytes));
+  double2* M = (double2*)ctx->adj_grad.ptr;
+  KPM_HIP_TRY(hipMemsetAsync(M, 0, mbytes, ctx->stream));
+  // beta_NC = beta_{NC+1} = 0
+  KPM_HIP_TRY(hipMemsetAsync(beta(0).d, 0, L.bbytes * (size_t)L.nbr, ctx->stream));
+
+  // pending gradient terms (beta_j, alpha_{j-1}, c_j), added kWin at a time
+  kpm_block pb[kWin], pa[kWin];
+  const kpm_block* pbp[kWin];
+  const kpm_block* pap[kWin];
+  double pc[kWin];
+  int np = 0;
+  auto flush = [&]() -> int {
+    if (np == 0) return KPM_OK;
+    for (int w = 0; w < np; ++w) {
+      pbp[w] = &pb[w];
+      pap[w] = &pa[w];
+    }
+    const int r = launch_sddmm(ctx, H, pbp, pap, pc, np, M, nnz);
+    np = 0;
+    return r;
+  };
+  // beta_j = g_j v + 2 H beta_{j+1} - beta_{j+2}, then queue its term with alpha_{j-1}
+  auto reverse = [&](int64_t j, const kpm_block& a_prev) -> int {
+    kpm_block bx = beta(j + 1), bz = beta(j + 2), bo = beta(j);
+    KPM_TRY(adj_step(ctx, H, &bx, &bz, &bo, &v, 2.0, -1.0, g[j], MODE_SRC));
+    pb[np] = bo;
+    pa[np] = a_prev;
+    pc[np] = j == 1 ? 1.0 : 2.0;
+    if (++np == kWin) return flush();
+    return KPM_OK;
+  };
+  if (L.full) {
+    for (int64_t j = L.NC - 1; j >= 1; --j) KPM_TRY(reverse(j, alpha_full(j - 1)));
+    KPM_TRY(flush());
+  } else {
+    // one recomputed segment at a time; pending terms are flushed before the
+    // next segment overwrites the segment buffers
+    for (int64_t sg = L.nseg - 1; sg >= 0; --sg) {
+      const int64_t lo = sg * L.K, hi = std::min(lo + L.K - 1, L.nalpha - 1);
+      kpm_block s0 = seg(0), a = ckA(sg), b = ckB(sg);
+      KPM_TRY(kpm_block_copy(&s0, &a));
+      if (hi > lo) {
+        kpm_block s1 = seg(1);
+        KPM_TRY(kpm_block_copy(&s1, &b));
+      }
+      for (int64_t i = lo + 2; i <= hi; ++i) {
+        kpm_block si = seg(i - lo), sm1 = seg(i - lo - 1), sm2 = seg(i - lo - 2);
+        KPM_TRY(adj_step(ctx, H, &sm1, &sm2, &si, nullptr, 2.0, -1.0, 0.0, MODE_NONE));
+      }
+      for (int64_t i = hi; i >= lo; --i) KPM_TRY(reverse(i + 1, seg(i - lo)));
+      KPM_TRY(flush());
+    }
+  }
+  std::vector<double> m(2 * nnz * chunks);
+  KPM_HIP_TRY(hipMemcpyAsync(m.data(), M, mbytes, hipMemcpyDeviceToHost, ctx->stream));
+  ++ctx->counters[KPM_COUNT_SYNC];
+  KPM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  const double inv = 1.0 / (double)L.NR;
+  for (int64_t k = 0; k < H->nnz; ++k) {
+    const size_t p = (size_t)H->csc_to_csr[k];
+    double gr = 0.0, gi = 0.0;
+    for (int c = 0; c < chunks; ++c) {  // row chunks in order
+      gr += m[2 * (p + c * nnz)];
+      gi += m[2 * (p + c * nnz) + 1];
+    }
+    grad[2 * k] = gr * inv;
+    grad[2 * k + 1] = gi * inv;
+  }
+  return KPM_OK;
+}
+
+static int adj_check(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t NR, int64_t NH, int vec_prec,
+                     int64_t ckpt_stride) {
+  if (!ctx || !H) return fail(KPM_EINVAL, "null argument");
   if (NC < 2 || NR <= 0 || NH != H->n) return fail(KPM_EINVAL, "bad sizes");
   if (vec_prec != KPM_VEC_F64 && vec_prec != KPM_VEC_F32) return fail(KPM_EINVAL, "bad vec_prec");
   if (ckpt_stride < 0) return fail(KPM_EINVAL, "ckpt_stride must be >= 0");
   if ((int64_t)H->csc_to_csr.size() != H->nnz)
     return fail(KPM_EINVAL, "operator was not built from CSC arrays (kpm_csr_from_csc)");
+  return KPM_OK;
+}
+
+}  // namespace kpm
+
+using namespace kpm;
+
+extern "C" {
+
+int kpm_1d_adjoint_forward(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t NR, int64_t NH,
+                           const double* psi, double* mu_all, int vec_prec, int64_t ckpt_stride) {
+  KPM_TRY(adj_check(ctx, H, NC, NR, NH, vec_prec, ckpt_stride));
+  if (!psi || !mu_all) return fail(KPM_EINVAL, "null argument");
   KPM_HIP_TRY(hipSetDevice(ctx->device));
-  const int64_t nalpha = NC - 1;  // alpha_0 .. alpha_{NC-2} are used
-  const int64_t NRp = padded_nr(NR, vec_prec);
-  const size_t bbytes = (size_t)(NH * NRp) * elem_bytes(vec_prec);
-  const int64_t nbr = kWin + 2;   // beta ring: the recurrence's two plus a window
+  ctx->adj1.valid = false;
   // storage plan: every alpha when asked (stride 1) or when it fits in 60% of
-  // the free memory plus the workspace this context already holds (stride 0);
-  // checkpoint pairs every K steps otherwise
+  // the free memory plus the workspace this context already holds (stride
+  // 0); checkpoint pairs every K steps otherwise.  The workspace stays with
+  // the context (grow-only, released by kpm_ctx_destroy).
   size_t fr = 0, tot = 0;
   KPM_HIP_TRY(hipMemGetInfo(&fr, &tot));
-  const size_t avail = fr + ctx->adjoint.bytes;
-  const bool full = ckpt_stride == 1 ||
-                    (ckpt_stride == 0 && bbytes * (size_t)(1 + nbr + nalpha) <= avail / 10 * 6);
-  const int64_t K = full ? 1 : (ckpt_stride > 1 ? ckpt_stride : (int64_t)std::ceil(std::sqrt((double)NC)));
-  const int64_t nseg = full ? 0 : (nalpha + K - 1) / K;
-  // workspace: v | beta ring | (full) alpha_0..alpha_{nalpha-1}  or
-  //                           (checkpoints) forward ring, 2*nseg pairs, K segment
-  // It stays with the context (grow-only, released by kpm_ctx_destroy), so a
-  // loop of gradient steps allocates it once.
-  const int64_t nblk = full ? 1 + nbr + nalpha : 1 + nbr + 2 + 2 * nseg + K;
-  Buffer& ws = ctx->adjoint;
-  KPM_TRY(ensure(ctx, ws, bbytes * (size_t)nblk));
-  auto blk = [&](int64_t i) {
-    kpm_block b;
-    b.ctx = ctx;
-    b.NH = NH;
-    b.NR = NR;
-    b.NRp = NRp;
-    b.prec = vec_prec;
-    b.d = (char*)ws.ptr + bbytes * (size_t)i;
-    b.owned = false;
-    return b;
-  };
-  kpm_block v = blk(0);
-  auto beta = [&](int64_t j) { return blk(1 + j % nbr); };
-  auto alpha_full = [&](int64_t i) { return i == 0 ? v : blk(1 + nbr + i); };
-  auto ckA = [&](int64_t s) { return blk(1 + nbr + 2 + 2 * s); };
-  auto ckB = [&](int64_t s) { return blk(1 + nbr + 3 + 2 * s); };
-  auto seg = [&](int64_t i) { return blk(1 + nbr + 2 + 2 * nseg + i); };
-  Buffer& Mbuf = ctx->adj_grad;
-  int rc = KPM_OK;
-  do {
-    const size_t mbytes = (size_t)std::max<int64_t>(H->nnz, 1) * sizeof(double2);
-    if ((rc = ensure(ctx, Mbuf, mbytes)) != KPM_OK) break;
-    KPM_HIP_TRY(hipMemsetAsync(Mbuf.ptr, 0, mbytes, ctx->stream));
-    // v's padded columns and the beta ring (beta_NC = beta_{NC+1} = 0) start at zero
-    KPM_HIP_TRY(hipMemsetAsync(ws.ptr, 0, bbytes * (size_t)(1 + nbr), ctx->stream));
-    if ((rc = ensure(ctx, ctx->staging, (size_t)(NH * NR) * 16)) != KPM_OK) break;
-    KPM_HIP_TRY(hipMemcpyAsync(ctx->staging.ptr, psi, (size_t)(NH * NR) * 16, hipMemcpyHostToDevice, ctx->stream));
-    if ((rc = launch_upload(ctx, ctx->staging.ptr, &v)) != KPM_OK) break;
+  const AdjLayout L = adj_layout(ctx, NC, NR, NH, vec_prec, ckpt_stride, fr + ctx->adjoint.bytes);
+  int rc = adj_forward(ctx, H, L, psi);
+  if (rc == KPM_OK) {
+    hipError_t e = hipMemcpyAsync(mu_all, ctx->mu.ptr, (size_t)(NR * NC) * sizeof(double2),
+                                  hipMemcpyDeviceToHost, ctx->stream);
+    ++ctx->counters[KPM_COUNT_SYNC];
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) rc = fail(KPM_EHIP, std::string("moments download: ") + hipGetErrorString(e));
+  }
+  if (rc != KPM_OK) {
+    (void)hipStreamSynchronize(ctx->stream);
+    return rc;
+  }
+  ctx->adj1 = kpm_adj1_state{true, H->serial, NC, NR, NH, vec_prec, ckpt_stride,
+                             L.full ? 1 : L.K};
+  return KPM_OK;
+}
 
-    // pending gradient terms (beta_j, alpha_{j-1}, c_j), added kWin at a time
-    kpm_block pb[kWin], pa[kWin];
-    const kpm_block* pbp[kWin];
-    const kpm_block* pap[kWin];
-    double pc[kWin];
-    int np = 0;
-    auto flush = [&]() -> int {
-      if (np == 0) return KPM_OK;
-      for (int w = 0; w < np; ++w) {
-        pbp[w] = &pb[w];
-        pap[w] = &pa[w];
-      }
-      const int r = launch_sddmm(ctx, H, pbp, pap, pc, np, (double2*)Mbuf.ptr);
-      np = 0;
-      return r;
-    };
-    // one reverse step: beta_j = g_j v + 2 H beta_{j+1} - beta_{j+2}, then queue
-    // its term with alpha_{j-1}
-    auto reverse = [&](int64_t j, const kpm_block& a_prev) -> int {
-      kpm_block bx = beta(j + 1), bz = beta(j + 2), bo = beta(j);
-      KPM_TRY(adj_step(ctx, H, &bx, &bz, &bo, &v, 2.0, -1.0, g[j], MODE_SRC));
-      pb[np] = bo;
-      pa[np] = a_prev;
-      pc[np] = j == 1 ? 1.0 : 2.0;
-      if (++np == kWin) return flush();
-      return KPM_OK;
-    };
-
-    if (full) {
-      // forward: alpha_1 = H v, alpha_k = 2 H alpha_{k-1} - alpha_{k-2}, each kept
-      for (int64_t k = 1; k < nalpha && rc == KPM_OK; ++k) {
-        kpm_block x = alpha_full(k - 1), o = alpha_full(k);
-        if (k == 1) {
-          rc = adj_step(ctx, H, &x, nullptr, &o, nullptr, 1.0, 0.0, 0.0, MODE_NONE);
-        } else {
-          kpm_block z = alpha_full(k - 2);
-          rc = adj_step(ctx, H, &x, &z, &o, nullptr, 2.0, -1.0, 0.0, MODE_NONE);
-        }
-      }
-      for (int64_t j = NC - 1; j >= 1 && rc == KPM_OK; --j) rc = reverse(j, alpha_full(j - 1));
-      if (rc == KPM_OK) rc = flush();
-    } else {
-      // forward with checkpoint pairs (alpha_{sK}, alpha_{sK+1})
-      kpm_block ring[2] = {blk(1 + nbr), blk(2 + nbr)};
-      if ((rc = kpm_block_copy(&ring[0], &v)) != KPM_OK) break;
-      if ((rc = adj_step(ctx, H, &ring[0], nullptr, &ring[1], nullptr, 1.0, 0.0, 0.0, MODE_NONE)) != KPM_OK) break;
-      const int64_t klast = (nseg - 1) * K + 1;
-      for (int64_t k = 1; k <= klast && rc == KPM_OK; ++k) {
-        if (k >= 2)
-          rc = adj_step(ctx, H, &ring[(k - 1) % 2], &ring[k % 2], &ring[k % 2], nullptr, 2.0, -1.0, 0.0, MODE_NONE);
-        if (rc == KPM_OK && (k - 1) % K == 0) {  // pair (alpha_{k-1}, alpha_k) starts segment (k-1)/K
-          const int64_t sg = (k - 1) / K;
-          kpm_block a = ckA(sg), b = ckB(sg);
-          rc = kpm_block_copy(&a, &ring[(k - 1) % 2]);
-          if (rc == KPM_OK) rc = kpm_block_copy(&b, &ring[k % 2]);
-        }
-      }
-      // reverse, one recomputed segment at a time; pending terms are flushed
-      // before the next segment overwrites the segment buffers
-      for (int64_t sg = nseg - 1; sg >= 0 && rc == KPM_OK; --sg) {
-        const int64_t lo = sg * K, hi = std::min(lo + K - 1, nalpha - 1);
-        kpm_block s0 = seg(0), a = ckA(sg), b = ckB(sg);
-        rc = kpm_block_copy(&s0, &a);
-        if (rc == KPM_OK && hi > lo) {
-          kpm_block s1 = seg(1);
-          rc = kpm_block_copy(&s1, &b);
-        }
-        for (int64_t i = lo + 2; i <= hi && rc == KPM_OK; ++i) {
-          kpm_block si = seg(i - lo), sm1 = seg(i - lo - 1), sm2 = seg(i - lo - 2);
-          rc = adj_step(ctx, H, &sm1, &sm2, &si, nullptr, 2.0, -1.0, 0.0, MODE_NONE);
-        }
-        for (int64_t i = hi; i >= lo && rc == KPM_OK; --i) rc = reverse(i + 1, seg(i - lo));
-        if (rc == KPM_OK) rc = flush();
-      }
-    }
-    if (rc != KPM_OK) break;
-    std::vector<double> m(2 * std::max<int64_t>(H->nnz, 1));
-    KPM_HIP_TRY(hipMemcpyAsync(m.data(), Mbuf.ptr, mbytes, hipMemcpyDeviceToHost, ctx->stream));
-    KPM_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    const double inv = 1.0 / (double)NR;
-    for (int64_t k = 0; k < H->nnz; ++k) {
-      const int64_t p = H->csc_to_csr[k];
-      grad[2 * k] = m[2 * p] * inv;
-      grad[2 * k + 1] = m[2 * p + 1] * inv;
-    }
-  } while (false);
+int kpm_1d_adjoint_reverse(kpm_ctx* ctx, const kpm_csr* H, const double* g, double* grad) {
+  if (!ctx || !H || !g || !grad) return fail(KPM_EINVAL, "null argument");
+  const kpm_adj1_state& S = ctx->adj1;
+  if (!S.valid || S.H_serial != H->serial)
+    return fail(KPM_EINVAL, "no kept forward pass of this operator (kpm_1d_adjoint_forward)");
+  KPM_HIP_TRY(hipSetDevice(ctx->device));
+  // the layout the forward pass chose (its stride decision is kept)
+  const AdjLayout L = adj_layout(ctx, S.NC, S.NR, S.NH, S.prec, S.stride_used, 0);
+  const int rc = adj_reverse(ctx, H, L, g, grad);
   (void)hipStreamSynchronize(ctx->stream);
   return rc;
+}
+
+int kpm_1d_adjoint(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t NR, int64_t NH,
+                   const double* psi, const double* g, double* grad, int vec_prec,
+                   int64_t ckpt_stride) {
+  if (!g || !grad) return fail(KPM_EINVAL, "null argument");
+  std::vector<double> mu((size_t)(2 * std::max<int64_t>(NR, 0) * std::max<int64_t>(NC, 0)));
+  KPM_TRY(kpm_1d_adjoint_forward(ctx, H, NC, NR, NH, psi, mu.data(), vec_prec, ckpt_stride));
+  return kpm_1d_adjoint_reverse(ctx, H, g, grad);
 }
 
 // Host bookkeeping for the bond parametrisation H_ij = t_ij u_ij,
 // H_ji = t_ij conj(u_ij): for every stored pair i < j of a CSC pattern that
 // contains both (i, j) and (j, i), dL/dt_ij = Re(u M_ij + conj(u) M_ji) / a
-// with u = H_ij / |H_ij| (1 when H_ij = 0).  O(nnz) with one merge per column.
+// with u = H_ij / |H_ij| (1 when H_ij = 0).  O(nnz); columns sorted by row.
 int kpm_bond_gradient(int64_t n, const int64_t* colptr, const int64_t* rowval, const double* h,
                       const double* grad, double a, int64_t* bi, int64_t* bj, double* val,
                       int64_t* nbonds) {
   if (n <= 0 || !colptr || !rowval || !h || !grad || !bi || !bj || !val || !nbonds)
     return fail(KPM_EINVAL, "null argument");
   if (!(a > 0.0)) return fail(KPM_EINVAL, "a must be positive");
-  // position of (r, c) for every stored entry: search column r for row c
-  auto find = [&](int64_t col, int64_t row) -> int64_t {
-    const int64_t* b = rowval + colptr[col];
-    const int64_t* e = rowval + colptr[col + 1];
-    const int64_t* p = std::lower_bound(b, e, row);
-    return (p != e && *p == row) ? (int64_t)(p - rowval) : -1;
-  };
+  // Position of the transposed entry (c, r) of every stored (r, c): visiting
+  // the columns c in increasing order, the entries (c, r) of column r come up
+  // in increasing c, i.e. in column r's sorted order, so a cursor per column
+  // walks them (one O(nnz) pass, no search).
+  std::vector<int64_t> cur(colptr, colptr + n);
   int64_t m = 0;
   for (int64_t c = 0; c < n; ++c)
     for (int64_t k = colptr[c]; k < colptr[c + 1]; ++k) {
       const int64_t r = rowval[k];
+      if (r < 0 || r >= n) return fail(KPM_EINVAL, "row index out of range");
+      const int64_t kt = cur[r]++;  // entry (c, r)
+      if (kt >= colptr[r + 1] || rowval[kt] != c)
+        return fail(KPM_EINVAL,
+                    "pattern is not symmetric (or a column is unsorted): entry (j, i) missing");
       if (r >= c) continue;  // entry (r, c) with r < c: pair i = r, j = c
-      const int64_t kt = find(r, c);  // entry (c, r)
-      if (kt < 0) return fail(KPM_EINVAL, "pattern is not symmetric: entry (j, i) missing");
       const double hr = h[2 * k], hi = h[2 * k + 1];
-      const double mag = std::hypot(hr, hi);
-      const double ur = mag > 0.0 ? hr / mag : 1.0, ui = mag > 0.0 ? hi / mag : 0.0;
+      const double mag = std::sqrt(hr * hr + hi * hi);  // |H_ij| of a Hamiltonian entry
       const double gr = grad[2 * k], gi = grad[2 * k + 1];
       const double tr = grad[2 * kt], ti = grad[2 * kt + 1];
       bi[m] = r;
       bj[m] = c;
-      val[m] = (ur * (gr + tr) - ui * (gi - ti)) / a;
+      // Re(u (g + t) ...) with u = h / |h|, one division per bond
+      val[m] = mag > 0.0 ? (hr * (gr + tr) - hi * (gi - ti)) / (mag * a) : (gr + tr) / a;
       ++m;
     }
   *nbonds = m;

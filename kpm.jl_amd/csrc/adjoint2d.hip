@@ -137,13 +137,15 @@ int kpm_2d_adjoint(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, const kpm_
   // gradient accumulators (CSR order) and the two coefficient matrices
   const size_t nH = (size_t)std::max<int64_t>(H->nnz, 1), nA = (size_t)std::max<int64_t>(Ja->nnz, 1),
                nB = (size_t)std::max<int64_t>(Jb->nnz, 1);
-  const size_t gbytes = (nH + nA + nB) * sizeof(double2);
+  // rows wider than 64 words: one accumulator per row chunk, summed in order
+  const int chunks = sddmm_chunks(NR, prec);
+  const size_t gbytes = (nH + nA + nB) * chunks * sizeof(double2);
   const size_t cbytes = (size_t)(2 * NC * NC) * sizeof(double2);
   KPM_TRY(ensure(ctx, ctx->adj2_aux, gbytes + cbytes));
   double2* MH = (double2*)ctx->adj2_aux.ptr;
-  double2* MA = MH + nH;
-  double2* MB = MA + nA;
-  double2* CP = MB + nB;           // CP[m + NC n] = Gb[n, m]: P_n = sum_m CP[m, n] l_m
+  double2* MA = MH + nH * chunks;
+  double2* MB = MA + nA * chunks;
+  double2* CP = MB + nB * chunks;  // CP[m + NC n] = Gb[n, m]: P_n = sum_m CP[m, n] l_m
   double2* CQ = CP + NC * NC;      // CQ[n + NC m] = conj(Gb[n, m]): Q_m = sum_n CQ[n, m] r_n
   {
     std::vector<double> h(4 * (size_t)(NC * NC));
@@ -201,7 +203,7 @@ int kpm_2d_adjoint(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, const kpm_
       wbp[w] = &wb[w];
       wap[w] = &wa[w];
     }
-    const int r = launch_sddmm(ctx, A, wbp, wap, wc, nw, M);
+    const int r = launch_sddmm(ctx, A, wbp, wap, wc, nw, M, (size_t)std::max<int64_t>(A->nnz, 1));
     nw = 0;
     return r;
   };
@@ -253,21 +255,26 @@ int kpm_2d_adjoint(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, const kpm_
   KPM_TRY(reverse(ysrc, Ls));
 
   // to the host, scaled by 1/NR, in each operator's CSC order
-  std::vector<double> m(2 * (nH + nA + nB));
+  std::vector<double> m(2 * (nH + nA + nB) * chunks);
   KPM_HIP_TRY(hipMemcpyAsync(m.data(), MH, gbytes, hipMemcpyDeviceToHost, ctx->stream));
   ++ctx->counters[KPM_COUNT_SYNC];
   KPM_HIP_TRY(hipStreamSynchronize(ctx->stream));
   const double inv = 1.0 / (double)NR;
-  auto out = [&](const kpm_csr* A, size_t off, double* g) {
+  auto out = [&](const kpm_csr* A, size_t off, size_t n, double* g) {
     for (int64_t k = 0; k < A->nnz; ++k) {
       const size_t p = off + (size_t)A->csc_to_csr[k];
-      g[2 * k] = m[2 * p] * inv;
-      g[2 * k + 1] = m[2 * p + 1] * inv;
+      double gr = 0.0, gi = 0.0;
+      for (int c = 0; c < chunks; ++c) {
+        gr += m[2 * (p + c * n)];
+        gi += m[2 * (p + c * n) + 1];
+      }
+      g[2 * k] = gr * inv;
+      g[2 * k + 1] = gi * inv;
     }
   };
-  out(H, 0, grad_H);
-  out(Ja, nH, grad_Ja);
-  out(Jb, nH + nA, grad_Jb);
+  out(H, 0, nH, grad_H);
+  out(Ja, nH * chunks, nA, grad_Ja);
+  out(Jb, (nH + nA) * chunks, nB, grad_Jb);
   return KPM_OK;
 }
 

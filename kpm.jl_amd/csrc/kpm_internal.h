@@ -93,6 +93,16 @@ struct kpm_prim_state {
   bool pend_used[2] = {false, false};
 };
 
+// the forward pass kpm_1d_adjoint_forward keeps for kpm_1d_adjoint_reverse
+struct kpm_adj1_state {
+  bool valid = false;
+  uint64_t H_serial = 0;
+  int64_t NC = 0, NR = 0, NH = 0;
+  int prec = 0;
+  int64_t ckpt_req = 0;     // the caller's checkpoint stride
+  int64_t stride_used = 0;  // 1: every vector kept; K > 1: checkpoint pairs
+};
+
 struct kpm_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -131,6 +141,7 @@ struct kpm_ctx {
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
   kpm_prim_state prim;
+  kpm_adj1_state adj1;
 };
 
 // device moment accumulator (kpm_accum_*): NR x (NC + 1) double2,
@@ -313,10 +324,14 @@ int launch_contract(kpm_ctx* ctx, const void* Lslots, int nL, const void* Rslots
 // ---- reverse mode (adjoint.hip) ---------------------------------------------------
 // window of reverse steps whose gradient terms one SDDMM pass adds
 constexpr int kWin = 8;
-// M[k] += sum_{w < W} c[w] sum_r conj(beta_w[a, r]) alpha_w[colind[k], r] for
-// every stored entry k (row a) of A; W <= kWin
+// M[k] += sum_{w < W} c[w] sum_r conj(beta_w[a, r]) alpha_w[colind[k], r]
+// for every stored entry k (row a) of A; W <= kWin.  Rows wider than 64
+// words run in sddmm_chunks() row chunks, chunk c adding into
+// M + c * mstride (mstride >= nnz; the caller sums the chunks in order).
+int sddmm_chunks(int64_t NR, int prec);
 int launch_sddmm(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* const* beta,
-                 const kpm_block* const* alpha, const double* c, int W, double2* M);
+                 const kpm_block* const* alpha, const double* c, int W, double2* M,
+                 size_t mstride);
 // out = alpha*A*x + beta*z (+ gamma*src in MODE_SRC), no dots
 int adj_step(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* x, const kpm_block* z,
              kpm_block* out, const kpm_block* src, double alpha, double beta, double gamma, int mode);

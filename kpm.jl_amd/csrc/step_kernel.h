@@ -131,6 +131,30 @@ __device__ __forceinline__ int group_bcast(int v) {
     return __builtin_amdgcn_update_dpp(0, v, 0x150 + J, 0xF, 0xF, false);
   }
 }
+// Sum over each aligned group of L lanes (L = 1 .. 64), the same value in
+// every lane of the group: DPP row_mirror, row_half_mirror and the quad
+// exchanges [2,3,0,1], [1,0,3,2] within 16 lanes (each step adds a lane's
+// value to its partner's, so partners hold bitwise equal sums), then lane
+// exchanges for 32 and 64.  Every lane of the wave must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int L>
+__device__ __forceinline__ double group_sum(double v) {
+  static_assert(L == 1 || L == 2 || L == 4 || L == 8 || L == 16 || L == 32 || L == 64, "group");
+  if constexpr (L >= 16) v += dpp_move<0x140>(v);  // lane i + lane 15 - i of the row
+  if constexpr (L >= 8) v += dpp_move<0x141>(v);   // lane i + lane 7 - i of the half-row
+  if constexpr (L >= 4) v += dpp_move<0x4E>(v);    // quad [2, 3, 0, 1]
+  if constexpr (L >= 2) v += dpp_move<0xB1>(v);    // quad [1, 0, 3, 2]
+  if constexpr (L >= 32) v += __shfl_xor(v, 16);
+  if constexpr (L >= 64) v += __shfl_xor(v, 32);
+  return v;
+}
+
 template <int G, int J>
 __device__ __forceinline__ float group_bcast(float v) {
   return __int_as_float(group_bcast<G, J>(__float_as_int(v)));

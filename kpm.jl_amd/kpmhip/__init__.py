@@ -14,9 +14,9 @@ from ._lib import (ArgumentError, DomainError, KPMHipError, KPMHipMissing, KPMUn
 from .action import (chebyshev_action, chebyshev_action_, chern_marker, chern_marker_average,
                      chern_marker_region, evolution_coefficients, evolution_order,
                      evolution_tail, evolve, evolve_, fermi_coefficients, fermi_projector)
-from .adjoint import (bond_gradient, dos_loss, hopping_gradient, kpm_1d_adjoint, kpm_2d_adjoint,
-                      kubo_bastin_weights, moments_rrule, sigma_rrule,
-                      site_gradient)
+from .adjoint import (AdjointTape, bond_gradient, dos_loss, hopping_gradient, kpm_1d_adjoint,
+                      kpm_1d_adjoint_forward, kpm_2d_adjoint, kubo_bastin_weights, moments_rrule,
+                      sigma_rrule, site_gradient)
 from .applications import (Gamma_nm, Gamma_nm_mu, chebyshev_lin_trans, chebyshevT,
                            chebyshevT_0, chebyshevT_accurate, chebyshevT_xn, d_dc_cond, dc_cond0,
                            dc_cond_single, fermiFunction, fermiFunctions, gausschebyshevt)

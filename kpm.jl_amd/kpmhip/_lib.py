@@ -119,6 +119,8 @@ _SIGS = {
     "kpm_bond_gradient": (C.c_int, [_i64, _pi64, _pi64, _pd, _pd, C.c_double, _pi64, _pi64, _pd,
                                     _pi64]),
     "kpm_1d_adjoint": (C.c_int, [_p, _p, _i64, _i64, _i64, _pd, _pd, _pd, C.c_int, _i64]),
+    "kpm_1d_adjoint_forward": (C.c_int, [_p, _p, _i64, _i64, _i64, _pd, _pd, C.c_int, _i64]),
+    "kpm_1d_adjoint_reverse": (C.c_int, [_p, _p, _pd, _pd]),
     "kpm_2d_adjoint": (C.c_int, [_p, _p, _p, _p, _p, _i64, _i64, _i64, _pd, _pd, _pd, _pd, _pd,
                                  _pd]),
     "kpm_batch_create": (C.c_int, [_p, C.c_int, _pi64, _pi64, _pi64, _pd, C.c_int, C.POINTER(_p)]),

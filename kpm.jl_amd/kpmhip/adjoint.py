@@ -6,8 +6,10 @@ mu_n = (1/NR) sum_r <psi_r|T_n(H)|psi_r> (the moments of kpm_1d), given
 g = dL/dmu.  It runs libkpm_hip's reverse Chebyshev (Clenshaw) recurrence
 with sqrt(NC) checkpointing on the GPU.
 
-``moments_rrule`` packages it the way ChainRulesCore.rrule does for a Julia
-function: it returns the primal moments and a pullback closure g -> dL/dH.
+``kpm_1d_adjoint_forward`` / ``AdjointTape.pullback`` are its two halves;
+``moments_rrule`` packages them the way ChainRulesCore.rrule does for a
+Julia function: the primal moments and a pullback closure g -> dL/dH from
+one forward pass.
 ``dos_loss`` gives the L2 distance between the Jackson-smoothed DOS on an
 energy grid and a target, with its gradient in the moments.
 ``site_gradient`` and ``bond_gradient`` map dL/dH_norm to per-site
@@ -22,9 +24,8 @@ import numpy as np
 import scipy.sparse as sp
 
 from ._lib import ArgumentError, check, dptr
-from .device import get_context, prec_code, to_device
+from .device import DeviceCSR, get_context, prec_code, to_device
 from .kernels import JacksonKernel, hn
-from .moment import kpm_1d
 
 
 def _csc(H):
@@ -33,7 +34,8 @@ def _csc(H):
     return A
 
 
-def kpm_1d_adjoint(H, NC: int, psi, g, *, precision="f64", ckpt_stride: int = 0) -> sp.csc_matrix:
+def kpm_1d_adjoint(H, NC: int, psi, g, *, precision="f64", ckpt_stride: int = 0,
+                   operator=None) -> sp.csc_matrix:
     """dL/dH_ab on H's sparsity pattern (a CSC matrix with H's pattern):
     dL = sum_ab dH_ab * grad[a, b] for any perturbation dH on the pattern."""
     A = _csc(H)
@@ -44,22 +46,112 @@ def kpm_1d_adjoint(H, NC: int, psi, g, *, precision="f64", ckpt_stride: int = 0)
     if g.shape[0] != NC:
         raise ArgumentError(f"g must have NC={NC} entries (got {g.shape[0]})")
     ctx = get_context()
-    Hd = to_device(A, ctx)
+    Hd = _operator(A, operator, ctx)
     out = np.zeros(A.nnz, dtype=np.complex128)
+    ctx._adjoint_tape = None
     check(ctx.lib.kpm_1d_adjoint(ctx.handle, Hd.handle, NC, NR, NH, dptr(psi), dptr(g), dptr(out),
                                  prec_code(precision), int(ckpt_stride)), "kpm_1d_adjoint")
     return sp.csc_matrix((out, A.indices.copy(), A.indptr.copy()), shape=A.shape)
 
 
-def moments_rrule(H, NC: int, psi, *, precision="f64"):
-    """(mu, pullback): mu = probe-averaged kpm_1d moments (real, NC), and
-    pullback(g) = dL/dH on H's pattern for g = dL/dmu."""
-    mu = kpm_1d(H, NC, np.asarray(psi).shape[1], psi_in=psi, precision=precision)
+class AdjointTape:
+    """The forward half of kpm_1d_adjoint, kept on the device.
 
-    def pullback(g):
-        return kpm_1d_adjoint(H, NC, psi, g, precision=precision)
+    ``kpm_1d_adjoint_forward`` runs the forward recurrence once, keeps its
+    vectors (or sqrt(NC) checkpoints) in the context and returns this tape
+    with the moments of the same passes; ``pullback(g)`` runs the reverse
+    pass on them.  The context keeps one forward pass: a later
+    ``kpm_1d_adjoint_forward`` or ``kpm_1d_adjoint`` on it retires this tape,
+    and its pullback then raises instead of differentiating the wrong pass.
+    """
 
-    return mu, pullback
+    def __init__(self, ctx, Hd, A, NC, NR, mu_all):
+        self.ctx, self.Hd, self.A = ctx, Hd, A
+        self.NC, self.NR = NC, NR
+        self.mu_all = mu_all              # NR x NC, <psi_r|T_n(H)|psi_r>
+        self.mu = mu_all.real.mean(axis=0)  # the probe-averaged moments of kpm_1d
+        self.token = object()
+        ctx._adjoint_tape = self.token
+
+    def pullback(self, g) -> sp.csc_matrix:
+        """dL/dH on H's pattern for g = dL/dmu (NC reals)."""
+        g = np.ascontiguousarray(np.asarray(g, dtype=np.float64).reshape(-1))
+        if g.shape[0] != self.NC:
+            raise ArgumentError(f"g must have NC={self.NC} entries (got {g.shape[0]})")
+        if getattr(self.ctx, "_adjoint_tape", None) is not self.token:
+            raise ArgumentError("this forward pass was replaced by a later one on the same context")
+        out = np.zeros(self.A.nnz, dtype=np.complex128)
+        check(self.ctx.lib.kpm_1d_adjoint_reverse(self.ctx.handle, self.Hd.handle, dptr(g), dptr(out)),
+              "kpm_1d_adjoint_reverse")
+        return sp.csc_matrix((out, self.A.indices.copy(), self.A.indptr.copy()), shape=self.A.shape)
+
+
+def _operator(A, operator, ctx):
+    """The device operator of A: a caller's DeviceCSR built from the same
+    matrix (reused across calls), or a fresh upload."""
+    if operator is None:
+        return to_device(A, ctx)
+    if not isinstance(operator, DeviceCSR) or operator.adjoint:
+        raise ArgumentError("operator must be the DeviceCSR of H (to_device(H))")
+    if operator.shape != A.shape or operator.nnz != A.nnz or operator.ctx is not ctx:
+        raise ArgumentError("operator does not match H (shape, stored entries or context)")
+    return operator
+
+
+def kpm_1d_adjoint_forward(H, NC: int, psi, *, precision="f64", ckpt_stride: int = 0,
+                           operator=None) -> AdjointTape:
+    """Forward half of kpm_1d_adjoint: one recurrence pass that keeps what the
+    reverse pass needs and returns the moments of the same passes (tape.mu,
+    tape.mu_all).  ``operator`` may pass H's DeviceCSR (to_device(H)) to skip
+    the per-call upload; it must come from the same matrix."""
+    A = _csc(H)
+    NC = int(NC)
+    if NC < 2:
+        raise ArgumentError(f"NC must be >= 2 (got {NC})")
+    psi = np.asfortranarray(np.asarray(psi).reshape(A.shape[0], -1), dtype=np.complex128)
+    NH, NR = psi.shape
+    ctx = get_context()
+    Hd = _operator(A, operator, ctx)
+    mu_all = np.zeros((NC, NR), dtype=np.complex128)  # column-major NR x NC
+    ctx._adjoint_tape = None
+    check(ctx.lib.kpm_1d_adjoint_forward(ctx.handle, Hd.handle, NC, NR, NH, dptr(psi), dptr(mu_all),
+                                         prec_code(precision), int(ckpt_stride)),
+          "kpm_1d_adjoint_forward")
+    return AdjointTape(ctx, Hd, A, NC, NR, np.ascontiguousarray(mu_all.T))
+
+
+def moments_rrule(H, NC: int, psi, *, precision="f64", ckpt_stride: int = 0, operator=None):
+    """(mu, pullback): mu = probe-averaged equal-vector moments (real, NC,
+    the moments of kpm_1d to rounding), and pullback(g) = dL/dH on H's
+    pattern for g = dL/dmu.  The primal and the pullback share one forward
+    pass."""
+    tape = kpm_1d_adjoint_forward(H, NC, psi, precision=precision, ckpt_stride=ckpt_stride,
+                                  operator=operator)
+    return tape.mu, tape.pullback
+
+
+_DOS_CACHE: dict = {}
+
+
+def _dos_matrix(NC, a, b, E, kernel):
+    """D[k, n] = h_n g_n T_n(x_k) / (a pi sqrt(1 - x_k^2)), x = (E - b)/a; the
+    last few (NC, a, b, E, kernel) are kept, since a training loop asks for the
+    same grid every step."""
+    key = (NC, a, b, E.tobytes(), kernel)
+    D = _DOS_CACHE.get(key)
+    if D is None:
+        x = (E - b) / a
+        if np.any(np.abs(x) >= 1):
+            raise ArgumentError("energies must lie strictly inside the rescaled band")
+        ns = np.arange(NC)
+        w = np.asarray(kernel(ns, NC), dtype=np.float64) * hn(ns)
+        T = np.cos(ns[None, :] * np.arccos(x)[:, None])           # (nE, NC)
+        D = T * w[None, :] / (a * math.pi * np.sqrt(1 - x * x))[:, None]
+        D.setflags(write=False)
+        if len(_DOS_CACHE) >= 4:
+            _DOS_CACHE.pop(next(iter(_DOS_CACHE)))
+        _DOS_CACHE[key] = D
+    return D
 
 
 def dos_loss(mu, a: float, b: float, E, rho_target, kernel=JacksonKernel):
@@ -67,15 +159,8 @@ def dos_loss(mu, a: float, b: float, E, rho_target, kernel=JacksonKernel):
     rho(E) = sum_n h_n g_n mu_n T_n(x) / (a pi sqrt(1-x^2)), x = (E-b)/a
     (the reconstruction of src/applications/dos.jl:76-149); returns (L, dL/dmu)."""
     mu = np.asarray(mu, dtype=np.float64)
-    NC = len(mu)
     E = np.asarray(E, dtype=np.float64)
-    x = (E - b) / a
-    if np.any(np.abs(x) >= 1):
-        raise ArgumentError("energies must lie strictly inside the rescaled band")
-    ns = np.arange(NC)
-    w = np.asarray(kernel(ns, NC), dtype=np.float64) * hn(ns)
-    T = np.cos(ns[None, :] * np.arccos(x)[:, None])           # (nE, NC)
-    D = T * w[None, :] / (a * math.pi * np.sqrt(1 - x * x))[:, None]
+    D = _dos_matrix(len(mu), float(a), float(b), E, kernel)
     rho = D @ mu
     r = rho - np.asarray(rho_target, dtype=np.float64)
     return float(np.sum(r * r)), 2.0 * (D.T @ r)

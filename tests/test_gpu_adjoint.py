@@ -139,3 +139,71 @@ def test_adjoint_ragged_rows_vs_tangent(w, NR):
         _, dmu = oracle.kpm_1d_tangent(A, dH, NC, psi)
         tang = float(np.sum(g * np.real(dmu.mean(axis=0))))
         assert abs(pred - tang) <= 1e-10 * max(1.0, abs(tang)), (seed, pred, tang)
+
+
+@pytest.mark.parametrize("ckpt", [0, 1, 5])
+@pytest.mark.parametrize("NR", [3, 65])
+def test_tape_moments_and_pullback(ckpt, NR):
+    """The split adjoint: the forward half's moments are the oracle's
+    per-probe moments <psi_r|T_n(H)|psi_r> of every order (to 1e-12), and its
+    pullback is bitwise the one-call kpm_1d_adjoint (same passes, same
+    summation order)."""
+    H = model(L=6, seed=8)
+    NC = 30
+    psi = K.random_phase_vectors(Xoshiro256pp(70 + NR), H.shape[0], NR)
+    print(f"adjoint tape NR={NR} ckpt={ckpt}", flush=True)
+    tape = K.kpm_1d_adjoint_forward(H, NC, psi, ckpt_stride=ckpt)
+    want = oracle.kpm_1d(H, NC, psi)
+    assert tape.mu_all.shape == (NR, NC)
+    assert np.max(np.abs(tape.mu_all - want)) <= 1e-12
+    np.testing.assert_allclose(tape.mu, K.kpm_1d(H, NC, NR, psi_in=psi), rtol=0, atol=1e-12)
+    g = Xoshiro256pp(9).rand(NC) - 0.5
+    G1 = tape.pullback(g)
+    G2 = K.kpm_1d_adjoint(H, NC, psi, g, ckpt_stride=ckpt)
+    assert np.array_equal(G1.data, G2.data)
+    # the one-call adjoint replaced the kept pass: the old tape refuses
+    with pytest.raises(K.ArgumentError):
+        tape.pullback(g)
+
+
+def test_tape_pullback_is_repeatable_and_checks_operator():
+    H = model(L=6, seed=8)
+    NC, NR = 20, 4
+    psi = K.random_phase_vectors(Xoshiro256pp(3), H.shape[0], NR)
+    mu, pb = K.moments_rrule(H, NC, psi)
+    g1 = Xoshiro256pp(1).rand(NC) - 0.5
+    g2 = Xoshiro256pp(2).rand(NC) - 0.5
+    a1, b, a2 = pb(g1), pb(g2), pb(g1)  # several cotangents on one forward pass
+    assert np.array_equal(a1.data, a2.data)
+    # linear in g
+    ab = pb(g1 + g2)
+    np.testing.assert_allclose(ab.data, a1.data + b.data, rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("NR", [65, 128])
+def test_adjoint_bitwise_repeatable_wide_rows(NR):
+    """Rows of two and four 64-word chunks: per-chunk gradient accumulators
+    summed in a fixed order, so two runs agree bitwise."""
+    H = ragged_hermitian(300, 9, seed=49)
+    NC = 24
+    psi = K.random_phase_vectors(Xoshiro256pp(80 + NR), H.shape[0], NR)
+    g = Xoshiro256pp(81).rand(NC) - 0.5
+    print(f"adjoint repeatability NR={NR}", flush=True)
+    G1 = K.kpm_1d_adjoint(H, NC, psi, g)
+    G2 = K.kpm_1d_adjoint(H, NC, psi, g)
+    assert np.array_equal(G1.data, G2.data)
+
+
+def test_tape_with_resident_operator():
+    """A DeviceCSR passed as operator gives bitwise the gradient of a fresh
+    upload; a mismatched one is refused."""
+    H = model(L=6, seed=8)
+    NC, NR = 20, 4
+    psi = K.random_phase_vectors(Xoshiro256pp(3), H.shape[0], NR)
+    g = Xoshiro256pp(1).rand(NC) - 0.5
+    Hd = K.to_device(H)
+    a = K.kpm_1d_adjoint_forward(H, NC, psi, operator=Hd).pullback(g)
+    b = K.kpm_1d_adjoint(H, NC, psi, g)
+    assert np.array_equal(a.data, b.data)
+    with pytest.raises(K.ArgumentError):
+        K.kpm_1d_adjoint_forward(H, NC, psi, operator=K.to_device(model(L=5, seed=8)))

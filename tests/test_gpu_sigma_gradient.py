@@ -123,3 +123,18 @@ def test_sigma_xy_gradient_vs_finite_differences():
               sigma_oracle(H - h * dH, Jx - h * dX, Jy - h * dY, a, Ef, NC, psi, area)) / (2 * h)
         print(f"  bond ({i},{j}) |t|={mag:.2f}: adjoint {bonds.data[k]:.10e} fd {fd:.10e}", flush=True)
         assert abs(bonds.data[k] - fd) <= 1e-5 * abs(fd)
+
+
+def test_kpm2d_adjoint_bitwise_repeatable_wide_rows():
+    """NR = 65: rows of two 64-word chunks, one gradient accumulator per
+    chunk summed in a fixed order, so two runs agree bitwise."""
+    H, Jx, Jy, area = models.haldane_model(6, 6, t2=0.2)
+    Hn = (H / 4.4).tocsc()
+    NC, NR = 16, 65
+    pl, pr = phases(40, H.shape[0], NR), phases(41, H.shape[0], NR)
+    gbar = (Xoshiro256pp(42).rand(NC, NC) - 0.5) + 1j * (Xoshiro256pp(43).rand(NC, NC) - 0.5)
+    print(f"kpm_2d adjoint repeatability NR={NR}", flush=True)
+    a = K.kpm_2d_adjoint(Hn, Jx, Jy, NC, pl, pr, gbar)
+    b = K.kpm_2d_adjoint(Hn, Jx, Jy, NC, pl, pr, gbar)
+    for x, y in zip(a, b):
+        assert np.array_equal(sp.csc_matrix(x).data, sp.csc_matrix(y).data)

@@ -298,3 +298,17 @@ def test_bond_gradient_bookkeeping():
     assert got.nnz == ref.nnz and abs(got - ref).max() < 1e-15
     with pytest.raises(K.ArgumentError):  # gradient on a different pattern
         K.bond_gradient(H, sp.identity(A.shape[0], format="csc"), 2.5)
+
+
+def test_bond_gradient_rejects_asymmetric_pattern():
+    """A stored (i, j) without (j, i) is refused: the transposed entry is
+    looked up by the per-column cursor and checked."""
+    H = models.square_disordered(5, seed=4).tolil()
+    H[0, 7] = 0.3  # (7, 0) absent
+    H = sp.csc_matrix(H)
+    A = sp.csc_matrix(H, dtype=np.complex128)
+    A.sort_indices()
+    G = sp.csc_matrix((np.ones(A.nnz, dtype=np.complex128), A.indices.copy(), A.indptr.copy()),
+                      shape=A.shape)
+    with pytest.raises(K.ArgumentError, match="not symmetric"):
+        K.bond_gradient(H, G, 1.0)
