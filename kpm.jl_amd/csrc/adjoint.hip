@@ -147,7 +147,7 @@ __global__ void __launch_bounds__(kBlock)
     const int rc = live ? row : 0;
     T bw[kWin];
 #pragma unroll
-    for (int w = 0; w < kWin; ++w) bw[w] = w < win.W ? win.beta[w][(size_t)rc * RW + sub] : T{};
+    for (int w = 0; w < kWin; ++w) bw[w] = w < win.W ? ld_stream(&win.beta[w][(size_t)rc * RW + sub]) : T{};
     const int s0 = rowptr[rc];
     const int n = live ? rowptr[rc + 1] - s0 : 0;
     int nmax = n;
@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(kBlock)
       const bool v = idx < n;
       const int k = v ? s0 + idx : 0;
       cl[e] = v ? colind[k] : 0;
-      m[e] = v ? M[k] : make_double2(0.0, 0.0);
+      m[e] = v ? ld_stream(&M[k]) : make_double2(0.0, 0.0);
     }
     double br[kWin][CPL], bi[kWin][CPL];
 #pragma unroll
@@ -204,8 +204,26 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int e = 0; e < ME; ++e) {
       const int idx = e * MG + j;
-      if (owner && idx < n) M[s0 + idx] = m[e];
+      if (owner && idx < n) st_stream(&M[s0 + idx], m[e]);
     }
+  }
+}
+
+// out[k] = inv * sum_c M[perm[k] + c * mstride] (row chunks in order): the
+// gradient of the k-th stored entry of the caller's CSC arrays.
+__global__ void __launch_bounds__(kBlock)
+    k_csc_gradient(const double2* __restrict__ M, const int* __restrict__ perm, int chunks,
+                   size_t mstride, double inv, double2* __restrict__ out, int64_t nnz) {
+  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < nnz;
+       k += (int64_t)gridDim.x * kBlock) {
+    const size_t p = (size_t)perm[k];
+    double gr = 0.0, gi = 0.0;
+    for (int c = 0; c < chunks; ++c) {
+      const double2 m = M[p + c * mstride];
+      gr += m.x;
+      gi += m.y;
+    }
+    out[k] = make_double2(gr * inv, gi * inv);
   }
 }
 
@@ -470,21 +488,31 @@ static int adj_reverse(kpm_ctx* ctx, const kpm_csr* H, const AdjLayout& L, const
       KPM_TRY(flush());
     }
   }
-  std::vector<double> m(2 * nnz * chunks);
-  KPM_HIP_TRY(hipMemcpyAsync(m.data(), M, mbytes, hipMemcpyDeviceToHost, ctx->stream));
+  // CSC order on the device, then one download into the caller's array
+  if (H->nnz > 0) {
+    if (!H->csc_to_csr_dev) {
+      int* d = nullptr;
+      KPM_HIP_TRY(hipMalloc(&d, (size_t)H->nnz * sizeof(int)));
+      hipError_t e = hipMemcpyAsync(d, H->csc_to_csr.data(), (size_t)H->nnz * sizeof(int),
+                                    hipMemcpyHostToDevice, ctx->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess) {
+        (void)hipFree(d);
+        return fail(KPM_EHIP, std::string("gradient permutation upload: ") + hipGetErrorString(e));
+      }
+      const_cast<kpm_csr*>(H)->csc_to_csr_dev = d;
+    }
+    KPM_TRY(ensure(ctx, ctx->staging, (size_t)H->nnz * sizeof(double2)));
+    const int64_t gb = std::min<int64_t>((H->nnz + kBlock - 1) / kBlock, 65535);
+    hipLaunchKernelGGL(k_csc_gradient, dim3((unsigned)gb), dim3(kBlock), 0, ctx->stream, M,
+                       H->csc_to_csr_dev, chunks, nnz, 1.0 / (double)L.NR,
+                       (double2*)ctx->staging.ptr, H->nnz);
+    KPM_HIP_TRY(hipGetLastError());
+    KPM_HIP_TRY(hipMemcpyAsync(grad, ctx->staging.ptr, (size_t)H->nnz * sizeof(double2),
+                               hipMemcpyDeviceToHost, ctx->stream));
+  }
   ++ctx->counters[KPM_COUNT_SYNC];
   KPM_HIP_TRY(hipStreamSynchronize(ctx->stream));
-  const double inv = 1.0 / (double)L.NR;
-  for (int64_t k = 0; k < H->nnz; ++k) {
-    const size_t p = (size_t)H->csc_to_csr[k];
-    double gr = 0.0, gi = 0.0;
-    for (int c = 0; c < chunks; ++c) {  // row chunks in order
-      gr += m[2 * (p + c * nnz)];
-      gi += m[2 * (p + c * nnz) + 1];
-    }
-    grad[2 * k] = gr * inv;
-    grad[2 * k + 1] = gi * inv;
-  }
   return KPM_OK;
 }
 

@@ -157,6 +157,7 @@ struct kpm_csr {
   // position in this CSR of the k-th stored entry of the CSC arrays it was
   // built from (the adjoint returns gradients in the caller's CSC order)
   std::vector<int> csc_to_csr;
+  int* csc_to_csr_dev = nullptr;  // device copy, uploaded by the first adjoint that needs it
   int64_t n = 0;
   int64_t nnz = 0;
   int is_real = 0;

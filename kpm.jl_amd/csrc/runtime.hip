@@ -433,6 +433,7 @@ int kpm_csr_destroy(kpm_csr* A) {
   (void)hipFree(A->rowptr);
   (void)hipFree(A->colind);
   (void)hipFree(A->vals);
+  if (A->csc_to_csr_dev) (void)hipFree(A->csc_to_csr_dev);
   delete A;
   return KPM_OK;
 }
