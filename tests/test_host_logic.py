@@ -17,23 +17,25 @@ from kpmhip.rng import Xoshiro256pp, splitmix64_stream
 # ---- probe generator known answers ------------------------------------------------
 
 def test_splitmix64_known_answer():
-    # splitmix64 seeded with 0: first output 0xE220A8397B1DCDAF (Vigna's reference)
-    assert int(splitmix64_stream(0, 1)[0]) == 0xE220A8397B1DCDAF
+    # splitmix64 seeded with 0: the first outputs of Vigna's reference
+    # splitmix64.c (also the known-answer vector of rand_xoshiro's SplitMix64)
+    assert [int(x) for x in splitmix64_stream(0, 3)] == [
+        0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
 
 
 def test_xoshiro256pp_known_answer():
-    # state (1, 2, 3, 4): first output rotl(1 + 4, 23) + 1 = 41943041
+    # state (1, 2, 3, 4): the first ten outputs of Vigna's reference
+    # xoshiro256plusplus.c, as published in rand_xoshiro's known-answer test
+    # (the first is rotl(1 + 4, 23) + 1 = 41943041)
     st = np.array([[1], [2], [3], [4]], dtype=np.uint64)
     g = Xoshiro256pp(state=st)
-    assert int(g.next_u64()[0]) == (5 << 23) + 1
-    # second output by hand-stepping the generator once
-    s0, s1, s2, s3 = 1, 2, 3, 4
-    t = (s1 << 17) & 0xFFFFFFFFFFFFFFFF
-    s2 ^= s0; s3 ^= s1; s1 ^= s2; s0 ^= s3; s2 ^= t  # noqa: E702
-    s3 = ((s3 << 45) | (s3 >> 19)) & 0xFFFFFFFFFFFFFFFF
-    r = (((s0 + s3) & 0xFFFFFFFFFFFFFFFF) << 23 | ((s0 + s3) & 0xFFFFFFFFFFFFFFFF) >> 41)
-    r = (r + s0) & 0xFFFFFFFFFFFFFFFF
-    assert int(g.next_u64()[0]) == r
+    want = [41943041, 58720359, 3588806011781223, 3591011842654386, 9228616714210784205,
+            9973669472204895162, 14011001112246962877, 12406186145184390807,
+            15849039046786891736, 10450023813501588000]
+    assert [int(g.next_u64()[0]) for _ in range(10)] == want
+    # every lane runs its own copy of the generator
+    g2 = Xoshiro256pp(state=np.array([[1, 7], [2, 7], [3, 7], [4, 7]], dtype=np.uint64))
+    assert [int(g2.next_u64()[0]) for _ in range(10)] == want
 
 
 def test_random_phase_vectors_reproducible_and_unit():
