@@ -435,10 +435,11 @@ def bench_adjoint(args) -> int:
     here) and dots each order with the probes (the moments), the loss, the
     reverse Chebyshev gradient on the kept vectors, and the per-site /
     per-bond gradients, host H in.  Algorithmic bytes (FP64, per step):
-    NC-1 forward passes (x, z and the probe block read, out written), NC-1
-    reverse passes (x, z and the source read, out written) and the gradient
-    terms: beta and alpha read once each per reverse step, the nnz gradient
-    read and written once per 8-step window."""
+    NC-1 forward passes (x and z read, out written; the moment dots of the
+    first NC/2 use the same vectors), NC-1 reverse passes (x, z and the source
+    read, out written) and the gradient terms: beta and alpha read once each
+    per reverse step, the nnz gradient read and written once per 8-step
+    window."""
     import numpy as np
 
     import kpmhip as K
@@ -485,7 +486,7 @@ def bench_adjoint(args) -> int:
     vec = NH * nrp * 16
     hb = nnz * (8 + 4) + (NH + 1) * 4
     windows = -(-(NC - 1) // 8)
-    total = ((NC - 1) * (hb + 4 * vec) + (NC - 1) * (hb + 4 * vec) + (NC - 1) * 2 * vec
+    total = ((NC - 1) * (hb + 3 * vec) + (NC - 1) * (hb + 4 * vec) + (NC - 1) * 2 * vec
              + windows * (nnz * (4 + 32) + (NH + 1) * 4))
     out = {"metric": "DOS-loss gradient step time (forward + reverse Chebyshev)",
            "value": round(wall * 1e3, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps,
