@@ -5,11 +5,13 @@ graphene flake, 1024 x 1024 honeycomb cells x 2 orbitals (NH = 2,097,152,
 nnz = 8,388,608 stored: 3 bonds per site each way + the disorder diagonal),
 t = 1, on-site disorder U[-0.5, 0.5], rescaled by a fixed a = 3.6; NC = 2048,
 NR = 64 unit random-phase probes.  One step = one full kpm_1d call (NC/2 =
-1024 fused recurrence passes) with the probe block already resident in HBM.
-The metric is moments/s = NC * NR * nnz / seconds, for the FP32-recurrence
-form (ComplexF32 vectors, FP64 arithmetic and FP64 moment accumulation); the
-plain FP64 form is timed beside it and the max moment error of FP32 against
-FP64 is reported.
+1024 fused recurrence passes) with the probe block already resident in HBM,
+returning what the DOS uses: the NC probe-averaged moments (kpm_1d's default
+avg_output = true, as dos_moments calls it).  The metric is moments/s =
+NC * NR * nnz / seconds, for the FP32-recurrence form (ComplexF32 vectors,
+FP32 recurrence arithmetic, FP64 moment accumulation); the plain FP64 form is
+timed beside it and the max moment error of FP32 against FP64 is reported,
+and so is one call returning the NR x NC per-probe table (avg_output = false).
 
 Prints one JSON line.
 """
@@ -133,27 +135,41 @@ def main() -> int:
             tuned = K.autotune(Hd, NR, precision)
         print(f"[bench] launch shape {precision}: {tuned}", file=sys.stderr, flush=True)
         blk = K.DeviceBlock.from_host(psi, precision)
-        mu = np.zeros((NR, NC), dtype=np.complex128, order="F")
+        # the DOS call: kpm_1d(H, NC, NR; psi_in) with its default avg_output =
+        # true (src/moment.jl:339-418, dos_moments at src/frontend.jl:254-256),
+        # i.e. the NC probe-averaged moments
+        mu = np.zeros(NC, dtype=np.complex128)
         lib = ctx.lib
         for _ in range(args.warmup):
-            K._lib.check(lib.kpm_1d_block(ctx.handle, Hd.handle, NC, blk.handle,
-                                          K._lib.dptr(mu)), "kpm_1d_block")
+            K._lib.check(lib.kpm_1d_block_avg(ctx.handle, Hd.handle, NC, blk.handle,
+                                              K._lib.dptr(mu)), "kpm_1d_block_avg")
         ctx.synchronize()
         t0 = time.perf_counter()
         ctx.event_record(0)
         for _ in range(args.steps):
-            K._lib.check(lib.kpm_1d_block(ctx.handle, Hd.handle, NC, blk.handle,
-                                          K._lib.dptr(mu)), "kpm_1d_block")
+            K._lib.check(lib.kpm_1d_block_avg(ctx.handle, Hd.handle, NC, blk.handle,
+                                              K._lib.dptr(mu)), "kpm_1d_block_avg")
         ctx.event_record(1)
         ctx.synchronize()
         wall = time.perf_counter() - t0
         ev_ms = ctx.event_elapsed(0, 1)
-        # the dominant kernel alone: back-to-back fused passes on the same stream
+        # the per-probe table (avg_output = false): one warm call, timed
+        mu_all = np.zeros((NR, NC), dtype=np.complex128, order="F")
+        K._lib.check(lib.kpm_1d_block(ctx.handle, Hd.handle, NC, blk.handle,
+                                      K._lib.dptr(mu_all)), "kpm_1d_block")
+        ctx.synchronize()
+        tp = time.perf_counter()
+        K._lib.check(lib.kpm_1d_block(ctx.handle, Hd.handle, NC, blk.handle,
+                                      K._lib.dptr(mu_all)), "kpm_1d_block")
+        per_probe_s = time.perf_counter() - tp
+        # the dominant kernel alone: back-to-back fused probe-summed passes on
+        # the same stream
         x = K.DeviceBlock.from_host(psi, precision)
         z = K.DeviceBlock.from_host(psi, precision)
         ms = ctypes.c_float(0.0)
-        K._lib.check(lib.kpm_bench_step(ctx.handle, Hd.handle, x.handle, z.handle,
-                                        args.kernel_launches, ctypes.byref(ms)), "bench_step")
+        K._lib.check(lib.kpm_bench_step_mode(ctx.handle, Hd.handle, x.handle, z.handle,
+                                             args.kernel_launches, 1, ctypes.byref(ms)),
+                     "bench_step")
         cms = ctypes.c_float(0.0)
         K._lib.check(lib.kpm_bench_copy(ctx.handle, z.handle, x.handle, 20, ctypes.byref(cms)),
                      "bench_copy")
@@ -163,14 +179,14 @@ def main() -> int:
         del x, z, blk
         tuned["copy_kernel_gbs"] = round(copy_gbs, 1)
         return (mu.copy(), wall / args.steps, ev_ms / 1e3 / args.steps,
-                ms.value / args.kernel_launches, tuned)
+                ms.value / args.kernel_launches, tuned, mu_all.copy(), per_probe_s)
 
     print("[bench] FP32-recurrence leg ...", file=sys.stderr, flush=True)
-    mu32, wall32, ev32, kms32, tune32 = run_leg("f32")
+    mu32, wall32, ev32, kms32, tune32, all32, pp32 = run_leg("f32")
     mu64 = None
     if not args.skip_fp64:
         print("[bench] FP64 leg ...", file=sys.stderr, flush=True)
-        mu64, wall64, ev64, kms64, tune64 = run_leg("f64")
+        mu64, wall64, ev64, kms64, tune64, all64, pp64 = run_leg("f64")
 
     units = NC * NR * nnz
     value = units / wall32
@@ -184,7 +200,7 @@ def main() -> int:
     achieved32 = b32 / (call_ms32 * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved32, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved32 / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_step<FP32,LPR=32,MODE_DBL>", "bytes_per_launch": b32,
+                "kernel": "k_step<FP32,LPR=32,UNR=2,KB=4,MODE_DBLA>", "bytes_per_launch": b32,
                 "avg_launch_ms": round(call_ms32, 5),
                 "launches_per_step": NC // 2,
                 "kernel_only_avg_launch_ms": round(kms32, 5),
@@ -195,12 +211,18 @@ def main() -> int:
         with open(tfile) as f:
             roofline["traffic"] = json.load(f).get("hbm_bytes_per_launch")
 
-    extra = {"ms_per_step_event": round(ev32 * 1e3, 3)}
+    extra = {"ms_per_step_event": round(ev32 * 1e3, 3),
+             # the same FP32 recurrence returning the NR x NC per-probe table
+             # (kpm_1d avg_output = false, per-workgroup partials of NR columns)
+             "per_probe_value": units / pp32,
+             "per_probe_ms_per_step": round(pp32 * 1e3, 3),
+             "per_probe_frac": round(b32 / (pp32 / (NC // 2)) / 1e9 / HBM_PEAK_GBS, 4),
+             "max_abs_avg_vs_per_probe_mean": float(np.max(np.abs(mu32 - all32.mean(axis=0))))}
     if mu64 is not None:
         nrp64 = K.moment._padded_nr(NR, 0)
         b64 = step_bytes(NH, nnz, nrp64, 16, val_bytes)
-        avg32 = mu32.mean(axis=0)
-        avg64 = mu64.mean(axis=0)
+        avg32 = mu32
+        avg64 = mu64
         extra.update({
             "fp64_value": units / wall64,
             "fp64_ms_per_step": round(wall64 * 1e3, 3),
@@ -210,7 +232,8 @@ def main() -> int:
             "fp64_kernel_only_frac": round(b64 / (kms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "fp64_tuned": tune64,
             "max_abs_dmu_fp32_vs_fp64": float(np.max(np.abs(avg32 - avg64))),
-            "max_abs_dmu_per_probe": float(np.max(np.abs(mu32 - mu64))),
+            "max_abs_dmu_per_probe": float(np.max(np.abs(all32 - all64))),
+            "fp64_per_probe_ms_per_step": round(pp64 * 1e3, 3),
         })
 
     cpu = None
@@ -243,7 +266,7 @@ def main() -> int:
                "nproc": os.cpu_count(), "cpus_available": avail, "cpu_model": cpu_model()}
 
     out = {
-        "metric": "moments/s (NC*NR*nnz/s), FP32-recurrence kpm_1d",
+        "metric": "moments/s (NC*NR*nnz/s), FP32-recurrence kpm_1d DOS",
         "value": value,
         "unit": "moments/s",
         "n_gpus": 1,
@@ -256,7 +279,8 @@ def main() -> int:
         "dtype": "c32-storage/f64-accumulate",
         "data": "synthetic",
         "config": {"workload": f"graphene {L}x{L}x2 DOS kpm_1d", "NH": NH, "nnz": nnz, "NC": NC,
-                   "NR": NR, "disorder": "U[-0.5,0.5]", "a": 3.6},
+                   "NR": NR, "disorder": "U[-0.5,0.5]", "a": 3.6,
+                   "output": "NC probe-averaged moments (kpm_1d avg_output=true, as dos_moments)"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         **extra,

@@ -196,6 +196,16 @@ int kpm_1d_block(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, const kpm_block* ps
                  double* mu_all);
 int kpm_1d_lr_block(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, const kpm_block* psi_l,
                     const kpm_block* psi_r, double* mu_all);
+/* kpm_1d with avg_output = true on equal vectors (src/moment.jl:339-418 with
+ * :410-416): mu_avg (NC ComplexF64) = sum(mu_all, dims = 1) ./ NR, the moments
+ * dos_moments stores (src/frontend.jl:254-256).  The doubling passes sum the
+ * moment dots over the probe columns as they go (a few bytes of partials per
+ * wave instead of NR columns per workgroup), so mu_all is never formed.
+ * Same arguments and errors as kpm_1d / kpm_1d_block otherwise. */
+int kpm_1d_avg(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t NR, int64_t NH,
+               const double* psi, double* mu_avg, int vec_prec);
+int kpm_1d_block_avg(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, const kpm_block* psi,
+                     double* mu_avg);
 /* No counterpart in the reference: for callers that cannot allocate device
  * memory while they run (code under HIP graph capture, serving loops).
  * kpm_1d_block with every buffer supplied: psi (read), the two recurrence
@@ -406,6 +416,10 @@ int kpm_batch_graph_builds(const kpm_batch* B, int64_t* builds);
  * The output is not meaningful (timing only). */
 int kpm_bench_step(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z, int64_t nsteps,
                    float* ms);
+/* kpm_bench_step for either doubling pass: mode 0 the per-probe pass of
+ * kpm_1d (MODE_DBL), mode 1 the probe-summed pass of kpm_1d_avg. */
+int kpm_bench_step_mode(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z,
+                        int64_t nsteps, int mode, float* ms);
 /* Streaming copy src -> dst (same shape) `reps` times on the context
  * stream, bracketed by HIP events: the plain copy kernel of known size that
  * calibrates the HBM byte counters and measures the bandwidth ceiling. */
@@ -429,7 +443,9 @@ int kpm_event_elapsed(kpm_ctx* ctx, int slot_a, int slot_b, float* ms);
  * persistent grid of 2..16 workgroups per CU or a one-shot grid of 2..64
  * row tiles per workgroup) on the caller's operator and blocks X, Z (Z is
  * overwritten), records the fastest for this shape and returns it
- * (best_grid: the workgroups it launches). */
+ * (best_grid: the workgroups it launches).  The probe-summed doubling pass
+ * is timed over the same candidates and recorded on its own
+ * (kpm_tune_get_mode). */
 int kpm_autotune_step(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z, int reps,
                       int* best_unroll, int* best_grid, float* best_ms_per_step);
 /* Set the table entry for a shape directly (unroll 1 or 2; grid 0 = auto). */
@@ -443,6 +459,12 @@ int kpm_tune_set_chunked(kpm_ctx* ctx, const kpm_csr* H, int64_t NR, int vec_pre
 /* The launch shape the engines use for a shape (chunk 0: persistent grid). */
 int kpm_tune_get(kpm_ctx* ctx, const kpm_csr* H, int64_t NR, int vec_prec, int* unroll, int* grid,
                  int* chunk, int* xcd_grouped);
+/* The same for either pass the tuner times separately: mode 0 the
+ * per-probe passes (kpm_tune_get), mode 1 the probe-summed doubling pass of
+ * kpm_1d_avg / kpm_1d_block_avg.  kpm_tune_set and kpm_tune_set_chunked set
+ * both. */
+int kpm_tune_get_mode(kpm_ctx* ctx, const kpm_csr* H, int64_t NR, int vec_prec, int mode,
+                      int* unroll, int* grid, int* chunk, int* xcd_grouped);
 
 #ifdef __cplusplus
 }

@@ -18,7 +18,7 @@ struct DotSteps {
   int64_t NR, NRp;
   double2* half[2];
   int k = 0;
-  int max_grid = 0;          // workgroups each half holds (0: kMaxGridPerCU * CUs)
+  int max_grid = 0;          // partial rows each half holds (0: kMaxGridPerCU * CUs)
   double2* l2buf[2] = {nullptr, nullptr};  // slice sums of the two-level reduction
   double2* mu_dev = nullptr;  // null: ctx->mu
   // reductions still owed: a final one (into the moment columns) and a
@@ -45,6 +45,8 @@ struct DotSteps {
   int init(int nd, const kpm_block* x, int mode) {
     max_grid = cfg ? partials_grid_cfg(ctx, H, x->NH, x->NR, x->prec, *cfg)
                    : partials_grid(ctx, H, x->NH, x->NR, x->prec, mode);
+    // MODE_DBLA writes one (single-column) partial row per wave and row chunk
+    if (mode == MODE_DBLA) max_grid *= kStepWaves * step_geom(H, x->NR, x->prec, StepConfig{}).chunks;
     const size_t per = (size_t)max_grid * nd * NRp;
     const size_t l2per = (size_t)((max_grid + kSlice - 1) / kSlice) * nd * NRp;
     KPM_TRY(ensure(ctx, ctx->partials, (2 * per + 2 * l2per) * sizeof(double2)));

@@ -59,15 +59,16 @@ static int upload_into(kpm_ctx* ctx, const double* host, kpm_block* b) {
 }
 
 int time_engine_steps(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z,
-                      const StepConfig& cfg, int reps, float* ms) {
+                      const StepConfig& cfg, int reps, float* ms, int mode) {
+  const bool avg = mode == MODE_DBLA;
   KPM_TRY(ensure(ctx, ctx->mu, (size_t)(X->NR * 2) * sizeof(double2)));
-  DotSteps ds{ctx, H, X->NR, X->NRp, {nullptr, nullptr}};
+  DotSteps ds{ctx, H, avg ? 1 : X->NR, avg ? 1 : X->NRp, {nullptr, nullptr}};
   ds.cfg = &cfg;
-  KPM_TRY(ds.init(2, X, MODE_DBL));
+  KPM_TRY(ds.init(2, X, mode));
   const Combine c{2, {0, 1}, {2.0, 2.0}, {-1, -1}, {0.0, 0.0}};
-  for (int r = 0; r < 2; ++r) KPM_TRY(ds.step(X, Z, Z, nullptr, 2.0, -1.0, MODE_DBL, c));
+  for (int r = 0; r < 2; ++r) KPM_TRY(ds.step(X, Z, Z, nullptr, 2.0, -1.0, mode, c));
   KPM_HIP_TRY(hipEventRecord(ctx->events[12], ctx->stream));
-  for (int r = 0; r < reps; ++r) KPM_TRY(ds.step(X, Z, Z, nullptr, 2.0, -1.0, MODE_DBL, c));
+  for (int r = 0; r < reps; ++r) KPM_TRY(ds.step(X, Z, Z, nullptr, 2.0, -1.0, mode, c));
   KPM_TRY(ds.finish());
   KPM_HIP_TRY(hipEventRecord(ctx->events[13], ctx->stream));
   KPM_HIP_TRY(hipEventSynchronize(ctx->events[13]));
@@ -108,6 +109,37 @@ static int run_1d_doubling(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, kpm_block
                            kpm_block* w1) {
   KPM_TRY(mu_buffer(ctx, w0->NR, NC));
   return run_1d_doubling_into(ctx, H, NC, w0, w1, ctx->mu.ptr, nullptr, nullptr, 0);
+}
+
+// The doubling recurrence of run_1d_doubling_into with every dot summed over
+// the probe columns in the pass itself (MODE_DBLA): ctx->mu receives the
+// 1 x NC probe sums sum_r mu[r, n], the numerator of kpm_1d's avg_output
+// (src/moment.jl:410-416).  The doubling combination is linear, so it holds
+// for the probe sums as for each probe.
+static int run_1d_doubling_avg(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, kpm_block* w0,
+                               kpm_block* w1) {
+  KPM_TRY(mu_buffer(ctx, 1, NC));
+  DotSteps ds{ctx, H, 1, 1, {nullptr, nullptr}};
+  KPM_TRY(ds.init(2, w0, MODE_DBLA));
+  kpm_block* v[2] = {w0, w1};
+  Combine c0{2, {0, 1}, {1.0, 1.0}, {-1, -1}, {0.0, 0.0}};
+  KPM_TRY(ds.step(v[0], nullptr, v[1], nullptr, 1.0, 0.0, MODE_DBLA, c0));
+  int ip = 1, ipp = 0;
+  for (int64_t n = 2; n <= NC / 2; ++n) {
+    Combine c{2, {(int)(2 * n - 2), (int)(2 * n - 1)}, {2.0, 2.0}, {0, 1}, {-1.0, -1.0}};
+    KPM_TRY(ds.step(v[ip], v[ipp], v[ipp], nullptr, 2.0, -1.0, MODE_DBLA, c));
+    ip = 1 - ip;
+    ipp = 1 - ipp;
+  }
+  return ds.finish();
+}
+
+// probe sums -> probe averages (src/moment.jl:411: sum(mu_all, dims = 1) ./ NR)
+static int avg_to_host(kpm_ctx* ctx, double* host, int64_t NC, int64_t NR) {
+  KPM_TRY(mu_to_host(ctx, host, NC));
+  const double inv = 1.0 / (double)NR;
+  for (int64_t i = 0; i < 2 * NC; ++i) host[i] *= inv;
+  return KPM_OK;
 }
 
 // ---- bra/ket and current: one dot per step against l ------------------------
@@ -166,6 +198,35 @@ int kpm_1d_block(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, const kpm_block* ps
   KPM_TRY(kpm_block_copy(&w0, psi));
   KPM_TRY(run_1d_doubling(ctx, H, NC, &w0, &w1));
   return mu_to_host(ctx, mu_all, psi->NR * NC);
+}
+
+int kpm_1d_avg(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t NR, int64_t NH,
+               const double* psi, double* mu_avg, int vec_prec) {
+  if (!ctx || !H || !psi || !mu_avg) return fail(KPM_EINVAL, "null argument");
+  if (NC % 2 != 0 || NC < 2) return fail(KPM_EINVAL, "Invalid NC: NC should be even.");
+  if (NR <= 0 || NH != H->n) return fail(KPM_EINVAL, "Invalid `psi_in` size: expecting (NH, NR)");
+  if (vec_prec != KPM_VEC_F64 && vec_prec != KPM_VEC_F32) return fail(KPM_EINVAL, "bad vec_prec");
+  KPM_HIP_TRY(hipSetDevice(ctx->device));
+  kpm_block w0, w1;
+  KPM_TRY(work_block(ctx, ctx->work[0], NH, NR, vec_prec, &w0));
+  KPM_TRY(work_block(ctx, ctx->work[1], NH, NR, vec_prec, &w1));
+  KPM_TRY(upload_into(ctx, psi, &w0));
+  KPM_TRY(run_1d_doubling_avg(ctx, H, NC, &w0, &w1));
+  return avg_to_host(ctx, mu_avg, NC, NR);
+}
+
+int kpm_1d_block_avg(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, const kpm_block* psi,
+                     double* mu_avg) {
+  if (!ctx || !H || !psi || !mu_avg) return fail(KPM_EINVAL, "null argument");
+  if (NC % 2 != 0 || NC < 2) return fail(KPM_EINVAL, "Invalid NC: NC should be even.");
+  if (psi->NH != H->n) return fail(KPM_EINVAL, "probe block rows do not match H");
+  KPM_HIP_TRY(hipSetDevice(ctx->device));
+  kpm_block w0, w1;
+  KPM_TRY(work_block(ctx, ctx->work[0], psi->NH, psi->NR, psi->prec, &w0));
+  KPM_TRY(work_block(ctx, ctx->work[1], psi->NH, psi->NR, psi->prec, &w1));
+  KPM_TRY(kpm_block_copy(&w0, psi));
+  KPM_TRY(run_1d_doubling_avg(ctx, H, NC, &w0, &w1));
+  return avg_to_host(ctx, mu_avg, NC, psi->NR);
 }
 
 int kpm_1d_lr(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, int64_t NR, int64_t NH,
@@ -456,13 +517,19 @@ int kpm_accum_download(kpm_ctx* ctx, const kpm_accum* acc, double* host) {
 
 int kpm_bench_step(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z, int64_t nsteps,
                    float* ms) {
-  if (!ctx || !ms || nsteps < 1) return fail(KPM_EINVAL, "bad arguments");
+  return kpm_bench_step_mode(ctx, H, X, Z, nsteps, 0, ms);
+}
+
+int kpm_bench_step_mode(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z,
+                        int64_t nsteps, int mode, float* ms) {
+  if (!ctx || !ms || nsteps < 1 || mode < 0 || mode > 1) return fail(KPM_EINVAL, "bad arguments");
   KPM_TRY(check_block_pair(H, X, Z));
   KPM_HIP_TRY(hipSetDevice(ctx->device));
   ++ctx->epoch;
+  const int m = mode ? MODE_DBLA : MODE_DBL;
   KPM_HIP_TRY(hipEventRecord(ctx->events[14], ctx->stream));
   for (int64_t s = 0; s < nsteps; ++s)
-    KPM_TRY(step(ctx, H, X, Z, Z, nullptr, 2.0, -1.0, MODE_DBL, nullptr));
+    KPM_TRY(step(ctx, H, X, Z, Z, nullptr, 2.0, -1.0, m, nullptr));
   KPM_HIP_TRY(hipEventRecord(ctx->events[15], ctx->stream));
   KPM_HIP_TRY(hipEventSynchronize(ctx->events[15]));
   KPM_HIP_TRY(hipEventElapsedTime(ms, ctx->events[14], ctx->events[15]));

@@ -204,8 +204,13 @@ int work_block(kpm_ctx* ctx, Buffer& buf, int64_t NH, int64_t NR, int prec, kpm_
 // MODE_SRC: out += gamma * l (no dots), the reverse Clenshaw step of the adjoint
 // MODE_DUAL: also out2 = gamma * B * x from the same gathers, where A and B
 // share one (merged) pattern and B's complex values are vals2 (no dots)
+// MODE_DBLA: MODE_DBL with the dots summed over the probe columns as well
+// (the probe-averaged moments of kpm_1d(...; avg_output = true)); one pair of
+// partials per wave, [wave][d], so its partials count kBlock/64 per
+// workgroup and row chunk
+constexpr int kStepWaves = 4;  // waves per workgroup of the fused pass
 enum StepMode { MODE_NONE = 0, MODE_DBL = 1, MODE_LR = 2, MODE_DBL_SEG = 3, MODE_SRC = 4,
-                MODE_DUAL = 5 };
+                MODE_DUAL = 5, MODE_DBLA = 6 };
 
 // One fused pass out <- alpha*A*x + beta*z with optional dot partials:
 //   MODE_DBL: d0 = <x|x>, d1 = <x|out>;  MODE_LR: d0 = <l|out>.
@@ -302,7 +307,7 @@ int partials_grid_cfg(kpm_ctx* ctx, const kpm_csr* A, int64_t NH, int64_t NR, in
 // deferred reductions) under launch shape cfg on blocks X (v_n) and Z
 // (v_{n-1}, overwritten), after two untimed steps: what the tuner compares
 int time_engine_steps(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z,
-                      const StepConfig& cfg, int reps, float* ms);
+                      const StepConfig& cfg, int reps, float* ms, int mode = MODE_DBL);
 // layout conversions between the host column-major ComplexF64 block (in the
 // staging buffer, already on the device) and a device recurrence block
 int launch_upload(kpm_ctx* ctx, const void* staging, kpm_block* b);
@@ -352,5 +357,7 @@ int run_1d_doubling_into(kpm_ctx* ctx, const kpm_csr* H, int64_t NC, kpm_block* 
 // tuner table (tune.hip): best StepConfig for a shape, or defaults; record one
 StepConfig tuned_config(kpm_ctx* ctx, const kpm_csr* A, int64_t NR, int prec, int mode);
 void tune_record(kpm_ctx* ctx, const kpm_csr* A, int64_t NR, int prec, const StepConfig& c);
+// the same for the probe-summed doubling pass (MODE_DBLA) alone
+void tune_record_avg(kpm_ctx* ctx, const kpm_csr* A, int64_t NR, int prec, const StepConfig& c);
 
 }  // namespace kpm

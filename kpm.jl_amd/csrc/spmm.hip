@@ -206,9 +206,15 @@ int launch_step(kpm_ctx* ctx, const StepLaunch& s, const StepConfig& cfg, int* g
   const int64_t ntiles = (x->NH + g.TR - 1) / g.TR;
   if (ntiles > INT32_MAX) return fail(KPM_EINVAL, "row count too large for the recurrence kernel");
   int G = cfg.grid > 0 ? (cfg.grid + 7) / 8 * 8 : default_grid(ctx, ntiles);
-  const int nd = s.mode == MODE_DBL ? 2 : (s.mode == MODE_LR ? 1 : 0);
+  const bool avg = s.mode == MODE_DBLA;
+  const int nd = (s.mode == MODE_DBL || avg) ? 2 : (s.mode == MODE_LR ? 1 : 0);
+  // partial rows per workgroup: one per wave and row chunk for MODE_DBLA
+  // (one column each), one per workgroup (NRp columns) otherwise
+  const int upw = avg ? kStepWaves * g.chunks : 1;
+  const int64_t pcols = avg ? 1 : x->NRp;
   double2* part = s.part_out;
-  const int cap = s.max_grid > 0 ? s.max_grid : kMaxGridPerCU * std::max(ctx->num_cus, 1);
+  // s.max_grid counts partial rows; in workgroups:
+  const int cap = (s.max_grid > 0 ? s.max_grid : kMaxGridPerCU * std::max(ctx->num_cus, 1) * upw) / upw;
   // one-shot schedule: one workgroup per chunk of tiles (a multiple of 8
   // workgroups), unless the caller's partials buffer cannot hold them
   int chunk = 0;
@@ -219,10 +225,12 @@ int launch_step(kpm_ctx* ctx, const StepLaunch& s, const StepConfig& cfg, int* g
       chunk = cfg.chunk;
     }
   }
-  if (part && chunk == 0) {  // caller-owned buffer sized for `cap` workgroups
-    G = std::max(8, std::min(G, cap / 8 * 8));
+  if (part) {
+    // caller-owned buffer sized for `cap` workgroups: a persistent grid is
+    // clamped to it, a one-shot grid was only taken above if it fits
+    if (chunk == 0) G = std::max(8, std::min(G, cap / 8 * 8));
   } else {
-    if (nd > 0) KPM_TRY(ensure(ctx, ctx->partials, (size_t)G * nd * x->NRp * sizeof(double2)));
+    if (nd > 0) KPM_TRY(ensure(ctx, ctx->partials, (size_t)G * upw * nd * pcols * sizeof(double2)));
     part = (double2*)ctx->partials.ptr;
   }
   if ((s.mode == MODE_LR || s.mode == MODE_SRC) && !s.l) return fail(KPM_EINVAL, "missing operand block");
@@ -246,7 +254,7 @@ int launch_step(kpm_ctx* ctx, const StepLaunch& s, const StepConfig& cfg, int* g
   KPM_HIP_TRY(hipGetLastError());
   ++ctx->counters[KPM_COUNT_STEP];
   if (s.mode == MODE_DUAL) ++ctx->counters[KPM_COUNT_DUAL];
-  if (grid_used) *grid_used = G;
+  if (grid_used) *grid_used = G * upw;  // partial rows written
   return KPM_OK;
 }
 

@@ -9,6 +9,8 @@
 //
 //   out[i,:] = alpha * sum_k A[i,k] x[k,:] + beta * z[i,:]
 //   MODE_DBL: d0[:] += conj(x[i,:]) x[i,:],  d1[:] += conj(x[i,:]) out[i,:]
+//   MODE_DBLA: the MODE_DBL dots summed over the probe columns too (the
+//             probe-averaged moments), one partial pair per wave
 //   MODE_LR : d0[:] += conj(l[i,:]) out[i,:]
 //   MODE_SRC: out[i,:] += gamma * l[i,:], no dots (the adjoint's source term)
 // z may alias out (in place); when beta == 0 the launcher passes x as z and
@@ -99,7 +101,7 @@ struct Word<KPM_VEC_F32> {
   __device__ static inline T zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
 };
 
-constexpr int kBlock = 256;  // 4 waves
+constexpr int kBlock = 64 * kStepWaves;  // 4 waves
 
 // compile-time loop: f(std::integral_constant<int, I>) for I in [B, E)
 template <int B, int E, class F>
@@ -233,7 +235,11 @@ __device__ inline void combine_unit(const double2* __restrict__ partials, int G,
   const size_t gs = (size_t)nd * NRp;  // between consecutive workgroups
   const double2* __restrict__ p = partials + (size_t)d * NRp + (live ? col : 0);
   double sr = 0.0, si = 0.0;
-  constexpr int kBatch = 16;
+  // loads in flight per thread: kept small because the fused pass carries
+  // this code, and its register allocation is the maximum over the kernel
+  // (16 in flight took the FP32 graphene pass from 107 to 148 VGPRs, 4 to 3
+  // waves per SIMD)
+  constexpr int kBatch = 4;
   int g = lane;
   for (; g + (kBatch - 1) * kCombRows < G; g += kBatch * kCombRows) {
     double2 v[kBatch];
@@ -295,7 +301,7 @@ __device__ inline void reduce1_unit(const double2* __restrict__ partials, int G,
   const double2* __restrict__ p = partials + (size_t)d * NRp + (live ? col : 0);
   const int g1 = min(G, (slice + 1) * kSlice);
   double sr = 0.0, si = 0.0;
-  constexpr int kBatch = 8;
+  constexpr int kBatch = 4;
   int g = slice * kSlice + lane;
   for (; g + (kBatch - 1) * kCombRows < g1; g += kBatch * kCombRows) {
     double2 v[kBatch];
@@ -379,7 +385,9 @@ __global__ void __launch_bounds__(kBlock)
   constexpr int CPL = WT::CPL;
   constexpr int RPW = 64 / LPR;
   constexpr int TR = 4 * RPW * UNR;
-  constexpr int ND = MODE == MODE_DBL ? 2 : (MODE == MODE_LR ? 1 : 0);
+  constexpr bool kAvg = MODE == MODE_DBLA;
+  constexpr bool kDbl = MODE == MODE_DBL || kAvg;
+  constexpr int ND = kDbl ? 2 : (MODE == MODE_LR ? 1 : 0);
   constexpr int NDA = ND > 0 ? ND : 1;
   // rows of 2 or more lanes take their nonzeros spread over the lanes
   constexpr int MG = LPR >= 16 ? 16 : LPR;
@@ -538,9 +546,11 @@ __global__ void __launch_bounds__(kBlock)
   };
 
   // fixed-order workgroup reduction of the per-lane dot sums into dst
-  __shared__ double2 red[ND > 0 ? kBlock * NDA * CPL : 1];
+  // (MODE_DBLA uses it only as the pending reductions' kBlock-entry scratch)
+  __shared__ double2 red[ND > 0 ? (kAvg ? kBlock : kBlock * NDA * CPL) : 1];
   // (dot d, column col) goes to dst[(d * NRp + col) * stride]
   auto flush = [&](double2* dst, size_t stride) {
+    if constexpr (kAvg) return;  // per-wave partials instead (below)
 #pragma unroll
     for (int d = 0; d < NDA; ++d)
 #pragma unroll
@@ -635,7 +645,7 @@ __global__ void __launch_bounds__(kBlock)
       const int rc = R0.row[u] < NH ? R0.row[u] : 0;
       const size_t off = (size_t)rc * RW + wcol;
       zv[u] = ld_stream(z + off);  // z is always a valid block; unused when beta == 0
-      if constexpr (MODE == MODE_DBL) sv[u] = x[off];
+      if constexpr (kDbl) sv[u] = x[off];
       if constexpr (MODE == MODE_LR || MODE == MODE_SRC) sv[u] = l[off];
     }
     // 2. prefetch: column indices of the next tile, row pointers of the one after
@@ -715,7 +725,7 @@ __global__ void __launch_bounds__(kBlock)
         }
         if (live) st_stream(out2 + (size_t)R0.row[u] * o2.rw + o2col, WT::pack_a(pr, pi));
       }
-      if constexpr (MODE == MODE_DBL || MODE == MODE_LR) {
+      if constexpr (kDbl || MODE == MODE_LR) {
         double sr[CPL], si[CPL], ar[CPL], ai[CPL];
         WT::unpack(ow, sr, si);
         WT::unpack(sv[u], ar, ai);
@@ -725,7 +735,7 @@ __global__ void __launch_bounds__(kBlock)
           ar[c] *= w;
           ai[c] *= w;
         }
-        if constexpr (MODE == MODE_DBL) {
+        if constexpr (kDbl) {
 #pragma unroll
           for (int c = 0; c < CPL; ++c) {
             accr[0][c] = fma(ar[c], ar[c], fma(ai[c], ai[c], accr[0][c]));
@@ -758,8 +768,30 @@ __global__ void __launch_bounds__(kBlock)
     }
   }
 
-  // per-workgroup partials [b][d][col]: each workgroup writes whole lines
-  if constexpr (!SEG && ND > 0) flush(partials + (size_t)b * ND * NRp, 1);
+  if constexpr (kAvg) {
+    // probe-summed dots: the wave's rows and columns summed with DPP moves in
+    // a fixed order, one partial pair per wave at [wave id][d] (no LDS
+    // round trip and no barrier; 32 bytes per wave instead of 32 * NRp per
+    // workgroup)
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      s0 += accr[0][c];
+      s1 += accr[1][c];
+      s2 += acci[1][c];
+    }
+    s0 = group_sum<64>(s0);
+    s1 = group_sum<64>(s1);
+    s2 = group_sum<64>(s2);
+    const size_t wid = ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * (kBlock / 64) + wave;
+    if (lane == 0) {
+      partials[wid * 2] = make_double2(s0, 0.0);
+      partials[wid * 2 + 1] = make_double2(s1, s2);
+    }
+  } else if constexpr (!SEG && ND > 0) {
+    // per-workgroup partials [b][d][col]: each workgroup writes whole lines
+    flush(partials + (size_t)b * ND * NRp, 1);
+  }
 }
 
 // ---- launcher tables -------------------------------------------------------------
@@ -787,6 +819,7 @@ template <int PREC, int LPR, int UNR, int KB, bool REALH>
 static StepLauncher pick_mode(int mode) {
   switch (mode) {
     case MODE_DBL: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_DBL, false>;
+    case MODE_DBLA: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_DBLA, false>;
     case MODE_LR: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_LR, false>;
     case MODE_DBL_SEG: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_DBL, true>;
     case MODE_SRC: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_SRC, false>;

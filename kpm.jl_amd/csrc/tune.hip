@@ -39,6 +39,11 @@ void tune_record(kpm_ctx* ctx, const kpm_csr* A, int64_t NR, int prec, const Ste
   for (int mode : {MODE_NONE, MODE_DBL, MODE_LR}) g_table[make_key(ctx, A, NR, prec, mode)] = c;
 }
 
+void tune_record_avg(kpm_ctx* ctx, const kpm_csr* A, int64_t NR, int prec, const StepConfig& c) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_table[make_key(ctx, A, NR, prec, MODE_DBLA)] = c;
+}
+
 }  // namespace kpm
 
 using namespace kpm;
@@ -85,20 +90,30 @@ int kpm_autotune_step(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z
       }
     }
   }
-  float best = 1e30f;
-  StepConfig bestc;
   // each candidate runs as the engine runs it: fused passes with the
-  // deferred moment reductions their grid implies
-  for (const StepConfig& c : cand) {
-    float ms = 0.f;
-    KPM_TRY(time_engine_steps(ctx, H, X, Z, c, reps, &ms));
-    ms /= (float)reps;
-    if (ms < best) {
-      best = ms;
-      bestc = c;
+  // deferred moment reductions their grid implies.  The per-probe pass
+  // (MODE_DBL, whose shape the other modes share) and the probe-summed pass
+  // (MODE_DBLA: a few bytes of partials per wave instead of NRp per
+  // workgroup, so it favours shorter chunks) are tuned separately.
+  auto best_of = [&](int mode, StepConfig* bc, float* bms) -> int {
+    *bms = 1e30f;
+    for (const StepConfig& c : cand) {
+      float ms = 0.f;
+      KPM_TRY(time_engine_steps(ctx, H, X, Z, c, reps, &ms, mode));
+      ms /= (float)reps;
+      if (ms < *bms) {
+        *bms = ms;
+        *bc = c;
+      }
     }
-  }
+    return KPM_OK;
+  };
+  float best = 0.f, best_avg = 0.f;
+  StepConfig bestc, bestc_avg;
+  KPM_TRY(best_of(MODE_DBL, &bestc, &best));
+  KPM_TRY(best_of(MODE_DBLA, &bestc_avg, &best_avg));
   tune_record(ctx, H, X->NR, X->prec, bestc);
+  tune_record_avg(ctx, H, X->NR, X->prec, bestc_avg);
   if (best_unroll) *best_unroll = bestc.unroll;
   if (best_grid) {  // the workgroup count the winner launches with
     const StepGeom g = step_geom(H, X->NR, X->prec, bestc);
@@ -121,10 +136,23 @@ int kpm_tune_set_chunked(kpm_ctx* ctx, const kpm_csr* H, int64_t NR, int vec_pre
   c.chunk = chunk;
   c.xmap = xcd_grouped ? 1 : 0;
   tune_record(ctx, H, NR, vec_prec, c);
+  tune_record_avg(ctx, H, NR, vec_prec, c);
   return KPM_OK;
 }
 
-// The table entry the engines use for a shape (defaults when untuned).
+// The table entry the engines use for a shape (defaults when untuned):
+// mode 0 the per-probe passes, mode 1 the probe-summed doubling pass.
+int kpm_tune_get_mode(kpm_ctx* ctx, const kpm_csr* H, int64_t NR, int vec_prec, int mode,
+                      int* unroll, int* grid, int* chunk, int* xcd_grouped) {
+  if (!ctx || !H || NR <= 0 || mode < 0 || mode > 1) return fail(KPM_EINVAL, "bad arguments");
+  const StepConfig c = tuned_config(ctx, H, NR, vec_prec, mode ? MODE_DBLA : MODE_DBL);
+  if (unroll) *unroll = c.unroll;
+  if (grid) *grid = c.grid;
+  if (chunk) *chunk = c.chunk;
+  if (xcd_grouped) *xcd_grouped = c.xmap;
+  return KPM_OK;
+}
+
 int kpm_tune_get(kpm_ctx* ctx, const kpm_csr* H, int64_t NR, int vec_prec, int* unroll, int* grid,
                  int* chunk, int* xcd_grouped) {
   if (!ctx || !H || NR <= 0) return fail(KPM_EINVAL, "bad arguments");
@@ -145,6 +173,7 @@ int kpm_tune_set(kpm_ctx* ctx, const kpm_csr* H, int64_t NR, int vec_prec, int u
   c.unroll = unroll;
   c.grid = grid;
   tune_record(ctx, H, NR, vec_prec, c);
+  tune_record_avg(ctx, H, NR, vec_prec, c);
   return KPM_OK;
 }
 

@@ -98,6 +98,8 @@ _SIGS = {
     "kpm_1d_lr": (C.c_int, [_p, _p, _i64, _i64, _i64, _pd, _pd, _pd, C.c_int]),
     "kpm_1d_current": (C.c_int, [_p, _p, _p, _i64, _i64, _i64, _pd, _pd, C.c_int]),
     "kpm_1d_block": (C.c_int, [_p, _p, _i64, _p, _pd]),
+    "kpm_1d_avg": (C.c_int, [_p, _p, _i64, _i64, _i64, _pd, _pd, C.c_int]),
+    "kpm_1d_block_avg": (C.c_int, [_p, _p, _i64, _p, _pd]),
     "kpm_1d_block_noalloc": (C.c_int, [_p, _p, _i64, _p, _p, _p, _p]),
     "kpm_1d_lr_block": (C.c_int, [_p, _p, _i64, _p, _p, _pd]),
     "kpm_2d_blocking": (C.c_int, [_p, _i64, _i64, _i64, _i64, _i64, _i64, _pi64, _pi64, _pi64]),
@@ -132,6 +134,7 @@ _SIGS = {
     "kpm_batch_autotune": (C.c_int, [_p, _p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                      C.POINTER(C.c_float)]),
     "kpm_bench_step": (C.c_int, [_p, _p, _p, _p, _i64, C.POINTER(C.c_float)]),
+    "kpm_bench_step_mode": (C.c_int, [_p, _p, _p, _p, _i64, C.c_int, C.POINTER(C.c_float)]),
     "kpm_bench_copy": (C.c_int, [_p, _p, _p, _i64, C.POINTER(C.c_float)]),
     "kpm_bench_contract": (C.c_int, [_p, _i64, _i64, _i64, _i64, C.c_int, C.c_int, _i64,
                                      C.POINTER(C.c_float)]),
@@ -143,6 +146,8 @@ _SIGS = {
     "kpm_tune_set_chunked": (C.c_int, [_p, _p, _i64, C.c_int, C.c_int, C.c_int, C.c_int]),
     "kpm_tune_get": (C.c_int, [_p, _p, _i64, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "kpm_tune_get_mode": (C.c_int, [_p, _p, _i64, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int), C.POINTER(C.c_int)]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -99,12 +99,35 @@ def kpm_1d_(H, NC: int, NR: int, NH: int, mu_all: np.ndarray, psi_in=None, psi_i
     return None
 
 
+def _kpm_1d_avg(Hd, NC: int, NR: int, NH: int, psi_in, precision) -> np.ndarray:
+    """Probe-averaged equal-vector moments, sum(mu_all, dims=1) ./ NR
+    (src/moment.jl:410-416), from the engine's probe-summed doubling pass:
+    the NR x NC per-probe table is never formed."""
+    if NC % 2 != 0:
+        raise AssertionError("Invalid NC: NC should be even.")
+    if Hd.shape[0] != NH:
+        raise AssertionError("H and psi_in disagree on NH")
+    ctx = get_context()
+    out = np.zeros(NC, dtype=np.complex128)
+    if psi_in is None:
+        # random probes drawn on the device (src/moment.jl:493-497)
+        blk = DeviceBlock(NH, NR, precision, ctx)
+        blk.random_phase(int(np.random.SeedSequence().entropy) & 0xFFFFFFFFFFFFFFFF)
+        check(ctx.lib.kpm_1d_block_avg(ctx.handle, Hd.handle, NC, blk.handle, dptr(out)), "kpm_1d!")
+    else:
+        psi = _probe(psi_in, NH, NR, "psi_in")
+        check(ctx.lib.kpm_1d_avg(ctx.handle, Hd.handle, NC, NR, NH, dptr(psi), dptr(out),
+                                 prec_code(precision)), "kpm_1d!")
+    return out
+
+
 def kpm_1d(H, NC: int, NR: int, NH: int | None = None, *, psi_in=None, psi_in_l=None,
            psi_in_r=None, force_norm: bool = False, verbose: int = 0, avg_output: bool = True,
            NR_parallel: bool = True, precision="f64"):
     """kpm_1d (src/moment.jl:339-418): NC moments averaged over NR probes
     (real for equal vectors, complex for bra/ket), or the NR x NC matrix with
-    ``avg_output=False``."""
+    ``avg_output=False``.  Averaged equal-vector calls run the probe-summed
+    doubling pass (``_kpm_1d_avg``)."""
     NC = int(NC)
     NR = int(NR)
     NH = _nh(H) if NH is None else int(NH)
@@ -129,6 +152,8 @@ def kpm_1d(H, NC: int, NR: int, NH: int | None = None, *, psi_in=None, psi_in_l=
                             np.asarray(psi_in_r)[:, i:i + 1], verbose=verbose, precision=precision)
                     mu_all[i, :] = row[0]
         else:
+            if NR_parallel and avg_output:
+                return _kpm_1d_avg(Hd, NC, NR, NH, None, precision).real.copy()
             if NR_parallel:
                 kpm_1d_(Hd, NC, NR, NH, mu_all, verbose=verbose, precision=precision)
             else:
@@ -142,6 +167,8 @@ def kpm_1d(H, NC: int, NR: int, NH: int | None = None, *, psi_in=None, psi_in_l=
                 "must either set `psi_in` or set `psi_in_l` and `psi_in_r`, but not both.")
         if force_norm:
             normalize_by_col(psi_in, NR)
+        if NR_parallel and avg_output:
+            return _kpm_1d_avg(Hd, NC, NR, NH, psi_in, precision).real.copy()
         if NR_parallel:
             kpm_1d_(Hd, NC, NR, NH, mu_all, psi_in, verbose=verbose, precision=precision)
         else:

@@ -28,17 +28,20 @@ def autotune(H, NR: int, precision="f64", reps: int = 5) -> dict:
     out = {"unroll": unroll.value, "grid": grid.value, "ms_per_pass": ms.value,
            "precision": prec_code(precision)}
     out.update({k: v for k, v in tune_get(Hd, NR, precision).items() if k in ("chunk", "xmap")})
+    out["avg"] = tune_get(Hd, NR, precision, avg=True)
     return out
 
 
-def tune_get(H, NR: int, precision="f64") -> dict:
+def tune_get(H, NR: int, precision="f64", avg: bool = False) -> dict:
     """The launch shape the engines use for (H, NR, precision); chunk 0 is
-    the persistent grid."""
+    the persistent grid.  ``avg``: the probe-summed doubling pass of the
+    averaged kpm_1d instead of the per-probe passes."""
     ctx = get_context()
     Hd = to_device(H, ctx)
     u, g, ch, xm = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
-    check(ctx.lib.kpm_tune_get(ctx.handle, Hd.handle, int(NR), prec_code(precision), C.byref(u),
-                               C.byref(g), C.byref(ch), C.byref(xm)), "tune_get")
+    check(ctx.lib.kpm_tune_get_mode(ctx.handle, Hd.handle, int(NR), prec_code(precision),
+                                    1 if avg else 0, C.byref(u), C.byref(g), C.byref(ch),
+                                    C.byref(xm)), "tune_get")
     return {"unroll": u.value, "grid": g.value, "chunk": ch.value, "xmap": xm.value}
 
 

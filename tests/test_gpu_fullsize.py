@@ -71,6 +71,42 @@ def test_graphene_2m_fp32_vs_fp64_full_block():
     assert err <= 1e-6
 
 
+def test_graphene_2m_headline_shapes_agree():
+    """The headline pass (FP32, NR = 64) under the launch shapes the tuner
+    picks from: one-shot grids of 1, 4 and 8 row tiles per workgroup (two-level
+    reductions of up to 2^17 per-workgroup partials) and the persistent grid.
+    The stored vectors are the same in every shape; only the FP64 grouping of
+    the dot sums moves, so the moments agree to roundoff.  Both the per-probe
+    table and the probe-averaged moments (kpm_1d_block_avg) are checked, and
+    the averaged ones against the per-probe mean."""
+    L, NC, NR = 1024, 512, 64
+    H = (models.graphene(L, L, disorder=0.5, seed=42) / 3.6).tocsc()
+    Hd = K.to_device(H)
+    ctx = K.get_context()
+    blk = K.DeviceBlock.from_host(K.DeviceBlock(H.shape[0], NR, "f64").random_phase(7).download(),
+                                  "f32")
+    res = {}
+    for shape in ((2, 0), (2, 4), (2, 8), (1, 8), (1, 16)):
+        u, ch = shape
+        print(f"graphene 2M FP32 NR=64 NC={NC} shape unroll={u} chunk={ch}", flush=True)
+        if ch:
+            K.tune_set_chunked(H, NR, "f32", unroll=u, chunk=ch)
+        else:
+            K.tune_set(H, NR, "f32", unroll=u, grid=0)
+        mu = np.zeros((NR, NC), dtype=np.complex128, order="F")
+        K._lib.check(ctx.lib.kpm_1d_block(ctx.handle, Hd.handle, NC, blk.handle, K._lib.dptr(mu)))
+        av = np.zeros(NC, dtype=np.complex128)
+        K._lib.check(ctx.lib.kpm_1d_block_avg(ctx.handle, Hd.handle, NC, blk.handle, K._lib.dptr(av)))
+        res[shape] = (mu, av)
+    K.tune_set(H, NR, "f32", unroll=2, grid=0)
+    base_mu, base_av = res[(2, 0)]
+    assert abs(base_av[0] - 1.0) < 1e-7  # unit probes rounded to FP32
+    np.testing.assert_allclose(base_av, base_mu.mean(axis=0), rtol=0, atol=1e-13)
+    for shape, (mu, av) in res.items():
+        np.testing.assert_allclose(mu, base_mu, rtol=0, atol=1e-12, err_msg=str(shape))
+        np.testing.assert_allclose(av, base_av, rtol=0, atol=1e-12, err_msg=str(shape))
+
+
 @pytest.fixture(scope="module")
 def haldane_c3():
     g = load("c3_sigma.npz")
