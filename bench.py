@@ -63,8 +63,9 @@ def main() -> int:
     ap.add_argument("--kernel-launches", type=int, default=100)
     ap.add_argument("--shape", default="",
                     help="fix the fused pass's launch shape instead of tuning it: "
-                         "'unroll:chunk' (chunk 0: the persistent grid), e.g. 2:8; used to "
-                         "profile the timed calls alone")
+                         "'unroll:chunk[:xmap]' (chunk 0: the persistent grid; xmap: chunk "
+                         "order, 0 dispatch order, m >= 2 runs of m chunks per XCD), e.g. 2:4:4; "
+                         "used to profile the timed calls alone")
     ap.add_argument("--workload", choices=["graphene", "ensemble", "conductivity", "core", "adjoint",
                                            "cpge", "cond_ensemble", "spmm"],
                     default="graphene",
@@ -125,9 +126,10 @@ def main() -> int:
 
     def run_leg(precision: str):
         if args.shape:
-            u, ch = (int(v) for v in args.shape.split(":"))
+            u, ch, *xm = (int(v) for v in args.shape.split(":"))
             if ch > 0:
-                K.tune_set_chunked(Hd, NR, precision, unroll=u, chunk=ch)
+                K.tune_set_chunked(Hd, NR, precision, unroll=u, chunk=ch,
+                                   xcd_grouped=xm[0] if xm else 0)
             else:
                 K.tune_set(Hd, NR, precision, unroll=u, grid=0)
             tuned = dict(K.tune_get(Hd, NR, precision), precision=precision, fixed=True)

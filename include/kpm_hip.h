@@ -451,8 +451,10 @@ int kpm_autotune_step(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z
 /* Set the table entry for a shape directly (unroll 1 or 2; grid 0 = auto). */
 int kpm_tune_set(kpm_ctx* ctx, const kpm_csr* H, int64_t NR, int vec_prec, int unroll, int grid);
 /* Set a one-shot schedule for a shape: one workgroup per `chunk` consecutive
- * row tiles, chunks in dispatch order (xcd_grouped = 0) or each XCD sweeping
- * its own contiguous eighth (xcd_grouped = 1).  The engines fall back to the
+ * row tiles, chunks in dispatch order (xcd_grouped = 0), each XCD sweeping
+ * its own contiguous eighth (xcd_grouped = 1), or in dispatch order with each
+ * XCD taking runs of m = xcd_grouped consecutive chunks (2 .. 64), which keeps
+ * one front and puts a row's near neighbours in the same XCD's L2.  The engines fall back to the
  * persistent schedule when the grid would exceed their partials buffers. */
 int kpm_tune_set_chunked(kpm_ctx* ctx, const kpm_csr* H, int64_t NR, int vec_prec, int unroll,
                          int chunk, int xcd_grouped);

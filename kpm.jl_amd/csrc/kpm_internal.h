@@ -245,7 +245,7 @@ struct StepConfig {
   int unroll = 2;     // rows per lane per tile
   int grid = 0;       // persistent workgroups (multiple of 8); 0 = auto
   int chunk = 0;      // > 0: one-shot schedule, this many tiles per workgroup
-  int xmap = 0;       // one-shot schedule: 1 = chunks grouped per XCD
+  int xmap = 0;       // one-shot schedule: 1 = chunks grouped per XCD, m >= 2 = runs of m chunks per XCD
 };
 int launch_step(kpm_ctx* ctx, const StepLaunch& s, const StepConfig& cfg, int* grid_used);
 // geometry of the fused pass for a block shape: lanes per row, row-chunks,

@@ -434,10 +434,21 @@ __global__ void __launch_bounds__(kBlock)
     // tiles.  xmap = 0: c = b, so the dispatcher's in-order hand-out keeps
     // the whole grid on one front; xmap = 1: XCD k (= b % 8) takes the
     // chunks of its own contiguous eighth in order (one front per XCD, the
-    // neighbour rows of a front in that XCD's L2).  Chunks past the last
-    // tile are empty and only write zero partials.
+    // neighbour rows of a front in that XCD's L2); xmap = m >= 2: one front,
+    // but XCD k takes runs of m consecutive chunks (round r of 8m
+    // workgroups covers chunks [8mr, 8m(r+1)), XCD k the k-th run), so a
+    // row's near neighbours sit in the same XCD's L2 except at every m-th
+    // chunk boundary (a short last round uses shorter runs; G is a multiple
+    // of 8).  Chunks past the last tile are empty and only write zero
+    // partials.
     int c = b;
-    if (xmap) c = (b & 7) * (G >> 3) + (b >> 3);
+    if (xmap == 1) {
+      c = (b & 7) * (G >> 3) + (b >> 3);
+    } else if (xmap > 1) {
+      const int r0 = b / (8 * xmap) * (8 * xmap);
+      const int q = min(xmap, (G - r0) >> 3);
+      c = r0 + (b & 7) * q + ((b - r0) >> 3);
+    }
     tstart = (int)min<int64_t>((int64_t)c * chunk, ntiles);
     t1 = (int)min<int64_t>((int64_t)tstart + chunk, ntiles);
     Bx = 1;

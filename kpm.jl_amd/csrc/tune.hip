@@ -81,7 +81,7 @@ int kpm_autotune_step(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z
     for (int ch : {2, 4, 8, 16, 32, 64}) {
       const int64_t gc = ((ntiles + ch - 1) / ch + 7) / 8 * 8;
       if (gc > cap || gc < 2LL * std::max(ctx->num_cus, 8)) continue;  // too many, or too few to fill the chip
-      for (int xm : {0, 1}) {
+      for (int xm : {0, 1, 2, 4, 8}) {
         StepConfig c;
         c.unroll = u;
         c.chunk = ch;
@@ -131,10 +131,11 @@ int kpm_tune_set_chunked(kpm_ctx* ctx, const kpm_csr* H, int64_t NR, int vec_pre
   if (!ctx || !H || NR <= 0) return fail(KPM_EINVAL, "bad arguments");
   if (unroll != 1 && unroll != 2) return fail(KPM_EINVAL, "unroll must be 1 or 2");
   if (chunk < 1) return fail(KPM_EINVAL, "chunk must be >= 1 tile");
+  if (xcd_grouped < 0 || xcd_grouped > 64) return fail(KPM_EINVAL, "xcd_grouped must be 0 .. 64");
   StepConfig c;
   c.unroll = unroll;
   c.chunk = chunk;
-  c.xmap = xcd_grouped ? 1 : 0;
+  c.xmap = xcd_grouped;
   tune_record(ctx, H, NR, vec_prec, c);
   tune_record_avg(ctx, H, NR, vec_prec, c);
   return KPM_OK;

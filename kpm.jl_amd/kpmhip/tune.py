@@ -46,12 +46,14 @@ def tune_get(H, NR: int, precision="f64", avg: bool = False) -> dict:
 
 
 def tune_set_chunked(H, NR: int, precision="f64", unroll: int = 1, chunk: int = 64,
-                     xcd_grouped: bool = False) -> None:
-    """Record the one-shot schedule (chunk row tiles per workgroup) for a shape."""
+                     xcd_grouped: int = 0) -> None:
+    """Record the one-shot schedule (chunk row tiles per workgroup) for a
+    shape.  xcd_grouped: 0 chunks in dispatch order, 1 one contiguous eighth
+    per XCD, m >= 2 dispatch order with runs of m chunks per XCD."""
     ctx = get_context()
     Hd = to_device(H, ctx)
     check(ctx.lib.kpm_tune_set_chunked(ctx.handle, Hd.handle, int(NR), prec_code(precision),
-                                       int(unroll), int(chunk), 1 if xcd_grouped else 0),
+                                       int(unroll), int(chunk), int(xcd_grouped)),
           "tune_set_chunked")
 
 

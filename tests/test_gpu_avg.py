@@ -98,11 +98,13 @@ def test_launch_shapes_and_repeatability(prec):
             K.tune_set(H, NR, prec, unroll=unroll, grid=grid)
             got = K.kpm_1d(H, NC, NR, psi_in=psi, precision=prec)
             np.testing.assert_allclose(got, base, rtol=0, atol=1e-13)
-        for chunk in (1, 2, 4, 16):
-            print(f"avg shape {prec} unroll={unroll} chunk={chunk}", flush=True)
-            K.tune_set_chunked(H, NR, prec, unroll=unroll, chunk=chunk)
+        for chunk, xm in ((1, 0), (2, 0), (4, 0), (16, 0), (2, 1), (1, 4), (4, 2), (2, 8)):
+            print(f"avg shape {prec} unroll={unroll} chunk={chunk} xmap={xm}", flush=True)
+            K.tune_set_chunked(H, NR, prec, unroll=unroll, chunk=chunk, xcd_grouped=xm)
             got = K.kpm_1d(H, NC, NR, psi_in=psi, precision=prec)
             np.testing.assert_allclose(got, base, rtol=0, atol=1e-13)
+            per = K.kpm_1d(H, NC, NR, psi_in=psi, avg_output=False, precision=prec)
+            np.testing.assert_allclose(per.mean(axis=0).real, base, rtol=0, atol=1e-13)
     picked = K.autotune(H, NR, prec)
     assert picked["avg"]["unroll"] in (1, 2)
     got = K.kpm_1d(H, NC, NR, psi_in=psi, precision=prec)

@@ -86,11 +86,11 @@ def test_graphene_2m_headline_shapes_agree():
     blk = K.DeviceBlock.from_host(K.DeviceBlock(H.shape[0], NR, "f64").random_phase(7).download(),
                                   "f32")
     res = {}
-    for shape in ((2, 0), (2, 4), (2, 8), (1, 8), (1, 16)):
-        u, ch = shape
-        print(f"graphene 2M FP32 NR=64 NC={NC} shape unroll={u} chunk={ch}", flush=True)
+    for shape in ((2, 0, 0), (2, 4, 0), (2, 4, 4), (2, 8, 0), (1, 8, 2), (1, 16, 1)):
+        u, ch, xm = shape
+        print(f"graphene 2M FP32 NR=64 NC={NC} shape unroll={u} chunk={ch} xmap={xm}", flush=True)
         if ch:
-            K.tune_set_chunked(H, NR, "f32", unroll=u, chunk=ch)
+            K.tune_set_chunked(H, NR, "f32", unroll=u, chunk=ch, xcd_grouped=xm)
         else:
             K.tune_set(H, NR, "f32", unroll=u, grid=0)
         mu = np.zeros((NR, NC), dtype=np.complex128, order="F")
@@ -99,7 +99,7 @@ def test_graphene_2m_headline_shapes_agree():
         K._lib.check(ctx.lib.kpm_1d_block_avg(ctx.handle, Hd.handle, NC, blk.handle, K._lib.dptr(av)))
         res[shape] = (mu, av)
     K.tune_set(H, NR, "f32", unroll=2, grid=0)
-    base_mu, base_av = res[(2, 0)]
+    base_mu, base_av = res[(2, 0, 0)]
     assert abs(base_av[0] - 1.0) < 1e-7  # unit probes rounded to FP32
     np.testing.assert_allclose(base_av, base_mu.mean(axis=0), rtol=0, atol=1e-13)
     for shape, (mu, av) in res.items():
