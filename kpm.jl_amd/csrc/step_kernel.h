@@ -220,53 +220,72 @@ __device__ inline void st_stream(float4* p, const float4& v) {
 // fixed tree over the kCombRows lanes of each column finishes the sum.
 // sm: kBlock double2 of LDS; every thread of the workgroup must call it.
 constexpr int kCombCols = 8;
-constexpr int kCombRows = kBlock / kCombCols;
 __host__ __device__ inline int combine_units(int nd, int NR) {
   return nd * ((NR + kCombCols - 1) / kCombCols);
 }
-__device__ inline void combine_unit(const double2* __restrict__ partials, int G, int nd, int NRp,
-                                    int NR, double2* __restrict__ mu, const Combine& cb, int unit,
-                                    double2* sm) {
-  const int ngrp = (NR + kCombCols - 1) / kCombCols;
-  const int d = unit / ngrp;
-  const int col = (unit % ngrp) * kCombCols + (int)threadIdx.x % kCombCols;
-  const int lane = (int)threadIdx.x / kCombCols;
-  const bool live = col < NR;
-  const size_t gs = (size_t)nd * NRp;  // between consecutive workgroups
-  const double2* __restrict__ p = partials + (size_t)d * NRp + (live ? col : 0);
+// Columns one unit reduces: kCombCols, or 1 for a one-column table (the
+// probe sums of MODE_DBLA), so that every thread of the unit adds rows.
+__device__ inline int comb_cols(int NR) { return NR == 1 ? 1 : kCombCols; }
+// Sum over partial rows [g0, g1) of one column (row stride gs): thread t
+// takes column t % ncols and rows g0 + t / ncols, + kBlock / ncols, ... in
+// that order (loads issued 4 at a time), then a fixed tree over the threads
+// of each column.  The column's sum lands in sm[t] of its thread with
+// t / ncols == 0.  sm: kBlock double2 of LDS; every thread must call it.
+__device__ inline void sum_rows(const double2* __restrict__ p, int g0, int g1, size_t gs, int ncols,
+                                double2* sm) {
+  const int lane = (int)threadIdx.x / ncols;
+  const int rows = kBlock / ncols;
   double sr = 0.0, si = 0.0;
   // loads in flight per thread: kept small because the fused pass carries
   // this code, and its register allocation is the maximum over the kernel
   // (16 in flight took the FP32 graphene pass from 107 to 148 VGPRs, 4 to 3
   // waves per SIMD)
   constexpr int kBatch = 4;
-  int g = lane;
-  for (; g + (kBatch - 1) * kCombRows < G; g += kBatch * kCombRows) {
+  int g = g0 + lane;
+  for (; g + (kBatch - 1) * rows < g1; g += kBatch * rows) {
     double2 v[kBatch];
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) v[u] = p[(size_t)(g + u * kCombRows) * gs];
+    for (int u = 0; u < kBatch; ++u) v[u] = p[(size_t)(g + u * rows) * gs];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       sr += v[u].x;
       si += v[u].y;
     }
   }
-  for (; g < G; g += kCombRows) {
+  for (; g < g1; g += rows) {
     const double2 v = p[(size_t)g * gs];
     sr += v.x;
     si += v.y;
   }
   sm[threadIdx.x] = make_double2(sr, si);
   __syncthreads();
-  for (int w = kCombRows / 2; w > 0; w >>= 1) {  // lanes of a column are kCombCols apart
+  for (int w = rows / 2; w > 0; w >>= 1) {  // lanes of a column are ncols apart
     if (lane < w) {
-      const double2 o = sm[threadIdx.x + w * kCombCols];
+      const double2 o = sm[threadIdx.x + w * ncols];
       sm[threadIdx.x].x += o.x;
       sm[threadIdx.x].y += o.y;
     }
     __syncthreads();
   }
-  if (lane == 0 && live) {
+}
+
+// Fixed-order reduction of the moments of one dot d and a group of
+// comb_cols(NR) consecutive probe columns over G partial rows stored
+// [g][d][col], then columnwise_dot!'s alpha * dot + beta_scale *
+// target[:, beta_col] (src/utils/vectors.jl:103-117).  The kCombCols columns
+// of one row (one 128-byte line) are read together by one unit.
+// sm: kBlock double2 of LDS; every thread of the workgroup must call it.
+__device__ inline void combine_unit(const double2* __restrict__ partials, int G, int nd, int NRp,
+                                    int NR, double2* __restrict__ mu, const Combine& cb, int unit,
+                                    double2* sm) {
+  const int ncols = comb_cols(NR);
+  const int ngrp = (NR + kCombCols - 1) / kCombCols;
+  const int d = unit / ngrp;
+  const int col = (unit % ngrp) * kCombCols + (int)threadIdx.x % ncols;
+  const bool live = col < NR;
+  const size_t gs = (size_t)nd * NRp;  // between consecutive partial rows
+  sum_rows(partials + (size_t)d * NRp + (live ? col : 0), 0, G, gs, ncols, sm);
+  if ((int)threadIdx.x < ncols && live) {
     const double2 tot = sm[threadIdx.x];
     double rr = cb.alpha[d] * tot.x;
     double ri = cb.alpha[d] * tot.y;
@@ -281,54 +300,25 @@ __device__ inline void combine_unit(const double2* __restrict__ partials, int G,
 }
 
 // First level of the two-level reduction: one unit sums, for one dot d and
-// kCombCols consecutive columns, the partials of the kSlice workgroups of one
-// slice (each lane its kSlice / kCombRows workgroups in order, then a fixed
-// tree) and writes the slice sum to out[slice][d][col].
+// a group of columns, the partials of the kSlice rows of one slice and writes
+// the slice sum to out[slice][d][col].
 __host__ __device__ inline int reduce1_units(int nd, int NR, int G) {
   return combine_units(nd, NR) * ((G + kSlice - 1) / kSlice);
 }
 __device__ inline void reduce1_unit(const double2* __restrict__ partials, int G, int nd, int NRp,
                                     int NR, double2* __restrict__ out, int unit, double2* sm) {
+  const int ncols = comb_cols(NR);
   const int nu = combine_units(nd, NR);
   const int slice = unit / nu;
   const int cu = unit % nu;
   const int ngrp = (NR + kCombCols - 1) / kCombCols;
   const int d = cu / ngrp;
-  const int col = (cu % ngrp) * kCombCols + (int)threadIdx.x % kCombCols;
-  const int lane = (int)threadIdx.x / kCombCols;
+  const int col = (cu % ngrp) * kCombCols + (int)threadIdx.x % ncols;
   const bool live = col < NR;
   const size_t gs = (size_t)nd * NRp;
-  const double2* __restrict__ p = partials + (size_t)d * NRp + (live ? col : 0);
-  const int g1 = min(G, (slice + 1) * kSlice);
-  double sr = 0.0, si = 0.0;
-  constexpr int kBatch = 4;
-  int g = slice * kSlice + lane;
-  for (; g + (kBatch - 1) * kCombRows < g1; g += kBatch * kCombRows) {
-    double2 v[kBatch];
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u) v[u] = p[(size_t)(g + u * kCombRows) * gs];
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u) {
-      sr += v[u].x;
-      si += v[u].y;
-    }
-  }
-  for (; g < g1; g += kCombRows) {
-    const double2 v = p[(size_t)g * gs];
-    sr += v.x;
-    si += v.y;
-  }
-  sm[threadIdx.x] = make_double2(sr, si);
-  __syncthreads();
-  for (int w = kCombRows / 2; w > 0; w >>= 1) {
-    if (lane < w) {
-      const double2 o = sm[threadIdx.x + w * kCombCols];
-      sm[threadIdx.x].x += o.x;
-      sm[threadIdx.x].y += o.y;
-    }
-    __syncthreads();
-  }
-  if (lane == 0 && live) out[(size_t)slice * gs + (size_t)d * NRp + col] = sm[threadIdx.x];
+  sum_rows(partials + (size_t)d * NRp + (live ? col : 0), slice * kSlice, min(G, (slice + 1) * kSlice),
+           gs, ncols, sm);
+  if ((int)threadIdx.x < ncols && live) out[(size_t)slice * gs + (size_t)d * NRp + col] = sm[threadIdx.x];
   __syncthreads();
 }
 
