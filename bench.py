@@ -340,7 +340,15 @@ def bench_spmm(args) -> int:
     NRp = K.moment._padded_nr(NR, pcode)
     byts = nnz * ((8 if real else 16) + 4) + (NH + 1) * 4 + 3 * NH * NRp * (16 if pcode == 0 else 8)
     Hd = K.to_device(H)
-    tuned = K.autotune(Hd, NR, args.prec)
+    if args.shape:
+        u, ch, *xm = (int(v) for v in args.shape.split(":"))
+        if ch > 0:
+            K.tune_set_chunked(Hd, NR, args.prec, unroll=u, chunk=ch, xcd_grouped=xm[0] if xm else 0)
+        else:
+            K.tune_set(Hd, NR, args.prec, unroll=u, grid=0)
+        tuned = dict(K.tune_get(Hd, NR, args.prec), fixed=True)
+    else:
+        tuned = K.autotune(Hd, NR, args.prec)
     x, z = DeviceBlock(NH, NR, args.prec, ctx), DeviceBlock(NH, NR, args.prec, ctx)
     ms = ctypes.c_float(0.0)
     n = max(args.kernel_launches, 1)
