@@ -45,7 +45,8 @@ def cpu_model() -> str:
 def step_bytes(nh: int, nnz: int, nrp: int, elem: int, val_bytes: int) -> int:
     """Algorithmic HBM bytes of one fused pass: H once (values + int32 column
     indices + row pointers) and each vector row once: read v_n, read v_{n-1},
-    write v_{n+1}."""
+    write v_{n+1}.  Values are 8 / 16 bytes (real / complex); FP32 blocks of
+    rows up to 8 words read 4 / 8-byte FP32 copies."""
     return nnz * (val_bytes + 4) + (nh + 1) * 4 + 3 * nh * nrp * elem
 
 
@@ -193,8 +194,9 @@ def main() -> int:
     units = NC * NR * nnz
     value = units / wall32
     nrp32 = K.moment._padded_nr(NR, 1)
-    val_bytes = 8 if Hd.is_real else 16
-    b32 = step_bytes(NH, nnz, nrp32, 8, val_bytes)
+    val_bytes = 8 if Hd.is_real else 16  # FP64 values (FP32 copies only for rows of <= 8 words)
+    vb32 = val_bytes // 2 if nrp32 <= 16 else val_bytes
+    b32 = step_bytes(NH, nnz, nrp32, 8, vb32)
     # the average launch of the timed calls themselves: event time of the
     # timed kpm_1d calls over their NC/2 fused passes (so it also carries the
     # call's probe copy, last reduction and moment download: an upper bound)
@@ -338,7 +340,8 @@ def bench_spmm(args) -> int:
     real = not (H.dtype.kind == "c" and np.any(H.data.imag != 0))
     pcode = 0 if args.prec == "f64" else 1
     NRp = K.moment._padded_nr(NR, pcode)
-    byts = nnz * ((8 if real else 16) + 4) + (NH + 1) * 4 + 3 * NH * NRp * (16 if pcode == 0 else 8)
+    vb = (8 if real else 16) // (2 if (pcode == 1 and NRp <= 16) else 1)  # FP32 values on rows <= 8 words
+    byts = nnz * (vb + 4) + (NH + 1) * 4 + 3 * NH * NRp * (16 if pcode == 0 else 8)
     Hd = K.to_device(H)
     if args.shape:
         u, ch, *xm = (int(v) for v in args.shape.split(":"))
@@ -439,11 +442,30 @@ def bench_core(args) -> int:
     Vp = K.DeviceBlock.from_host(K.random_phase_vectors(Xoshiro256pp(99), 1 << 18, 8))
     times["chebyshev_iter_single_NH262144_NR8"] = timed(
         lambda: K.chebyshev_iter_single(Hi, Vpp, Vp), sync=True)
+    # the CPU restatement of the same two kpm_1d calls on 8 host threads, the
+    # thread count of the reference's published CPU runs
+    # (perf-core-mike-cpu.json:10-11), as a fidelity check of the CPU baseline
+    port8 = {}
+    if not args.skip_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        oracle.set_threads(8)
+        for name, (H, NC, psi) in (("kpm_1d_dos_NH65536_NC1024_NR8", (H1, 1024, psi1)),
+                                   ("kpm_1d_dos_NH262144_NC512_NR4", (H2, 512, psi2))):
+            best = float("inf")
+            for _ in range(2):
+                t0 = time.perf_counter()
+                oracle.kpm_1d(H, NC, psi)
+                best = min(best, time.perf_counter() - t0)
+            port8[name] = round(best, 4)
+            print(f"[bench] CPU port (8 threads) {name}: {best:.3f} s", file=sys.stderr, flush=True)
     cases = {}
     for name, t in times.items():
         a100, cpu = CORE_PUBLISHED[name]
         cases[name] = {"seconds": round(t, 6), "a100_seconds": a100, "cpu8_seconds": cpu,
                        "speedup_vs_a100": round(a100 / t, 2), "speedup_vs_cpu8": round(cpu / t, 2)}
+        if name in port8:
+            cases[name]["port_cpu8_seconds"] = port8[name]
         print(f"[bench] {name}: {t * 1e3:.3f} ms  (A100 {a100 * 1e3:.2f} ms, "
               f"x{a100 / t:.1f})", file=sys.stderr, flush=True)
     head = "kpm_1d_dos_NH65536_NC1024_NR8"
@@ -455,7 +477,10 @@ def bench_core(args) -> int:
            "dtype": "c128", "data": "synthetic",
            "config": {"workload": "KPM.jl benchmarks/core_bench.jl suite (periodic square lattices)",
                       "baseline": "perf-core-mike-a100.json (A100, Julia 1.12.6)"},
-           "cases": cases}
+           "cases": cases,
+           "cpu_port": {"threads": 8, "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+                        "note": "oracle C restatement of kpm_1d! on the same inputs; "
+                                "port_cpu8_seconds beside the reference's cpu8_seconds"}}
     print(json.dumps(res))
     return 0
 

@@ -167,6 +167,9 @@ struct kpm_csr {
   int* rowptr = nullptr;  // n+1
   int* colind = nullptr;  // nnz
   void* vals = nullptr;   // nnz x (double2 | double)
+  // the same values rounded to FP32, nnz x (float2 | float): what the
+  // recurrence of FP32 blocks reads (attach_vals32, at creation)
+  void* vals32 = nullptr;
 };
 
 struct kpm_block {
@@ -194,6 +197,10 @@ int upload_csr(kpm_ctx* ctx, int64_t n, const std::vector<int64_t>& rp, const st
 int csr_from_csc_staged(kpm_ctx* ctx, int64_t n, const int64_t* colptr, const int64_t* rowval,
                         const double* nzval, int base, bool adjoint, std::vector<int>* perm,
                         kpm_csr** out);
+// round A's values to FP32 into A->vals32 (device allocation + one
+// conversion kernel on the context stream); every operator gets them at
+// creation, so passes over FP32 blocks never allocate
+int attach_vals32(kpm_ctx* ctx, kpm_csr* A);
 // validate Julia CSC arrays (square n x n, index base 0 or 1)
 int check_csc(int64_t n, const int64_t* colptr, const int64_t* rowval, int base);
 // A non-owning block view over a cached workspace buffer.

@@ -83,6 +83,31 @@ static void free_buffer(Buffer& b) {
   b.bytes = 0;
 }
 
+__global__ void k_round_f32(const double* __restrict__ src, float* __restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = (float)src[i];  // round to nearest even, as the FP32 pass's conversion on load
+}
+
+int attach_vals32(kpm_ctx* ctx, kpm_csr* A) {
+  const int64_t nreal = std::max<int64_t>(A->nnz, 1) * (A->is_real ? 1 : 2);
+  hipError_t e = hipMalloc(&A->vals32, (size_t)nreal * sizeof(float));
+  if (e != hipSuccess) {
+    A->vals32 = nullptr;
+    (void)hipGetLastError();  // clear the latched status (see ensure)
+    return fail(KPM_ENOMEM, std::string("hipMalloc for FP32 operator values failed: ") +
+                                hipGetErrorString(e));
+  }
+  if (A->nnz > 0) {
+    const int64_t n = A->nnz * (A->is_real ? 1 : 2);
+    const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_round_f32, dim3(blocks), dim3(256), 0, ctx->stream, (const double*)A->vals,
+                       (float*)A->vals32, n);
+    KPM_HIP_TRY(hipGetLastError());
+  }
+  return KPM_OK;
+}
+
 // Build device CSR arrays from host (rowptr, colind, vals) already in CSR
 // form with 0-based int64 indices.
 int upload_csr(kpm_ctx* ctx, int64_t n, const std::vector<int64_t>& rp,
@@ -138,6 +163,11 @@ int upload_csr(kpm_ctx* ctx, int64_t n, const std::vector<int64_t>& rp,
     (void)hipFree(A->vals);
     delete A;
     return fail(KPM_EHIP, std::string("hipMemcpy for CSR failed: ") + hipGetErrorString(e));
+  }
+  const int rc = attach_vals32(ctx, A);
+  if (rc != KPM_OK) {
+    kpm_csr_destroy(A);
+    return rc;
   }
   *out = A;
   return KPM_OK;
@@ -242,6 +272,11 @@ int csr_from_csc_staged(kpm_ctx* ctx, int64_t n, const int64_t* colptr, const in
     (void)hipFree(A->vals);
     delete A;
     return fail(KPM_EHIP, std::string("CSR upload failed: ") + hipGetErrorString(e));
+  }
+  const int rc = attach_vals32(ctx, A);
+  if (rc != KPM_OK) {
+    kpm_csr_destroy(A);
+    return rc;
   }
   *out = A;
   return KPM_OK;
@@ -433,6 +468,7 @@ int kpm_csr_destroy(kpm_csr* A) {
   (void)hipFree(A->rowptr);
   (void)hipFree(A->colind);
   (void)hipFree(A->vals);
+  if (A->vals32) (void)hipFree(A->vals32);
   if (A->csc_to_csr_dev) (void)hipFree(A->csc_to_csr_dev);
   delete A;
   return KPM_OK;

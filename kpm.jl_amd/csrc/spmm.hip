@@ -247,7 +247,10 @@ int launch_step(kpm_ctx* ctx, const StepLaunch& s, const StepConfig& cfg, int* g
   const PendingCombine pc = s.pend ? *s.pend : PendingCombine{};
   StepLauncher fn = pick(prec, g, s.A->is_real != 0, s.mode);
   if (!fn) return fail(KPM_EINVAL, "no dual pass for rows of one lane");
-  fn(dim3(G, g.chunks), ctx->stream, s.A->rowptr, s.A->colind, s.A->vals, x->d, s.z ? s.z->d : x->d,
+  // FP32 blocks of narrow rows read the operator's FP32 values
+  const void* vals = vals32_step(prec, g.lpr, s.mode) ? s.A->vals32 : s.A->vals;
+  if (!vals) return fail(KPM_EINVAL, "operator has no FP32 values");
+  fn(dim3(G, g.chunks), ctx->stream, s.A->rowptr, s.A->colind, vals, x->d, s.z ? s.z->d : x->d,
      s.out->d, s.l ? s.l->d : nullptr, part, (int)x->NH, g.RW, (int)x->NRp,
      (int)ntiles, s.alpha, s.beta, g.has_tail, nullptr, nullptr, nullptr, s.gamma, pc, s.vals2,
      s.out2 ? s.out2->d : nullptr, o2, chunk, chunk > 0 ? cfg.xmap : 0);
@@ -262,7 +265,9 @@ int launch_step_seg(kpm_ctx* ctx, const StepLaunch& s, const StepGeom& g, int G,
                     const int* tile_lo, const int* treal, const int* seg0, double2* partials) {
   const kpm_block* x = s.x;
   StepLauncher fn = pick(x->prec, g, s.A->is_real != 0, MODE_DBL_SEG);
-  fn(dim3(G, g.chunks), ctx->stream, s.A->rowptr, s.A->colind, s.A->vals, x->d, s.z ? s.z->d : x->d,
+  const void* vals = vals32_step(x->prec, g.lpr, MODE_DBL_SEG) ? s.A->vals32 : s.A->vals;
+  if (!vals) return fail(KPM_EINVAL, "operator has no FP32 values");
+  fn(dim3(G, g.chunks), ctx->stream, s.A->rowptr, s.A->colind, vals, x->d, s.z ? s.z->d : x->d,
      s.out->d, nullptr, partials, (int)x->NH, g.RW, (int)x->NRp, (int)ntiles, s.alpha, s.beta,
      g.has_tail, tile_lo, treal, seg0, 0.0, PendingCombine{}, nullptr, nullptr, SplitOut{}, 0, 0);
   KPM_HIP_TRY(hipGetLastError());

@@ -103,6 +103,11 @@ struct Word<KPM_VEC_F32> {
 
 constexpr int kBlock = 64 * kStepWaves;  // 4 waves
 
+// whether the fused pass reads the operator's FP32 values (kpm_csr::vals32)
+__host__ __device__ constexpr bool vals32_step(int prec, int lpr, int mode) {
+  return prec == KPM_VEC_F32 && lpr <= 8 && mode != MODE_DUAL;
+}
+
 // compile-time loop: f(std::integral_constant<int, I>) for I in [B, E)
 template <int B, int E, class F>
 __device__ __forceinline__ void static_for(F&& f) {
@@ -402,8 +407,19 @@ __global__ void __launch_bounds__(kBlock)
     o2col = (size_t)qb * o2.qs + (size_t)(wcol - qb * o2.cw);
   }
 
-  const double2* cvals = reinterpret_cast<const double2*>(vals_);
-  const double* rvals = reinterpret_cast<const double*>(vals_);
+  // operator values: FP32 copies for FP32 blocks of rows up to 8 words (the
+  // recurrence multiplies in FP32 there, so this is the same rounding as
+  // converting on load, at half the value bytes), FP64 otherwise and for the
+  // dual pass's merged operator.  Narrow rows are where the values are a
+  // large share of the pass's bytes (cubic NR = 4: 84 of 184 bytes per row
+  // with FP64 values); on 32-word rows (graphene NR = 64) the FP32 copies
+  // measured 0.5% slower, so those keep the FP64 values.  vals32_step()
+  // makes the launchers' choice.
+  constexpr bool kVal32 = vals32_step(PREC, LPR, MODE);
+  using RV = std::conditional_t<kVal32, float, double>;
+  using CV = std::conditional_t<kVal32, float2, double2>;
+  const CV* cvals = reinterpret_cast<const CV*>(vals_);
+  const RV* rvals = reinterpret_cast<const RV*>(vals_);
   const double2* cvals2 = reinterpret_cast<const double2*>(vals2_);
 
   double accr[NDA][CPL], acci[NDA][CPL];
@@ -506,11 +522,11 @@ __global__ void __launch_bounds__(kBlock)
           const int c = colind[k];
           Cm.c[u][e] = v ? c : 0;
           if constexpr (REALH) {
-            const double h = rvals[k];
+            const RV h = rvals[k];
             Cm.hr[u][e] = v ? (AT)h : (AT)0;
             Cm.hi[u][e] = (AT)0;
           } else {
-            const double2 h = cvals[k];
+            const CV h = cvals[k];
             Cm.hr[u][e] = v ? (AT)h.x : (AT)0;
             Cm.hi[u][e] = v ? (AT)h.y : (AT)0;
           }
@@ -629,11 +645,11 @@ __global__ void __launch_bounds__(kBlock)
           const bool v = jj < R0.cnt[u];
           const int k = v ? R0.s[u] + jj : 0;
           if constexpr (REALH) {
-            const double h = rvals[k];
+            const RV h = rvals[k];
             hr[u][jj] = v ? (AT)h : (AT)0;
             hi[u][jj] = (AT)0;
           } else {
-            const double2 h = cvals[k];
+            const CV h = cvals[k];
             hr[u][jj] = v ? (AT)h.x : (AT)0;
             hi[u][jj] = v ? (AT)h.y : (AT)0;
           }
@@ -681,7 +697,7 @@ __global__ void __launch_bounds__(kBlock)
             h_r = (AT)rvals[k];
             h_i = (AT)0;
           } else {
-            const double2 h = cvals[k];
+            const CV h = cvals[k];
             h_r = (AT)h.x;
             h_i = (AT)h.y;
           }
