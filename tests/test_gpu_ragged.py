@@ -3,7 +3,8 @@ empty to 41 nonzeros, so every nonzeros-per-row bucket of the recurrence pass
 (4, 8, 12, 16 and the long-row tail beyond 16) is taken, crossed with probe
 widths whose rows span 1, 2, 4, 8, 16 and 64 lanes.  kpm_1d is compared with the
 oracle at 1e-12 (FP64 vectors) and bounded at FP32 rounding (FP32 vectors),
-and the left/right form with distinct probes is checked the same way.
+and the left/right form with distinct probes and the probe-averaged form are
+checked the same way.
 CUSPARSE's SpMM (ext/KPMCUDAExt.jl:84-88) takes any CSR; the lattice models of
 the other tests have near-uniform rows, so this is where empty and long rows
 are exercised.
@@ -43,3 +44,11 @@ def test_ragged_rows_vs_oracle(w, NR, prec):
         np.testing.assert_allclose(got_lr, ref_lr, rtol=1e-12, atol=1e-12)
     else:
         assert np.max(np.abs(got_lr - ref_lr)) < 2e-5
+    # the probe-averaged form (kpm_1d's default, the probe-summed pass) on
+    # the same rows: the oracle's per-probe table averaged over probes
+    avg = K.kpm_1d(H, NC, NR, psi_in=psi, precision=prec)
+    ref_avg = ref.mean(axis=0).real
+    if prec == "f64":
+        np.testing.assert_allclose(avg, ref_avg, rtol=0, atol=1e-12)
+    else:
+        assert np.max(np.abs(avg - ref_avg)) < 2e-5
