@@ -251,12 +251,16 @@ static int launch_sddmm_t(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* const
   const int64_t nb = (beta[0]->NH + rows_per_block - 1) / rows_per_block;
   const unsigned G =
       (unsigned)std::min<int64_t>(nb, (int64_t)std::max(ctx->num_cus, 8) * KPM_SDDMM_WG_PER_CU);
+  // the row kernel takes one block of rows per workgroup (a one-shot grid,
+  // handed out in row order: 131.3 against 134.4 ms per c5 gradient step for
+  // the persistent grid-stride form)
+  const unsigned Grows = (unsigned)std::min<int64_t>(nb, (int64_t)INT32_MAX);
   const int w = A->max_row_nnz;
   if (lpr >= 2 && chunks == 1 && w <= 16) {
     const int kb = w <= 4 ? 4 : (w <= 8 ? 8 : (w <= 12 ? 12 : 16));
 #define KPM_SDDMM_ROWS(L, KBv)                                                                    \
   if (lpr == L && kb == KBv) {                                                                    \
-    hipLaunchKernelGGL((k_sddmm_rows<PREC, L, KBv>), dim3(G), dim3(kBlock), 0, ctx->stream,       \
+    hipLaunchKernelGGL((k_sddmm_rows<PREC, L, KBv>), dim3(Grows), dim3(kBlock), 0, ctx->stream,   \
                        A->rowptr, A->colind, win, M, (int)beta[0]->NH, RW);                       \
     KPM_HIP_TRY(hipGetLastError());                                                               \
     return KPM_OK;                                                                                \
