@@ -58,7 +58,22 @@ __global__ void k_batch_combine(const double2* __restrict__ partials, const int*
   const int r = idx / (NR * nd);
   double sr = 0.0, si = 0.0;
   const int s0 = sfirst[r], ns = scount[r];
-  for (int s = s0; s < s0 + ns; ++s) {
+  // segment partials added in segment order, loads issued 8 at a time (a
+  // realisation spans a segment per workgroup that touches it, tens of them
+  // on short-range schedules)
+  constexpr int kBatch = 8;
+  int s = s0;
+  for (; s + kBatch <= s0 + ns; s += kBatch) {
+    double2 v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) v[u] = partials[((size_t)(s + u) * nd + d) * NRp + col];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      sr += v[u].x;
+      si += v[u].y;
+    }
+  }
+  for (; s < s0 + ns; ++s) {
     const double2 v = partials[((size_t)s * nd + d) * NRp + col];
     sr += v.x;
     si += v.y;
@@ -355,21 +370,50 @@ int kpm_batch_autotune(kpm_ctx* ctx, kpm_batch* B, int reps, int* best_unroll, i
                              ctx->stream));
   KPM_HIP_TRY(hipMemcpyAsync(w1.d, pb.d, (size_t)(P * NRp) * elem_bytes(prec), hipMemcpyDeviceToDevice,
                              ctx->stream));
+  KPM_TRY(ensure(ctx, ctx->mu, (size_t)B->nreal * NR * 2 * sizeof(double2)));  // the timed combines' target
   float best = 1e30f;
   StepConfig bestc;
-  for (int u : {1, 2})
+  // candidates: 2..16 workgroups per CU (each sweeping a long contiguous
+  // tile range), and short ranges of 2..32 tiles per workgroup (a one-shot
+  // grid handed out in tile order)
+  std::vector<StepConfig> cand;
+  for (int u : {1, 2}) {
     for (int pc : {2, 4, 8, 16}) {
       StepConfig c;
       c.unroll = u;
       c.grid = std::max(ctx->num_cus, 8) * pc;
+      cand.push_back(c);
+    }
+    const StepGeom g0 = step_geom(B->A, NR, prec, StepConfig{u, 0, 0, 0});
+    const int64_t ntiles = P / g0.TR;
+    for (int ch : {2, 4, 8, 16, 32}) {
+      const int64_t G = (ntiles + ch - 1) / ch;
+      if (G <= (int64_t)std::max(ctx->num_cus, 8) * 16 || G > (int64_t)1 << 24) continue;
+      StepConfig c;
+      c.unroll = u;
+      c.grid = (int)G;
+      cand.push_back(c);
+    }
+  }
+  for (const StepConfig& c : cand) {
+    {
       const StepGeom g = step_geom(B->A, NR, prec, c);
       KPM_TRY(build_sched(B, g.TR, c.grid));
       KPM_TRY(ensure(ctx, B->partials, (size_t)std::max(B->sched.nseg, 1) * 2 * NRp * sizeof(double2)));
       StepLaunch s{B->A, &w0, &w1, &w1, nullptr, 2.0, -1.0, MODE_DBL_SEG};
-      auto launch = [&]() {
-        return launch_step_seg(ctx, s, g, B->sched.G, B->sched.ntiles, (const int*)B->sched.tile_lo.ptr,
-                               (const int*)B->sched.treal.ptr, (const int*)B->sched.seg0.ptr,
-                               (double2*)B->partials.ptr);
+      // a step as batch_steps runs it: the fused pass and its per-realisation
+      // combine (whose cost grows with the segments a schedule makes)
+      const int64_t total = (int64_t)B->nreal * 2 * NR;
+      auto launch = [&]() -> int {
+        KPM_TRY(launch_step_seg(ctx, s, g, B->sched.G, B->sched.ntiles, (const int*)B->sched.tile_lo.ptr,
+                                (const int*)B->sched.treal.ptr, (const int*)B->sched.seg0.ptr,
+                                (double2*)B->partials.ptr));
+        hipLaunchKernelGGL(k_batch_combine, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                           ctx->stream, (const double2*)B->partials.ptr, (const int*)B->sched.sfirst.ptr,
+                           (const int*)B->sched.scount.ptr, B->nreal, 2, (int)NRp, (int)NR, (int64_t)2,
+                           (double2*)ctx->mu.ptr, Combine{2, {0, 1}, {2.0, 2.0}, {-1, -1}, {0.0, 0.0}});
+        KPM_HIP_TRY(hipGetLastError());
+        return KPM_OK;
       };
       KPM_TRY(launch());
       KPM_HIP_TRY(hipEventRecord(ctx->events[12], ctx->stream));
@@ -384,6 +428,7 @@ int kpm_batch_autotune(kpm_ctx* ctx, kpm_batch* B, int reps, int* best_unroll, i
         bestc = c;
       }
     }
+  }
   tune_record(ctx, B->A, NR, prec, bestc);
   if (best_unroll) *best_unroll = bestc.unroll;
   if (best_grid) *best_grid = bestc.grid;

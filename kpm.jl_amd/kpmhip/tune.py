@@ -16,7 +16,7 @@ from ._lib import check
 from .device import DeviceBlock, get_context, prec_code, to_device
 
 
-def autotune(H, NR: int, precision="f64", reps: int = 5) -> dict:
+def autotune(H, NR: int, precision="f64", reps: int = 8) -> dict:
     ctx = get_context()
     Hd = to_device(H, ctx)
     n = Hd.shape[0]
