@@ -445,7 +445,9 @@ int kpm_event_elapsed(kpm_ctx* ctx, int slot_a, int slot_b, float* ms);
  * overwritten), records the fastest for this shape and returns it
  * (best_grid: the workgroups it launches).  The probe-summed doubling pass
  * is timed over the same candidates and recorded on its own
- * (kpm_tune_get_mode). */
+ * (kpm_tune_get_mode).  X and Z should hold data like the engine's (random
+ * phases, say): the pass's speed depends on the values, and all-zero blocks
+ * rank the schedules differently. */
 int kpm_autotune_step(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z, int reps,
                       int* best_unroll, int* best_grid, float* best_ms_per_step);
 /* Set the table entry for a shape directly (unroll 1 or 2; grid 0 = auto). */

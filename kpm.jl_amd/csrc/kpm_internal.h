@@ -163,6 +163,9 @@ struct kpm_csr {
   int is_real = 0;
   int max_row_nnz = 0;
   int sorted = 0;         // every row's column indices strictly increase
+  // every row stores its diagonal among its first diag_slots nonzeros
+  // (INT32_MAX when some row stores none)
+  int diag_slots = INT32_MAX;
   uint64_t serial = 0;    // process-unique identity (never reused)
   int* rowptr = nullptr;  // n+1
   int* colind = nullptr;  // nnz
@@ -218,6 +221,11 @@ int work_block(kpm_ctx* ctx, Buffer& buf, int64_t NH, int64_t NR, int prec, kpm_
 constexpr int kStepWaves = 4;  // waves per workgroup of the fused pass
 enum StepMode { MODE_NONE = 0, MODE_DBL = 1, MODE_LR = 2, MODE_DBL_SEG = 3, MODE_SRC = 4,
                 MODE_DUAL = 5, MODE_DBLA = 6 };
+// flag bits of the fused pass: rows longer than the gather slots exist; the
+// doubling modes may take the own-row words from the diagonal's gather slot
+// (kpm_csr::diag_slots <= slots)
+constexpr int kStepTail = 1;
+constexpr int kStepDiagGather = 2;
 
 // One fused pass out <- alpha*A*x + beta*z with optional dot partials:
 //   MODE_DBL: d0 = <x|x>, d1 = <x|out>;  MODE_LR: d0 = <l|out>.

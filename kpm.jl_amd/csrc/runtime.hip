@@ -62,8 +62,9 @@ int work_block(kpm_ctx* ctx, Buffer& buf, int64_t NH, int64_t NR, int prec, kpm_
 
 // identity and row order of a new operator: the serial keys the merged
 // operator of the dual pass (a freed operator's address can be reused, its
-// serial cannot), and a merge of two operators needs strictly increasing
-// column indices in every row
+// serial cannot), a merge of two operators needs strictly increasing column
+// indices in every row, and the fused pass reuses the diagonal's gather when
+// every row stores its diagonal within the pass's gather slots
 static void stamp_csr(kpm_csr* A, const int* rp, const int* ci) {
   static std::atomic<uint64_t> next{1};
   A->serial = next.fetch_add(1);
@@ -75,6 +76,17 @@ static void stamp_csr(kpm_csr* A, const int* rp, const int* ci) {
         break;
       }
   A->sorted = sorted ? 1 : 0;
+  int slots = 0;
+  for (int64_t i = 0; i < A->n && slots < INT32_MAX; ++i) {
+    int pos = INT32_MAX;
+    for (int k = rp[i]; k < rp[i + 1]; ++k)
+      if (ci[k] == (int)i) {
+        pos = k - rp[i] + 1;
+        break;
+      }
+    slots = std::max(slots, pos);
+  }
+  A->diag_slots = slots;
 }
 
 static void free_buffer(Buffer& b) {

@@ -179,6 +179,12 @@ StepGeom step_geom(const kpm_csr* A, int64_t NR, int prec, const StepConfig& cfg
   return g;
 }
 
+// the fused pass's flag bits for operator A at geometry g (the kernel reads
+// kStepDiagGather only in its doubling modes)
+static int step_flags(const kpm_csr* A, const StepGeom& g) {
+  return (g.has_tail ? kStepTail : 0) | (A->diag_slots <= g.kb ? kStepDiagGather : 0);
+}
+
 static StepLauncher pick(int prec, const StepGeom& g, bool realh, int mode) {
   return prec == KPM_VEC_F32 ? pick_step_f32(g.lpr, g.unr, g.kb, realh, mode)
                              : pick_step_f64(g.lpr, g.unr, g.kb, realh, mode);
@@ -252,7 +258,7 @@ int launch_step(kpm_ctx* ctx, const StepLaunch& s, const StepConfig& cfg, int* g
   if (!vals) return fail(KPM_EINVAL, "operator has no FP32 values");
   fn(dim3(G, g.chunks), ctx->stream, s.A->rowptr, s.A->colind, vals, x->d, s.z ? s.z->d : x->d,
      s.out->d, s.l ? s.l->d : nullptr, part, (int)x->NH, g.RW, (int)x->NRp,
-     (int)ntiles, s.alpha, s.beta, g.has_tail, nullptr, nullptr, nullptr, s.gamma, pc, s.vals2,
+     (int)ntiles, s.alpha, s.beta, step_flags(s.A, g), nullptr, nullptr, nullptr, s.gamma, pc, s.vals2,
      s.out2 ? s.out2->d : nullptr, o2, chunk, chunk > 0 ? cfg.xmap : 0);
   KPM_HIP_TRY(hipGetLastError());
   ++ctx->counters[KPM_COUNT_STEP];
@@ -269,7 +275,8 @@ int launch_step_seg(kpm_ctx* ctx, const StepLaunch& s, const StepGeom& g, int G,
   if (!vals) return fail(KPM_EINVAL, "operator has no FP32 values");
   fn(dim3(G, g.chunks), ctx->stream, s.A->rowptr, s.A->colind, vals, x->d, s.z ? s.z->d : x->d,
      s.out->d, nullptr, partials, (int)x->NH, g.RW, (int)x->NRp, (int)ntiles, s.alpha, s.beta,
-     g.has_tail, tile_lo, treal, seg0, 0.0, PendingCombine{}, nullptr, nullptr, SplitOut{}, 0, 0);
+     step_flags(s.A, g), tile_lo, treal, seg0, 0.0, PendingCombine{}, nullptr, nullptr, SplitOut{}, 0,
+     0);
   KPM_HIP_TRY(hipGetLastError());
   ++ctx->counters[KPM_COUNT_STEP];
   return KPM_OK;

@@ -73,6 +73,11 @@ struct Word<KPM_VEC_F64> {
     return make_double2(re[0], im[0]);
   }
   __device__ static inline T zero() { return make_double2(0.0, 0.0); }
+  // p ? a : b per component (a select of the whole vector type is lowered
+  // through private memory)
+  __device__ static inline T pick(bool p, const T& a, const T& b) {
+    return make_double2(p ? a.x : b.x, p ? a.y : b.y);
+  }
 };
 
 template <>
@@ -99,6 +104,9 @@ struct Word<KPM_VEC_F32> {
     return make_float4((float)re[0], (float)im[0], (float)re[1], (float)im[1]);
   }
   __device__ static inline T zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
+  __device__ static inline T pick(bool p, const T& a, const T& b) {
+    return make_float4(p ? a.x : b.x, p ? a.y : b.y, p ? a.z : b.z, p ? a.w : b.w);
+  }
 };
 
 constexpr int kBlock = 64 * kStepWaves;  // 4 waves
@@ -364,13 +372,20 @@ struct LaneMeta {
 // block w / o2.cw, o2.qs words apart, at row * o2.rw + w % o2.cw (one block
 // of RW words per row is the layout of out; narrower blocks split a wide row
 // into separate arrays).
+//
+// flags: bit 0 (kStepTail), rows longer than KB exist (the tail loop runs);
+// bit 1 (kStepDiagGather), every row of the operator stores its diagonal among
+// its first KB nonzeros, so the doubling modes take the own-row words x[row]
+// that their dots need from the gather of that slot instead of loading them
+// again (the slot whose column equals the row; the load it saves is a sixth of
+// the pass's vector loads on 4-entry rows).
 template <int PREC, int LPR, int UNR, int KB, bool REALH, int MODE, bool SEG>
 __global__ void __launch_bounds__(kBlock)
     k_step(const int* __restrict__ rowptr, const int* __restrict__ colind,
            const void* __restrict__ vals_, const typename Word<PREC>::T* __restrict__ x,
            const typename Word<PREC>::T* z, typename Word<PREC>::T* out,
            const typename Word<PREC>::T* __restrict__ l, double2* __restrict__ partials, int NH,
-           int RW, int NRp, int ntiles, double alpha, double beta, int has_tail,
+           int RW, int NRp, int ntiles, double alpha, double beta, int flags,
            const int* __restrict__ tile_lo, const int* __restrict__ treal,
            const int* __restrict__ seg0, double gamma, PendingCombine pc,
            const void* __restrict__ vals2_, typename Word<PREC>::T* __restrict__ out2,
@@ -401,6 +416,8 @@ __global__ void __launch_bounds__(kBlock)
   const int wcol = blockIdx.y * LPR + sub;
   const int rbase = wave * (RPW * UNR) + rslot;
   const bool has_z = beta != 0.0;  // uniform
+  const bool has_tail = (flags & kStepTail) != 0;
+  const bool dgat = kDbl && (flags & kStepDiagGather) != 0;
   size_t o2col = 0;
   if constexpr (MODE == MODE_DUAL) {
     const int qb = wcol / o2.cw;
@@ -622,6 +639,7 @@ __global__ void __launch_bounds__(kBlock)
     AT hr[UNR][KB], hi[UNR][KB];
     AT gr[kDual ? UNR : 1][kDual ? KB : 1], gi[kDual ? UNR : 1][kDual ? KB : 1];
     T xv[UNR][KB];
+    bool own[kDbl ? UNR : 1][kDbl ? KB : 1];  // slot jj of row u is the row's own column
     if constexpr (kLaneMeta) {  // every lane is active here (uniform loop)
 #pragma unroll
       for (int u = 0; u < UNR; ++u)
@@ -629,6 +647,7 @@ __global__ void __launch_bounds__(kBlock)
           constexpr int jj = decltype(J)::value;
           constexpr int e = jj / MG, src = jj % MG;
           const int c = group_bcast<MG, src>(C0.c[u][e]);
+          if constexpr (kDbl) own[u][jj] = c == R0.row[u];
           hr[u][jj] = group_bcast<MG, src>(C0.hr[u][e]);
           hi[u][jj] = REALH ? (AT)0 : group_bcast<MG, src>(C0.hi[u][e]);
           if constexpr (kDual) {
@@ -654,6 +673,7 @@ __global__ void __launch_bounds__(kBlock)
             hi[u][jj] = v ? (AT)h.y : (AT)0;
           }
           xv[u][jj] = x[(size_t)C0.c[u][jj] * RW + wcol];
+          if constexpr (kDbl) own[u][jj] = C0.c[u][jj] == R0.row[u];
         }
     }
     T zv[UNR], sv[UNR];
@@ -662,7 +682,9 @@ __global__ void __launch_bounds__(kBlock)
       const int rc = R0.row[u] < NH ? R0.row[u] : 0;
       const size_t off = (size_t)rc * RW + wcol;
       zv[u] = ld_stream(z + off);  // z is always a valid block; unused when beta == 0
-      if constexpr (kDbl) sv[u] = x[off];
+      if constexpr (kDbl) {
+        if (!dgat) sv[u] = x[off];
+      }
       if constexpr (MODE == MODE_LR || MODE == MODE_SRC) sv[u] = l[off];
     }
     // 2. prefetch: column indices of the next tile, row pointers of the one after
@@ -708,6 +730,17 @@ __global__ void __launch_bounds__(kBlock)
             mac((AT)g.x, (AT)g.y, xw, y2r[u], y2i[u]);
           }
         }
+    }
+
+    if constexpr (kDbl) {
+      if (dgat) {  // own-row words from the gathered slot (rows past NH: any slot, unused)
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+          sv[u] = xv[u][0];
+#pragma unroll
+          for (int jj = 1; jj < KB; ++jj) sv[u] = WT::pick(own[u][jj], xv[u][jj], sv[u]);
+        }
+      }
     }
 
     // 4. combine with z, store, fused dots

@@ -4,9 +4,13 @@ counterpart; the reference has no tuning knob).
 ``autotune(H, NR, precision)`` times every candidate geometry of the kpm_1d
 hot-loop pass (rows in flight per lane x a persistent grid of 2..16
 workgroups per CU or a one-shot grid of 2..64 row tiles per workgroup) on
-the caller's operator and a probe-sized block, and records the fastest for
-that shape in the library's table; every later engine call on the same shape
-uses it.
+the caller's operator and a probe-sized block of random-phase probes, and
+records the fastest for that shape in the library's table; every later engine
+call on the same shape uses it.  The blocks hold probes because the pass's
+speed depends on the values it moves: on all-zero blocks the schedules rank
+differently (graphene 2M sites, NR 64, FP32: per-XCD runs of 8 chunks time
+fastest on zeros, 561 us against 568-570 us for runs of 2 or none, but slowest
+on random phases, 587 us against 569-575 us).
 """
 from __future__ import annotations
 
@@ -20,8 +24,8 @@ def autotune(H, NR: int, precision="f64", reps: int = 8) -> dict:
     ctx = get_context()
     Hd = to_device(H, ctx)
     n = Hd.shape[0]
-    x = DeviceBlock(n, NR, precision, ctx)
-    z = DeviceBlock(n, NR, precision, ctx)
+    x = DeviceBlock(n, NR, precision, ctx).random_phase(0x5EED)
+    z = DeviceBlock(n, NR, precision, ctx).random_phase(0x5EED + 1)
     unroll, grid, ms = C.c_int(0), C.c_int(0), C.c_float(0.0)
     check(ctx.lib.kpm_autotune_step(ctx.handle, Hd.handle, x.handle, z.handle, int(reps),
                                     C.byref(unroll), C.byref(grid), C.byref(ms)), "autotune")
