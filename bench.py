@@ -614,9 +614,13 @@ def bench_conductivity(args) -> int:
     flops = 8.0 * NC * NC * NH * NR  # 4 real products per complex multiply-add
     Kc = NH * K.moment._padded_nr(NR, 0)
     rl = {}
+    # the contraction kernels on slots of pseudo-random values, as kpm_2d
+    # stores them: FP64 slots for the FP64 and FP32 modes, split-BF16 planes
+    # (vec_prec 2) for the split-BF16 mode (k_contract_planes_t128)
     for gemm, code, peak in (("f64", 0, 78.6), ("f32", 1, 157.3), ("bf16x3", 2, 2500.0 / 3)):
         ms = ctypes.c_float(0.0)
-        K._lib.check(ctx.lib.kpm_bench_contract(ctx.handle, NC, NC, Kc, 0, 0, code, 3, ctypes.byref(ms)),
+        vp = 2 if code == 2 else 0
+        K._lib.check(ctx.lib.kpm_bench_contract(ctx.handle, NC, NC, Kc, 0, vp, code, 3, ctypes.byref(ms)),
                      "bench_contract")
         t = ms.value / 3 * 1e-3
         # executed MFMA flops: the FP64 mode runs the 3M complex product
@@ -624,7 +628,7 @@ def bench_conductivity(args) -> int:
         executed = flops * 0.75 if code == 0 else flops
         rl[gemm] = {"bound": "mfma", "achieved": round(executed / t / 1e12, 2), "peak": peak,
                     "unit": "TFLOP/s", "frac": round(executed / t / 1e12 / peak, 4), "traffic": None,
-                    "kernel": ("k_contract_bf16x3<double>" if code == 2 else
+                    "kernel": ("k_contract_planes_t128" if code == 2 else
                                f"k_contract<double,{'double' if code == 0 else 'float'}>"),
                     "complex_gemm_4m_equivalent_tflops": round(flops / t / 1e12, 2),
                     "avg_launch_ms": round(t * 1e3, 3)}

@@ -44,6 +44,8 @@ extern "C" {
  * FP64 moment accumulation. */
 #define KPM_VEC_F64 0
 #define KPM_VEC_F32 1
+/* kpm_bench_contract only: slots held as split-BF16 hi/lo planes */
+#define KPM_VEC_PLANES 2
 
 /* kpm_2d contraction arithmetic: FP64 MFMA; FP32 MFMA (FP64 promotion every
  * 256 reals of K); 3-pass split BF16 (hi*hi + hi*lo + lo*hi on BF16 MFMA,
@@ -88,6 +90,13 @@ int kpm_ctx_synchronize(kpm_ctx* ctx);
  * row).  enable = 0: two separate passes.  The tables are the same either
  * way; the switch exists to show that. */
 int kpm_ctx_set_dual_pass(kpm_ctx* ctx, int enable);
+/* No counterpart in the reference (a switch of this engine).  enable = 1
+ * (default): in the split-BF16 contraction mode, when every contraction of a
+ * kpm_2d call has 128 or more slots a side, the recurrence passes write the
+ * slots as hi/lo BF16 planes (8 bytes per complex value) and the contraction
+ * reads them without a split.  enable = 0: FP64 / FP32 slots, split on every
+ * read.  The tables are bitwise the same either way. */
+int kpm_ctx_set_bf16_planes(kpm_ctx* ctx, int enable);
 /* No counterpart in the reference (a limit of this engine).  The largest
  * merged operator, in bytes, the dual pass may allocate; a larger one makes
  * kpm_2d / kpm_3d run the two passes separately, as when the allocation
@@ -427,9 +436,13 @@ int kpm_bench_copy(kpm_ctx* ctx, kpm_block* dst, const kpm_block* src, int64_t r
 /* Measurement helper (no reference counterpart): total ms of `reps`
  * back-to-back kpm_2d contractions of nL x nR slots of K complex values
  * (K a multiple of 16; the _kpm_2d_blocked! GEMM, src/moment.jl:1055-1061),
- * after one untimed launch, on scratch slots the call allocates and frees.
- * pitch: complex values between consecutive slots (>= K, a multiple of 8);
- * 0 selects kpm_2d's own pitch, K plus 256 bytes. */
+ * after one untimed launch, on scratch slots the call allocates, fills with
+ * pseudo-random values in [-0.5, 0.5) (the MFMA kernels run measurably
+ * faster on all-zero operands) and frees.  pitch: complex values between
+ * consecutive slots (>= K, a multiple of 8); 0 selects kpm_2d's own pitch, K
+ * plus 256 bytes.  vec_prec KPM_VEC_PLANES (with gemm_mode KPM_GEMM_BF16X3,
+ * nL and nR >= 128): slots stored as split-BF16 planes, as kpm_2d writes them
+ * in that mode (kpm_ctx_set_bf16_planes). */
 int kpm_bench_contract(kpm_ctx* ctx, int64_t nL, int64_t nR, int64_t K, int64_t pitch, int vec_prec,
                        int gemm_mode, int64_t reps, float* ms);
 /* HIP-event timer on the context stream: record into slot 0..15, read the
@@ -444,8 +457,8 @@ int kpm_event_elapsed(kpm_ctx* ctx, int slot_a, int slot_b, float* ms);
  * row tiles per workgroup) on the caller's operator and blocks X, Z (Z is
  * overwritten), records the fastest for this shape and returns it
  * (best_grid: the workgroups it launches).  The probe-summed doubling pass
- * is timed over the same candidates and recorded on its own
- * (kpm_tune_get_mode).  X and Z should hold data like the engine's (random
+ * and the plain pass without dots are timed over the same candidates and
+ * recorded on their own (kpm_tune_get_mode).  X and Z should hold data like the engine's (random
  * phases, say): the pass's speed depends on the values, and all-zero blocks
  * rank the schedules differently. */
 int kpm_autotune_step(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z, int reps,
@@ -463,10 +476,11 @@ int kpm_tune_set_chunked(kpm_ctx* ctx, const kpm_csr* H, int64_t NR, int vec_pre
 /* The launch shape the engines use for a shape (chunk 0: persistent grid). */
 int kpm_tune_get(kpm_ctx* ctx, const kpm_csr* H, int64_t NR, int vec_prec, int* unroll, int* grid,
                  int* chunk, int* xcd_grouped);
-/* The same for either pass the tuner times separately: mode 0 the
- * per-probe passes (kpm_tune_get), mode 1 the probe-summed doubling pass of
- * kpm_1d_avg / kpm_1d_block_avg.  kpm_tune_set and kpm_tune_set_chunked set
- * both. */
+/* The same for each pass the tuner times separately: mode 0 the per-probe
+ * passes with dots (kpm_tune_get), mode 1 the probe-summed doubling pass of
+ * kpm_1d_avg / kpm_1d_block_avg, mode 2 the plain passes without dots
+ * (kpm_2d / kpm_3d chains, the matrix-function action, the adjoint's source
+ * steps).  kpm_tune_set and kpm_tune_set_chunked set all three. */
 int kpm_tune_get_mode(kpm_ctx* ctx, const kpm_csr* H, int64_t NR, int vec_prec, int mode,
                       int* unroll, int* grid, int* chunk, int* xcd_grouped);
 

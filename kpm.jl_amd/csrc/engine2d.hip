@@ -24,6 +24,7 @@
 #include <climits>
 #include <vector>
 
+#include "bf16_planes.h"
 #include "kpm_internal.h"
 
 namespace kpm {
@@ -245,9 +246,165 @@ struct Plan2d {
   int parity, mn_sym, prec, gemm_mode;
 };
 
+// ---- split-BF16 mode on slots stored as planes ---------------------------------
+//
+// In the split-BF16 contraction mode the contraction reads each slot only
+// through its hi/lo BF16 split.  So the passes that produce the slots write
+// that split (bf16_planes.h, 8 bytes per complex value) instead of the FP64
+// or FP32 words the contraction would split on every read: the right slots
+// Jb T_n Ja psi_r are written only as planes (the dual pass's second product,
+// and the Jb pass), and the left chain runs on a ring of three blocks whose
+// step also writes the planes of its output into the left slot.  The
+// contraction then moves half the operand bytes per stage and does no split
+// (k_contract_planes_t128); its table is bitwise the one from the split of
+// the stored slots.  Taken when every contraction of the call has 128 or
+// more slots a side (the 128 x 128 tile kernel).
+
+// planes of a block's first n complex values
+template <typename E>
+__global__ void __launch_bounds__(256)
+    k_block_planes(const E* __restrict__ src, int64_t n, uint32_t* __restrict__ planes) {
+  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256)
+    store_plane_value(planes, k, (double)src[k].x, (double)src[k].y);
+}
+
+static int block_planes(kpm_ctx* ctx, const kpm_block* b, void* planes) {
+  const int64_t n = b->NH * b->NRp;
+  const unsigned grid = (unsigned)std::max<int64_t>(
+      1, std::min<int64_t>((n + 255) / 256, (int64_t)std::max(ctx->num_cus, 8) * 16));
+  if (b->prec == KPM_VEC_F32)
+    hipLaunchKernelGGL(k_block_planes<float2>, dim3(grid), dim3(256), 0, ctx->stream,
+                       (const float2*)b->d, n, (uint32_t*)planes);
+  else
+    hipLaunchKernelGGL(k_block_planes<double2>, dim3(grid), dim3(256), 0, ctx->stream,
+                       (const double2*)b->d, n, (uint32_t*)planes);
+  KPM_HIP_TRY(hipGetLastError());
+  return KPM_OK;
+}
+
+// out = alpha*A*x + beta*z, and the planes of out at `planes` (only the
+// planes when planes_only)
+static int spmm_planes(kpm_ctx* ctx, const kpm_csr* A, const kpm_block* x, const kpm_block* z,
+                       kpm_block* out, double alpha, double beta, void* planes, bool planes_only) {
+  StepLaunch s{A, x, z, out, nullptr, alpha, beta, MODE_NONE};
+  s.planes = planes;
+  s.planes_only = planes_only;
+  return launch_step(ctx, s, tuned_config(ctx, A, x->NR, x->prec, MODE_NONE), nullptr);
+}
+
+// out = alpha*H*x + beta*z, and the planes of Jb*x at `planes` (dual pass)
+static int spmm_dual_planes(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* M, const void* vals2,
+                            const kpm_block* x, const kpm_block* z, kpm_block* out, void* planes,
+                            double alpha, double beta) {
+  StepLaunch s{M, x, z, out, nullptr, alpha, beta, MODE_DUAL};
+  s.gamma = 1.0;
+  s.vals2 = vals2;
+  s.planes = planes;
+  return launch_step(ctx, s, tuned_config(ctx, H, x->NR, x->prec, MODE_NONE), nullptr);
+}
+
+// every contraction run_2d would launch has 128 or more slots a side
+static bool plane_tables(const Plan2d& p) {
+  if (p.gemm_mode != KPM_GEMM_BF16X3 || p.K_left < 128 || p.K_right < 128) return false;
+  const int64_t reps = (p.NC + p.K_left - 1) / p.K_left;
+  for (int64_t rep = 1; rep <= reps; ++rep) {
+    const int64_t m1 = (rep - 1) * p.K_left + 1, m2 = std::min(rep * p.K_left, p.NC);
+    if (m2 - m1 + 1 < 128) return false;
+    const int64_t nmax = p.mn_sym ? m2 : p.NC;
+    for (int64_t n1 = 1; n1 <= nmax; n1 += p.K_right)
+      if (std::min(n1 + p.K_right - 1, nmax) - n1 + 1 < 128) return false;
+  }
+  return true;
+}
+
+static int run_2d_planes(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, const kpm_csr* Jb,
+                         const Plan2d& p, const kpm_block* psi_l, const kpm_block* psi_r, void* mu_dev) {
+  const int64_t NH = p.NH, NR = p.NR, NC = p.NC;
+  const int prec = p.prec;
+  const int64_t NRp = padded_nr(NR, prec);
+  const int64_t used = NH * NRp;
+  const int64_t Kp = (used + 15) / 16 * 16;
+  const int64_t eb = elem_bytes(prec);
+  const int64_t pitch = Kp + 256 / eb;   // complex values between slots, as in run_2d
+  const int64_t stride = pitch * 8;      // plane bytes per slot
+  const int64_t KL = p.K_left, KR = p.K_right;
+  KPM_TRY(ensure(ctx, ctx->slots_l, (size_t)(stride * KL)));
+  KPM_TRY(ensure(ctx, ctx->slots_r, (size_t)(stride * KR)));
+  if (Kp > used) {  // the zero tail of each slot (the partial block's written words come later)
+    const int64_t t0 = used / 16 * 128;
+    KPM_HIP_TRY(hipMemset2DAsync((char*)ctx->slots_l.ptr + t0, (size_t)stride, 0, (size_t)(Kp * 8 - t0),
+                                 (size_t)KL, ctx->stream));
+    KPM_HIP_TRY(hipMemset2DAsync((char*)ctx->slots_r.ptr + t0, (size_t)stride, 0, (size_t)(Kp * 8 - t0),
+                                 (size_t)KR, ctx->stream));
+  }
+  auto lp = [&](int64_t i) { return (void*)((char*)ctx->slots_l.ptr + stride * i); };
+  auto rp = [&](int64_t i) { return (void*)((char*)ctx->slots_r.ptr + stride * i); };
+  kpm_block rr[3], jpsi0r, lr[3], sink;
+  for (int i = 0; i < 3; ++i) KPM_TRY(work_block(ctx, ctx->ring[i], NH, NR, prec, &rr[i]));
+  KPM_TRY(work_block(ctx, ctx->ring[3], NH, NR, prec, &jpsi0r));
+  KPM_TRY(work_block(ctx, ctx->ring[4], NH, NR, prec, &sink));  // out of planes-only passes (not written)
+  for (int i = 0; i < 3; ++i) KPM_TRY(work_block(ctx, ctx->ring[5 + i], NH, NR, prec, &lr[i]));
+
+  kpm_csr M;
+  const void* v2 = nullptr;
+  bool dual = false;
+  KPM_TRY(dual_operator(ctx, H, Jb, NR, prec, &M, &v2, &dual));
+  KPM_HIP_TRY(hipMemsetAsync(mu_dev, 0, (size_t)(NC * NC) * sizeof(double2), ctx->stream));
+  KPM_TRY(spmm(ctx, Ja, psi_r, nullptr, &jpsi0r, 1.0, 0.0));  // right starter (:1021)
+
+  const int64_t reps = (NC + KL - 1) / KL;
+  const double scale = 1.0 / (double)NR;
+  for (int64_t rep = 1; rep <= reps; ++rep) {
+    const int64_t m1 = (rep - 1) * KL + 1;
+    const int64_t m2 = std::min(rep * KL, NC);
+    const int64_t rep_size = m2 - m1 + 1;
+    // left orders m1-1 .. m2-1 (0-based): the starters (:1013-1018), then
+    // chebyshev_iter and, from the second rep, chebyshev_iter_wrap, which
+    // continue the same recurrence on the ring
+    for (int64_t o = m1 - 1; o < m2; ++o) {
+      void* pl = lp(o - (m1 - 1));
+      if (o == 0) {
+        KPM_TRY(kpm_block_copy(&lr[0], psi_l));
+        KPM_TRY(block_planes(ctx, psi_l, pl));
+      } else if (o == 1) {
+        KPM_TRY(spmm_planes(ctx, H, psi_l, nullptr, &lr[1], 1.0, 0.0, pl, false));
+      } else {
+        KPM_TRY(spmm_planes(ctx, H, &lr[(o - 1) % 3], &lr[(o - 2) % 3], &lr[o % 3], 2.0, -1.0, pl, false));
+      }
+    }
+    const int64_t nmax = p.mn_sym ? m2 : NC;
+    for (int64_t n1 = 1; n1 <= nmax; n1 += KR) {
+      const int64_t n2 = std::min(n1 + KR - 1, nmax);
+      const int64_t n_count = n2 - n1 + 1;
+      for (int64_t j = 1; j <= n_count; ++j) {
+        const int64_t n = n1 + j - 1;
+        const int ri = (int)((n - 1) % 3), rip = (int)((n + 1) % 3), ripp = (int)(n % 3);
+        const bool fuse = dual && j >= 2;  // this step also emits the planes of R order n - 1
+        if (n == 1) {
+          KPM_TRY(kpm_block_copy(&rr[0], &jpsi0r));
+        } else if (n == 2) {
+          if (fuse) KPM_TRY(spmm_dual_planes(ctx, H, &M, v2, &jpsi0r, nullptr, &rr[1], rp(j - 2), 1.0, 0.0));
+          else KPM_TRY(spmm(ctx, H, &jpsi0r, nullptr, &rr[1], 1.0, 0.0));
+        } else {
+          if (fuse)
+            KPM_TRY(spmm_dual_planes(ctx, H, &M, v2, &rr[rip], &rr[ripp], &rr[ri], rp(j - 2), 2.0, -1.0));
+          else KPM_TRY(spmm(ctx, H, &rr[rip], &rr[ripp], &rr[ri], 2.0, -1.0));
+        }
+        if (!dual) KPM_TRY(spmm_planes(ctx, Jb, &rr[ri], nullptr, &sink, 1.0, 0.0, rp(j - 1), true));
+      }
+      if (dual)  // the block's last order
+        KPM_TRY(spmm_planes(ctx, Jb, &rr[(int)((n2 - 1) % 3)], nullptr, &sink, 1.0, 0.0, rp(n_count - 1), true));
+      KPM_TRY(launch_contract_planes(ctx, ctx->slots_l.ptr, (int)rep_size, ctx->slots_r.ptr, (int)n_count,
+                                     Kp, pitch, scale, mu_dev, NC, n1 - 1, m1 - 1, p.parity));
+    }
+  }
+  return KPM_OK;
+}
+
 // mu_dev: NC x NC column-major double2, overwritten
 static int run_2d(kpm_ctx* ctx, const kpm_csr* H, const kpm_csr* Ja, const kpm_csr* Jb,
                   const Plan2d& p, const kpm_block* psi_l, const kpm_block* psi_r, void* mu_dev) {
+  if (ctx->bf16_planes && plane_tables(p)) return run_2d_planes(ctx, H, Ja, Jb, p, psi_l, psi_r, mu_dev);
   const int64_t NH = p.NH, NR = p.NR, NC = p.NC;
   const int prec = p.prec;
   const int64_t NRp = padded_nr(NR, prec);

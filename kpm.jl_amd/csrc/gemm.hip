@@ -30,6 +30,8 @@
 
 #include <algorithm>
 
+#include "bf16_planes.h"
+
 namespace kpm {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -272,19 +274,7 @@ __global__ void __launch_bounds__(256, 2)
 // double buffering, XCD-aware K-chunk dispatch) is k_contract's; the LDS holds
 // the hi and lo planes as bf16 pairs.
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
-
-typedef float f2v __attribute__((ext_vector_type(2)));
-typedef __bf16 b2v __attribute__((ext_vector_type(2)));
-
-// (hi, lo) bf16 splits of the pair (x0, x1), packed x0 in the low half: the
-// FP32 -> BF16 conversions are gfx950's packed v_cvt_pk_bf16_f32 (round to
-// nearest even), the residual x - hi is exact in FP32
-__device__ __forceinline__ void split_bf16_pair(float x0, float x1, uint32_t& hi, uint32_t& lo) {
-  hi = __builtin_bit_cast(uint32_t, __builtin_convertvector((f2v){x0, x1}, b2v));
-  const float r0 = x0 - __uint_as_float(hi << 16);
-  const float r1 = x1 - __uint_as_float(hi & 0xFFFF0000u);
-  lo = __builtin_bit_cast(uint32_t, __builtin_convertvector((f2v){r0, r1}, b2v));
-}
+// split_bf16_pair: bf16_planes.h
 
 // L'' = (-Li, Lr) on packed (Re, Im) bf16 pairs: swap the halves, negate the new low half
 __device__ __forceinline__ uint4 swap_neg(uint4 w) {
@@ -565,6 +555,126 @@ __global__ void __launch_bounds__(kT128Threads)
       }
 }
 
+// The same contraction from operands already stored as split-BF16 planes
+// (bf16_planes.h), written by the recurrence passes that produce the slots.
+// A stage row is then 128 contiguous bytes (the 16 hi words, then the 16 lo
+// words, exactly the LDS row of the kernel above without its pad), so the
+// stage is copied to LDS without a split, and the operand bytes read per
+// stage are half those of FP64 slots.  The LDS layout, the fragment reads and
+// the MFMA order are those of k_contract_bf16x3_t128, so the partials are
+// bitwise the same as from the FP64 (or FP32) slots the planes were split
+// from.  (Measured on the 512 x 512, K = 2.1M complex table: 11.0 ms against
+// 14.6 ms from FP64 slots, both on random data.)
+__global__ void __launch_bounds__(kT128Threads)
+    k_contract_planes_t128(const uint4* __restrict__ L, int nL, const uint4* __restrict__ R, int nR,
+                           int64_t K2, int64_t P2, int64_t kchunk, int tm, int ntiles, int splits,
+                           float2* __restrict__ part, int ldn, int ldm) {
+  constexpr int TT = 128;
+  constexpr int NT = kT128Threads;
+  constexpr int WN = 2, WM = 2;
+  constexpr int LDB = kStep / 2 + 4;   // LDS row stride in words (16 pairs + pad)
+  constexpr int PW = 8;                // 16-byte words per row and stage: 4 hi, then 4 lo
+  constexpr int PNW = 2 * TT * PW / NT;
+  static_assert(PNW == 2, "two plane words per thread and stage");
+  // [plane][buffer][operand][row * LDB]
+  __shared__ uint32_t lds[2][2][2][TT * LDB];
+  constexpr int PLANE = 2 * 2 * TT * LDB, BUF = 2 * TT * LDB;
+
+  int tile, z;
+  if ((splits & 7) == 0) {
+    const int b = blockIdx.x, local = b >> 3;
+    tile = local % ntiles;
+    z = (local / ntiles) * 8 + (b & 7);
+  } else {
+    tile = blockIdx.x % ntiles;
+    z = blockIdx.x / ntiles;
+  }
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int g = lane >> 4, i = lane & 15;
+  const int tile_n = (tile / tm) * TT, tile_m = (tile % tm) * TT;
+  constexpr int WAVES_M = TT / (16 * WM);
+  const int n0 = (wave / WAVES_M) * (16 * WN), m0 = (wave % WAVES_M) * (16 * WM);
+  const int64_t kbeg = (int64_t)z * kchunk;
+  const int64_t kend = min(kbeg + kchunk, K2);
+  const int nstage = kend > kbeg ? (int)((kend - kbeg) / kStep) : 0;
+
+  // the two words this thread moves per stage: global source, LDS offset
+  const uint4* src[PNW];
+  int loff[PNW];
+#pragma unroll
+  for (int q = 0; q < PNW; ++q) {
+    const int w = tid + NT * q;
+    const int op = w / (TT * PW), rem = w % (TT * PW);
+    const int row = rem / PW, wi = rem % PW;
+    const int grow = op == 0 ? min(tile_n + row, nR - 1) : min(tile_m + row, nL - 1);
+    // a slot of P2 / 2 complex values is P2 words = P2 / 4 16-byte words
+    src[q] = (op == 0 ? R : L) + (size_t)grow * (size_t)(P2 / 4) + kbeg / 4 + wi;
+    loff[q] = (wi >> 2) * PLANE + (op * TT + row) * LDB + (wi & 3) * 4;
+  }
+  uint32_t* const base = &lds[0][0][0][0];
+  uint4 w0, w1;  // (named words, not an array: an array of them went to scratch)
+  auto gload = [&](int stage) {
+    w0 = src[0][(int64_t)stage * PW];
+    w1 = src[1][(int64_t)stage * PW];
+  };
+  auto lstore = [&](int buf) {
+    *reinterpret_cast<uint4*>(base + buf * BUF + loff[0]) = w0;
+    *reinterpret_cast<uint4*>(base + buf * BUF + loff[1]) = w1;
+  };
+
+  f4 are[WN][WM], aim[WN][WM];
+#pragma unroll
+  for (int a = 0; a < WN; ++a)
+#pragma unroll
+    for (int b = 0; b < WM; ++b) are[a][b] = aim[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+
+  if (nstage > 0) {
+    gload(0);
+    lstore(0);
+  }
+  __syncthreads();
+  for (int st = 0; st < nstage; ++st) {
+    const int buf = st & 1;
+    if (st + 1 < nstage) gload(st + 1);
+    uint4 ah[WN], al[WN];
+#pragma unroll
+    for (int t = 0; t < WN; ++t) {
+      const int ra = buf * BUF + (n0 + 16 * t + i) * LDB + 4 * g;
+      ah[t] = *reinterpret_cast<const uint4*>(base + ra);
+      al[t] = *reinterpret_cast<const uint4*>(base + PLANE + ra);
+    }
+#pragma unroll
+    for (int tmi = 0; tmi < WM; ++tmi) {
+      const int rb = buf * BUF + (TT + m0 + 16 * tmi + i) * LDB + 4 * g;
+      const uint4 bh = *reinterpret_cast<const uint4*>(base + rb);
+      const uint4 bl = *reinterpret_cast<const uint4*>(base + PLANE + rb);
+      const uint4 b2h = swap_neg(bh), b2l = swap_neg(bl);
+#pragma unroll
+      for (int tn = 0; tn < WN; ++tn) {
+        are[tn][tmi] = mfma_bf16(al[tn], bh, are[tn][tmi]);
+        are[tn][tmi] = mfma_bf16(ah[tn], bl, are[tn][tmi]);
+        are[tn][tmi] = mfma_bf16(ah[tn], bh, are[tn][tmi]);
+        aim[tn][tmi] = mfma_bf16(al[tn], b2h, aim[tn][tmi]);
+        aim[tn][tmi] = mfma_bf16(ah[tn], b2l, aim[tn][tmi]);
+        aim[tn][tmi] = mfma_bf16(ah[tn], b2h, aim[tn][tmi]);
+      }
+    }
+    if (st + 1 < nstage) lstore(buf ^ 1);
+    __syncthreads();
+  }
+  float2* out = part + (size_t)z * ldn * ldm;
+#pragma unroll
+  for (int tn = 0; tn < WN; ++tn)
+#pragma unroll
+    for (int tmi = 0; tmi < WM; ++tmi)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int n = tile_n + n0 + 16 * tn + 4 * g + v;
+        const int m = tile_m + m0 + 16 * tmi + i;
+        if (n < nR && m < nL) out[(size_t)n * ldm + m] = make_float2(are[tn][tmi][v], aim[tn][tmi][v]);
+      }
+}
+
 // FP32 chunk partials -> FP64 sums in chunk order, then scale / mask / accumulate
 __global__ void k_contract_reduce_f32(const float2* __restrict__ part, int splits, int nR, int nL,
                                       int ldn, int ldm, double scale, double2* __restrict__ mu,
@@ -615,6 +725,7 @@ __global__ void k_contract_reduce(const double2* __restrict__ part, int splits, 
   *dst = cur;
 }
 
+// prec < 0: the slots are split-BF16 planes (k_contract_planes_t128)
 static int launch_contract_bf16_t128(kpm_ctx* ctx, const void* Lslots, int nL, const void* Rslots,
                                      int nR, int64_t K2, int64_t P2, int prec, double scale,
                                      void* mu_dev, int64_t NC, int64_t n0, int64_t m0, int parity) {
@@ -635,7 +746,11 @@ static int launch_contract_bf16_t128(kpm_ctx* ctx, const void* Lslots, int nL, c
   KPM_TRY(ensure(ctx, ctx->gemm_part, (size_t)splits * ldn * ldm * sizeof(float2)));
   const dim3 grid((unsigned)(ntiles * splits));
   const int sp = (int)splits;
-  if (prec == KPM_VEC_F64)
+  if (prec < 0)
+    hipLaunchKernelGGL(k_contract_planes_t128, grid, dim3(kT128Threads), 0, ctx->stream,
+                       (const uint4*)Lslots, nL, (const uint4*)Rslots, nR, K2, P2, kchunk, tm, ntiles, sp,
+                       (float2*)ctx->gemm_part.ptr, ldn, ldm);
+  else if (prec == KPM_VEC_F64)
     hipLaunchKernelGGL((k_contract_bf16x3_t128<double>), grid, dim3(kT128Threads), 0, ctx->stream,
                        (const double*)Lslots, nL, (const double*)Rslots, nR, K2, P2, kchunk, tm, ntiles, sp,
                        (float2*)ctx->gemm_part.ptr, ldn, ldm);
@@ -650,6 +765,16 @@ static int launch_contract_bf16_t128(kpm_ctx* ctx, const void* Lslots, int nL, c
                      (double2*)mu_dev, NC, n0, m0, parity);
   KPM_HIP_TRY(hipGetLastError());
   return KPM_OK;
+}
+
+int launch_contract_planes(kpm_ctx* ctx, const void* Lplanes, int nL, const void* Rplanes, int nR,
+                           int64_t K, int64_t pitch, double scale, void* mu_dev, int64_t NC, int64_t n0,
+                           int64_t m0, int parity) {
+  if ((2 * K) % kStep != 0) return fail(KPM_EINVAL, "contraction slot length must be a multiple of 16");
+  if (pitch < K || pitch % 8 != 0) return fail(KPM_EINVAL, "slot pitch must be >= K and a multiple of 8");
+  if (nL < 128 || nR < 128) return fail(KPM_EINVAL, "plane contraction needs 128 or more slots a side");
+  return launch_contract_bf16_t128(ctx, Lplanes, nL, Rplanes, nR, 2 * K, 2 * pitch, -1, scale, mu_dev, NC,
+                                   n0, m0, parity);
 }
 
 int launch_contract(kpm_ctx* ctx, const void* Lslots, int nL, const void* Rslots, int nR,
@@ -714,6 +839,48 @@ int launch_contract(kpm_ctx* ctx, const void* Lslots, int nL, const void* Rslots
   return KPM_OK;
 }
 
+// pseudo-random values in [-0.5, 0.5) from a 64-bit mix of the index (the
+// splitmix64 finaliser): synthetic contraction operands
+__device__ inline double hash_unit(uint64_t i, uint64_t seed) {
+  uint64_t x = i * 0x9E3779B97F4A7C15ull + seed;
+  x ^= x >> 30;
+  x *= 0xBF58476D1CE4E5B9ull;
+  x ^= x >> 27;
+  x *= 0x94D049BB133111EBull;
+  x ^= x >> 31;
+  return (double)(x >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+}
+
+template <typename E>
+__global__ void k_fill_unit(E* __restrict__ p, int64_t n, uint64_t seed) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = (E)hash_unit((uint64_t)i, seed);
+}
+
+// planes of nslots slots of K complex values, pitch complex values apart
+__global__ void k_fill_unit_planes(uint32_t* __restrict__ p, int64_t nslots, int64_t K, int64_t pitch,
+                                   uint64_t seed) {
+  const int64_t n = nslots * K;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t slot = i / K, k = i - slot * K;
+    store_plane_value(p + slot * 2 * pitch, k, hash_unit(2 * (uint64_t)i, seed),
+                      hash_unit(2 * (uint64_t)i + 1, seed));
+  }
+}
+
+static int fill_operand(kpm_ctx* ctx, void* p, int64_t nslots, int64_t K, int64_t pitch, int vec_prec,
+                        uint64_t seed) {
+  const dim3 grid((unsigned)std::max(ctx->num_cus, 8) * 16), block(256);
+  if (vec_prec == KPM_VEC_PLANES)
+    hipLaunchKernelGGL(k_fill_unit_planes, grid, block, 0, ctx->stream, (uint32_t*)p, nslots, K, pitch, seed);
+  else if (vec_prec == KPM_VEC_F32)
+    hipLaunchKernelGGL(k_fill_unit<float>, grid, block, 0, ctx->stream, (float*)p, nslots * pitch * 2, seed);
+  else
+    hipLaunchKernelGGL(k_fill_unit<double>, grid, block, 0, ctx->stream, (double*)p, nslots * pitch * 2, seed);
+  KPM_HIP_TRY(hipGetLastError());
+  return KPM_OK;
+}
+
 }  // namespace kpm
 
 using namespace kpm;
@@ -724,29 +891,35 @@ int kpm_bench_contract(kpm_ctx* ctx, int64_t nL, int64_t nR, int64_t K, int64_t 
                        int gemm_mode, int64_t reps, float* ms) {
   if (!ctx || !ms || nL < 1 || nR < 1 || reps < 1 || K < 16 || K % 16 != 0)
     return fail(KPM_EINVAL, "bad arguments (K must be a positive multiple of 16)");
+  const bool planes = vec_prec == KPM_VEC_PLANES;
   if (pitch == 0) pitch = K + 256 / elem_bytes(vec_prec == KPM_VEC_F32 ? KPM_VEC_F32 : KPM_VEC_F64);
   if (pitch < K || pitch % 8 != 0) return fail(KPM_EINVAL, "pitch must be >= K and a multiple of 8");
-  if (vec_prec != KPM_VEC_F64 && vec_prec != KPM_VEC_F32) return fail(KPM_EINVAL, "bad vec_prec");
+  if (vec_prec != KPM_VEC_F64 && vec_prec != KPM_VEC_F32 && !planes) return fail(KPM_EINVAL, "bad vec_prec");
   if (gemm_mode != KPM_GEMM_F64 && gemm_mode != KPM_GEMM_F32 && gemm_mode != KPM_GEMM_BF16X3)
     return fail(KPM_EINVAL, "bad gemm_mode");
+  if (planes && (gemm_mode != KPM_GEMM_BF16X3 || nL < 128 || nR < 128))
+    return fail(KPM_EINVAL, "plane slots: split-BF16 mode, 128 or more slots a side");
   KPM_HIP_TRY(hipSetDevice(ctx->device));
-  const size_t eb = (size_t)elem_bytes(vec_prec);
+  const size_t eb = planes ? 8 : (size_t)elem_bytes(vec_prec);  // bytes per complex value
   const int64_t NC = std::max(nL, nR);
   Buffer lb, rb, mb;
   int rc = ensure(ctx, lb, (size_t)(nL * pitch) * eb);
   if (rc == KPM_OK) rc = ensure(ctx, rb, (size_t)(nR * pitch) * eb);
   if (rc == KPM_OK) rc = ensure(ctx, mb, (size_t)(NC * NC) * sizeof(double2));
-  if (rc == KPM_OK && (hipMemsetAsync(lb.ptr, 0, lb.bytes, ctx->stream) != hipSuccess ||
-                       hipMemsetAsync(rb.ptr, 0, rb.bytes, ctx->stream) != hipSuccess ||
-                       hipMemsetAsync(mb.ptr, 0, mb.bytes, ctx->stream) != hipSuccess))
+  if (rc == KPM_OK) rc = fill_operand(ctx, lb.ptr, nL, K, pitch, vec_prec, 1);
+  if (rc == KPM_OK) rc = fill_operand(ctx, rb.ptr, nR, K, pitch, vec_prec, 2);
+  if (rc == KPM_OK && hipMemsetAsync(mb.ptr, 0, mb.bytes, ctx->stream) != hipSuccess)
     rc = fail(KPM_EHIP, "workspace clear failed");
-  if (rc == KPM_OK) rc = launch_contract(ctx, lb.ptr, (int)nL, rb.ptr, (int)nR, K, pitch, vec_prec,
-                                         gemm_mode, 1.0, mb.ptr, NC, 0, 0, KPM_PARITY_NONE);  // warm-up
+  auto one = [&]() {
+    return planes ? launch_contract_planes(ctx, lb.ptr, (int)nL, rb.ptr, (int)nR, K, pitch, 1.0, mb.ptr, NC,
+                                           0, 0, KPM_PARITY_NONE)
+                  : launch_contract(ctx, lb.ptr, (int)nL, rb.ptr, (int)nR, K, pitch, vec_prec, gemm_mode,
+                                    1.0, mb.ptr, NC, 0, 0, KPM_PARITY_NONE);
+  };
+  if (rc == KPM_OK) rc = one();  // warm-up
   if (rc == KPM_OK && hipEventRecord(ctx->events[14], ctx->stream) != hipSuccess)
     rc = fail(KPM_EHIP, "event record failed");
-  for (int64_t r = 0; r < reps && rc == KPM_OK; ++r)
-    rc = launch_contract(ctx, lb.ptr, (int)nL, rb.ptr, (int)nR, K, pitch, vec_prec, gemm_mode, 1.0,
-                         mb.ptr, NC, 0, 0, KPM_PARITY_NONE);
+  for (int64_t r = 0; r < reps && rc == KPM_OK; ++r) rc = one();
   if (rc == KPM_OK && (hipEventRecord(ctx->events[15], ctx->stream) != hipSuccess ||
                        hipEventSynchronize(ctx->events[15]) != hipSuccess ||
                        hipEventElapsedTime(ms, ctx->events[14], ctx->events[15]) != hipSuccess))

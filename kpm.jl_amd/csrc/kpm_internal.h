@@ -117,7 +117,7 @@ struct kpm_ctx {
   kpm::Buffer lwork;      // bra block of the bra/ket engine
   kpm::Buffer slots_l;    // kpm_2d left slot array
   kpm::Buffer slots_r;    // kpm_2d right slot array
-  kpm::Buffer ring[5];    // kpm_2d / kpm_3d ring and starter blocks
+  kpm::Buffer ring[8];    // kpm_2d / kpm_3d ring and starter blocks (5-7: plane mode's left ring)
   kpm::Buffer pinned;     // page-locked host staging for operator uploads
   kpm::Buffer adjoint;    // adjoint workspace: v | beta ring | alpha store
   kpm::Buffer adj_grad;   // adjoint gradient accumulator (nnz complex)
@@ -131,6 +131,7 @@ struct kpm_ctx {
   int merged_max_row = 0;
   int64_t merged_nnz = 0;
   int dual_pass = 1;      // fuse the kpm_2d / kpm_3d right chain with the Jb pass
+  int bf16_planes = 1;    // split-BF16 mode: slots written as planes (kpm_2d, 128+ slots a side)
   int64_t merged_limit = -1;  // bytes the merged dual-pass operator may take (< 0: no limit)
   // launch / synchronisation counters (kpm_ctx_counters)
   int64_t counters[KPM_COUNT_N] = {};
@@ -220,12 +221,17 @@ int work_block(kpm_ctx* ctx, Buffer& buf, int64_t NH, int64_t NR, int prec, kpm_
 // workgroup and row chunk
 constexpr int kStepWaves = 4;  // waves per workgroup of the fused pass
 enum StepMode { MODE_NONE = 0, MODE_DBL = 1, MODE_LR = 2, MODE_DBL_SEG = 3, MODE_SRC = 4,
-                MODE_DUAL = 5, MODE_DBLA = 6 };
+                MODE_DUAL = 5, MODE_DBLA = 6, MODE_DUALP = 7 };
 // flag bits of the fused pass: rows longer than the gather slots exist; the
 // doubling modes may take the own-row words from the diagonal's gather slot
 // (kpm_csr::diag_slots <= slots)
 constexpr int kStepTail = 1;
 constexpr int kStepDiagGather = 2;
+// ... and (MODE_NONE) write the split-BF16 contraction planes (bf16_planes.h)
+// of out to out2, with kStepPlanesOnly instead of out; MODE_DUALP writes the
+// dual pass's second product as planes
+constexpr int kStepPlanes = 4;
+constexpr int kStepPlanesOnly = 8;
 
 // One fused pass out <- alpha*A*x + beta*z with optional dot partials:
 //   MODE_DBL: d0 = <x|x>, d1 = <x|out>;  MODE_LR: d0 = <l|out>.
@@ -248,6 +254,10 @@ struct StepLaunch {
   // MODE_DUAL: out2 split into blocks of out2_cols complex columns, each an
   // NH x out2_cols array, out2_stride complex values apart (0: out's layout)
   int64_t out2_cols = 0, out2_stride = 0;
+  // split-BF16 contraction planes of this pass's contraction operand (out in
+  // MODE_NONE, the second product in MODE_DUAL), one slot at this address
+  void* planes = nullptr;
+  bool planes_only = false;  // MODE_NONE: write the planes instead of out
 };
 // Word placement of the dual pass's second output (words of the block type):
 // word w of a row goes to block w / cw, qs words apart, at row * rw + w % cw.
@@ -341,6 +351,12 @@ int launch_copy16(kpm_ctx* ctx, const void* src, void* dst, int64_t nwords);
 int launch_contract(kpm_ctx* ctx, const void* Lslots, int nL, const void* Rslots, int nR,
                     int64_t K, int64_t pitch, int prec, int gemm_mode, double scale, void* mu_dev,
                     int64_t NC, int64_t n0, int64_t m0, int parity);
+// the split-BF16 contraction of slots stored as planes (bf16_planes.h; 2 *
+// pitch words per slot), for 128 or more slots a side: bitwise the table the
+// split-BF16 mode of launch_contract gives from the slots they were split from
+int launch_contract_planes(kpm_ctx* ctx, const void* Lplanes, int nL, const void* Rplanes, int nR,
+                           int64_t K, int64_t pitch, double scale, void* mu_dev, int64_t NC, int64_t n0,
+                           int64_t m0, int parity);
 
 // ---- reverse mode (adjoint.hip) ---------------------------------------------------
 // window of reverse steps whose gradient terms one SDDMM pass adds

@@ -397,6 +397,12 @@ int kpm_ctx_set_dual_pass(kpm_ctx* ctx, int enable) {
   return KPM_OK;
 }
 
+int kpm_ctx_set_bf16_planes(kpm_ctx* ctx, int enable) {
+  if (!ctx) return fail(KPM_EINVAL, "null context");
+  ctx->bf16_planes = enable ? 1 : 0;
+  return KPM_OK;
+}
+
 int kpm_ctx_set_merged_limit(kpm_ctx* ctx, int64_t bytes) {
   if (!ctx) return fail(KPM_EINVAL, "null context");
   ctx->merged_limit = bytes;

@@ -240,7 +240,7 @@ int launch_step(kpm_ctx* ctx, const StepLaunch& s, const StepConfig& cfg, int* g
     part = (double2*)ctx->partials.ptr;
   }
   if ((s.mode == MODE_LR || s.mode == MODE_SRC) && !s.l) return fail(KPM_EINVAL, "missing operand block");
-  if (s.mode == MODE_DUAL && (!s.vals2 || !s.out2 || s.A->is_real))
+  if (s.mode == MODE_DUAL && (!s.vals2 || (!s.out2 && !s.planes) || s.A->is_real))
     return fail(KPM_EINVAL, "dual pass needs complex merged values and a second output");
   SplitOut o2{g.RW, g.RW, 0};
   if (s.mode == MODE_DUAL && s.out2_cols > 0) {
@@ -250,16 +250,26 @@ int launch_step(kpm_ctx* ctx, const StepLaunch& s, const StepConfig& cfg, int* g
       return fail(KPM_EINVAL, "dual pass: second output blocks do not tile the row");
     o2 = SplitOut{(int)(s.out2_cols / cpw), (int)(s.out2_cols / cpw), s.out2_stride / cpw};
   }
+  int flags = step_flags(s.A, g);
+  int kmode = s.mode;
+  void* out2 = s.out2 ? s.out2->d : nullptr;
+  if (s.planes) {
+    if (s.mode != MODE_NONE && s.mode != MODE_DUAL) return fail(KPM_EINVAL, "planes from a doubling pass");
+    if (s.mode == MODE_DUAL && s.out2_cols > 0) return fail(KPM_EINVAL, "planes of a split second output");
+    if (s.mode == MODE_DUAL) kmode = MODE_DUALP;
+    else flags |= kStepPlanes | (s.planes_only ? kStepPlanesOnly : 0);
+    out2 = s.planes;
+  }
   const PendingCombine pc = s.pend ? *s.pend : PendingCombine{};
-  StepLauncher fn = pick(prec, g, s.A->is_real != 0, s.mode);
+  StepLauncher fn = pick(prec, g, s.A->is_real != 0, kmode);
   if (!fn) return fail(KPM_EINVAL, "no dual pass for rows of one lane");
   // FP32 blocks of narrow rows read the operator's FP32 values
-  const void* vals = vals32_step(prec, g.lpr, s.mode) ? s.A->vals32 : s.A->vals;
+  const void* vals = vals32_step(prec, g.lpr, kmode) ? s.A->vals32 : s.A->vals;
   if (!vals) return fail(KPM_EINVAL, "operator has no FP32 values");
   fn(dim3(G, g.chunks), ctx->stream, s.A->rowptr, s.A->colind, vals, x->d, s.z ? s.z->d : x->d,
      s.out->d, s.l ? s.l->d : nullptr, part, (int)x->NH, g.RW, (int)x->NRp,
-     (int)ntiles, s.alpha, s.beta, step_flags(s.A, g), nullptr, nullptr, nullptr, s.gamma, pc, s.vals2,
-     s.out2 ? s.out2->d : nullptr, o2, chunk, chunk > 0 ? cfg.xmap : 0);
+     (int)ntiles, s.alpha, s.beta, flags, nullptr, nullptr, nullptr, s.gamma, pc, s.vals2, out2, o2,
+     chunk, chunk > 0 ? cfg.xmap : 0);
   KPM_HIP_TRY(hipGetLastError());
   ++ctx->counters[KPM_COUNT_STEP];
   if (s.mode == MODE_DUAL) ++ctx->counters[KPM_COUNT_DUAL];

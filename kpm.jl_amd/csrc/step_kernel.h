@@ -46,6 +46,7 @@
 
 #include <type_traits>
 
+#include "bf16_planes.h"
 #include "kpm_internal.h"
 
 namespace kpm {
@@ -113,7 +114,7 @@ constexpr int kBlock = 64 * kStepWaves;  // 4 waves
 
 // whether the fused pass reads the operator's FP32 values (kpm_csr::vals32)
 __host__ __device__ constexpr bool vals32_step(int prec, int lpr, int mode) {
-  return prec == KPM_VEC_F32 && lpr <= 8 && mode != MODE_DUAL;
+  return prec == KPM_VEC_F32 && lpr <= 8 && mode != MODE_DUAL && mode != MODE_DUALP;
 }
 
 // compile-time loop: f(std::integral_constant<int, I>) for I in [B, E)
@@ -378,7 +379,11 @@ struct LaneMeta {
 // its first KB nonzeros, so the doubling modes take the own-row words x[row]
 // that their dots need from the gather of that slot instead of loading them
 // again (the slot whose column equals the row; the load it saves is a sixth of
-// the pass's vector loads on 4-entry rows).
+// the pass's vector loads on 4-entry rows); bit 2 (kStepPlanes, MODE_NONE),
+// out2 also receives the split-BF16 planes (bf16_planes.h) of out, and bit 3
+// (kStepPlanesOnly) leaves out unwritten.  MODE_DUALP is MODE_DUAL with the
+// second product written as planes instead of its words (a mode of its own:
+// a runtime branch there raised the dual pass from 68 to 86 VGPRs).
 template <int PREC, int LPR, int UNR, int KB, bool REALH, int MODE, bool SEG>
 __global__ void __launch_bounds__(kBlock)
     k_step(const int* __restrict__ rowptr, const int* __restrict__ colind,
@@ -403,7 +408,8 @@ __global__ void __launch_bounds__(kBlock)
   constexpr int MG = LPR >= 16 ? 16 : LPR;
   constexpr bool kLaneMeta = MG >= 2 && KB <= 16;
   constexpr int ME = kLaneMeta ? (KB + MG - 1) / MG : 1;
-  constexpr bool kDual = MODE == MODE_DUAL;
+  constexpr bool kDual = MODE == MODE_DUAL || MODE == MODE_DUALP;
+  constexpr bool kDualPlanes = MODE == MODE_DUALP;  // the second product as split-BF16 planes
   static_assert(!kDual || (kLaneMeta && !REALH && !SEG), "dual pass: complex values, spread metadata");
   using AT = typename WT::A;
   using CMeta = std::conditional_t<kLaneMeta, LaneMeta<UNR, ME, kDual, AT>, ColMeta<UNR, KB>>;
@@ -418,8 +424,17 @@ __global__ void __launch_bounds__(kBlock)
   const bool has_z = beta != 0.0;  // uniform
   const bool has_tail = (flags & kStepTail) != 0;
   const bool dgat = kDbl && (flags & kStepDiagGather) != 0;
+  // split-BF16 planes (bf16_planes.h) at out2: of out (MODE_NONE; without out
+  // when planes_only) or of the second product (MODE_DUALP)
+  const bool planes = MODE == MODE_NONE && !SEG && (flags & kStepPlanes) != 0;
+  const bool planes_only = planes && (flags & kStepPlanesOnly) != 0;
+  auto store_planes = [&](uint32_t* P, int row, const AT* re, const AT* im) {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c)
+      store_plane_value(P, (int64_t)row * NRp + (int64_t)wcol * CPL + c, (double)re[c], (double)im[c]);
+  };
   size_t o2col = 0;
-  if constexpr (MODE == MODE_DUAL) {
+  if constexpr (kDual) {
     const int qb = wcol / o2.cw;
     o2col = (size_t)qb * o2.qs + (size_t)(wcol - qb * o2.cw);
   }
@@ -765,7 +780,10 @@ __global__ void __launch_bounds__(kBlock)
         }
       }
       const T ow = WT::pack_a(orr, oii);
-      if (live) st_stream(out + (size_t)R0.row[u] * RW + wcol, ow);
+      if (live && !planes_only) st_stream(out + (size_t)R0.row[u] * RW + wcol, ow);
+      if constexpr (MODE == MODE_NONE && !SEG) {
+        if (planes && live) store_planes(reinterpret_cast<uint32_t*>(out2), R0.row[u], orr, oii);
+      }
       if constexpr (kDual) {
         AT pr[CPL], pi[CPL];
 #pragma unroll
@@ -773,7 +791,10 @@ __global__ void __launch_bounds__(kBlock)
           pr[c] = ga * y2r[u][c];
           pi[c] = ga * y2i[u][c];
         }
-        if (live) st_stream(out2 + (size_t)R0.row[u] * o2.rw + o2col, WT::pack_a(pr, pi));
+        if (live) {
+          if constexpr (kDualPlanes) store_planes(reinterpret_cast<uint32_t*>(out2), R0.row[u], pr, pi);
+          else st_stream(out2 + (size_t)R0.row[u] * o2.rw + o2col, WT::pack_a(pr, pi));
+        }
       }
       if constexpr (kDbl || MODE == MODE_LR) {
         double sr[CPL], si[CPL], ar[CPL], ai[CPL];
@@ -875,6 +896,9 @@ static StepLauncher pick_mode(int mode) {
     case MODE_SRC: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_SRC, false>;
     case MODE_DUAL:  // complex merged values on the spread-metadata path only
       if constexpr (!REALH && LPR >= 2) return &launch_step_t<PREC, LPR, UNR, KB, false, MODE_DUAL, false>;
+      return nullptr;
+    case MODE_DUALP:
+      if constexpr (!REALH && LPR >= 2) return &launch_step_t<PREC, LPR, UNR, KB, false, MODE_DUALP, false>;
       return nullptr;
     default: return &launch_step_t<PREC, LPR, UNR, KB, REALH, MODE_NONE, false>;
   }

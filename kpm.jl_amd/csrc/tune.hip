@@ -45,6 +45,26 @@ void tune_record_avg(kpm_ctx* ctx, const kpm_csr* A, int64_t NR, int prec, const
   g_table[make_key(ctx, A, NR, prec, MODE_DBLA)] = c;
 }
 
+// the passes without dots (MODE_NONE's entry: kpm_2d / kpm_3d chains and
+// their dual passes, the matrix-function action, the adjoint's source steps)
+static void tune_record_plain(kpm_ctx* ctx, const kpm_csr* A, int64_t NR, int prec, const StepConfig& c) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_table[make_key(ctx, A, NR, prec, MODE_NONE)] = c;
+}
+
+// reps plain passes Z <- 2 H X - Z in place, no dots
+static int time_plain_steps(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z,
+                            const StepConfig& c, int reps, float* ms) {
+  StepLaunch s{H, X, Z, Z, nullptr, 2.0, -1.0, MODE_NONE};
+  for (int r = 0; r < 2; ++r) KPM_TRY(launch_step(ctx, s, c, nullptr));
+  KPM_HIP_TRY(hipEventRecord(ctx->events[12], ctx->stream));
+  for (int r = 0; r < reps; ++r) KPM_TRY(launch_step(ctx, s, c, nullptr));
+  KPM_HIP_TRY(hipEventRecord(ctx->events[13], ctx->stream));
+  KPM_HIP_TRY(hipEventSynchronize(ctx->events[13]));
+  KPM_HIP_TRY(hipEventElapsedTime(ms, ctx->events[12], ctx->events[13]));
+  return KPM_OK;
+}
+
 }  // namespace kpm
 
 using namespace kpm;
@@ -53,8 +73,8 @@ extern "C" {
 
 // Time every candidate geometry of the MODE_DBL pass (the kpm_1d hot loop)
 // over `reps` launches each on blocks X, Z (Z is overwritten), record the
-// fastest for this shape (it then applies to all modes of the shape), and
-// return it.  Candidates: rows in flight per lane {1, 2} x workgroups per CU
+// fastest for this shape (it then applies to the passes with dots), and
+// return it; the probe-summed and the plain passes get their own entries.  Candidates: rows in flight per lane {1, 2} x workgroups per CU
 // {2, 4, 8, 16}.
 int kpm_autotune_step(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z, int reps,
                       int* best_unroll, int* best_grid, float* best_ms_per_step) {
@@ -93,17 +113,25 @@ int kpm_autotune_step(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z
   }
   // each candidate runs as the engine runs it: fused passes with the
   // deferred moment reductions their grid implies.  The per-probe pass
-  // (MODE_DBL, whose shape the other modes share) and the probe-summed pass
+  // (MODE_DBL, whose shape the passes with dots share), the probe-summed pass
   // (MODE_DBLA: a few bytes of partials per wave instead of NRp per
-  // workgroup, so it favours shorter chunks) are tuned separately.
+  // workgroup, so it favours shorter chunks) and the plain pass without dots
+  // (MODE_NONE) are tuned separately.  (On the Haldane 256 x 256, NR 16
+  // shape the per-probe pass's pick, a persistent grid, made kpm_2d's plain
+  // and dual passes 10 ms slower per call than one-shot chunks of 2 tiles.)
   auto best_of = [&](int mode, StepConfig* bc, float* bms) -> int {
     *bms = 1e30f;
     for (const StepConfig& c : cand) {
       // the better of two timings: candidates often lie within a few percent
       // of one another, and one timing alone let a transient pick the shape
       float ms = 0.f, ms2 = 0.f;
-      KPM_TRY(time_engine_steps(ctx, H, X, Z, c, reps, &ms, mode));
-      KPM_TRY(time_engine_steps(ctx, H, X, Z, c, reps, &ms2, mode));
+      if (mode == MODE_NONE) {
+        KPM_TRY(time_plain_steps(ctx, H, X, Z, c, reps, &ms));
+        KPM_TRY(time_plain_steps(ctx, H, X, Z, c, reps, &ms2));
+      } else {
+        KPM_TRY(time_engine_steps(ctx, H, X, Z, c, reps, &ms, mode));
+        KPM_TRY(time_engine_steps(ctx, H, X, Z, c, reps, &ms2, mode));
+      }
       ms = std::min(ms, ms2) / (float)reps;
       if (ms < *bms) {
         *bms = ms;
@@ -112,12 +140,14 @@ int kpm_autotune_step(kpm_ctx* ctx, const kpm_csr* H, kpm_block* X, kpm_block* Z
     }
     return KPM_OK;
   };
-  float best = 0.f, best_avg = 0.f;
-  StepConfig bestc, bestc_avg;
+  float best = 0.f, best_avg = 0.f, best_plain = 0.f;
+  StepConfig bestc, bestc_avg, bestc_plain;
   KPM_TRY(best_of(MODE_DBL, &bestc, &best));
   KPM_TRY(best_of(MODE_DBLA, &bestc_avg, &best_avg));
+  KPM_TRY(best_of(MODE_NONE, &bestc_plain, &best_plain));
   tune_record(ctx, H, X->NR, X->prec, bestc);
   tune_record_avg(ctx, H, X->NR, X->prec, bestc_avg);
+  tune_record_plain(ctx, H, X->NR, X->prec, bestc_plain);
   if (best_unroll) *best_unroll = bestc.unroll;
   if (best_grid) {  // the workgroup count the winner launches with
     const StepGeom g = step_geom(H, X->NR, X->prec, bestc);
@@ -146,11 +176,12 @@ int kpm_tune_set_chunked(kpm_ctx* ctx, const kpm_csr* H, int64_t NR, int vec_pre
 }
 
 // The table entry the engines use for a shape (defaults when untuned):
-// mode 0 the per-probe passes, mode 1 the probe-summed doubling pass.
+// mode 0 the per-probe passes, mode 1 the probe-summed doubling pass, mode 2
+// the plain passes without dots.
 int kpm_tune_get_mode(kpm_ctx* ctx, const kpm_csr* H, int64_t NR, int vec_prec, int mode,
                       int* unroll, int* grid, int* chunk, int* xcd_grouped) {
-  if (!ctx || !H || NR <= 0 || mode < 0 || mode > 1) return fail(KPM_EINVAL, "bad arguments");
-  const StepConfig c = tuned_config(ctx, H, NR, vec_prec, mode ? MODE_DBLA : MODE_DBL);
+  if (!ctx || !H || NR <= 0 || mode < 0 || mode > 2) return fail(KPM_EINVAL, "bad arguments");
+  const StepConfig c = tuned_config(ctx, H, NR, vec_prec, mode == 2 ? MODE_NONE : mode ? MODE_DBLA : MODE_DBL);
   if (unroll) *unroll = c.unroll;
   if (grid) *grid = c.grid;
   if (chunk) *chunk = c.chunk;

@@ -70,6 +70,7 @@ _SIGS = {
     "kpm_ctx_destroy": (C.c_int, [_p]),
     "kpm_ctx_synchronize": (C.c_int, [_p]),
     "kpm_ctx_set_dual_pass": (C.c_int, [_p, C.c_int]),
+    "kpm_ctx_set_bf16_planes": (C.c_int, [_p, C.c_int]),
     "kpm_device_free_memory": (C.c_int, [_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "kpm_device_name": (C.c_int, [_p, C.c_char_p, C.c_size_t]),
     "kpm_csr_from_csc": (C.c_int, [_p, _i64, _pi64, _pi64, _pd, C.c_int, C.POINTER(_p)]),

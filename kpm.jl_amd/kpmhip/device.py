@@ -96,6 +96,12 @@ class Context:
         previous order (default) or run them as two passes; same tables."""
         check(self.lib.kpm_ctx_set_dual_pass(self.handle, 1 if enable else 0), "kpm_ctx_set_dual_pass")
 
+    def set_bf16_planes(self, enable: bool):
+        """In the split-BF16 contraction mode, write kpm_2d's slots as BF16
+        hi/lo planes from the recurrence passes (default) or as FP64 / FP32
+        slots split on every read; bitwise the same tables."""
+        check(self.lib.kpm_ctx_set_bf16_planes(self.handle, 1 if enable else 0), "kpm_ctx_set_bf16_planes")
+
     def set_merged_limit(self, nbytes: int):
         """Cap the merged operator of the dual pass (bytes; negative: no cap).
         Over the cap kpm_2d / kpm_3d run the two passes separately."""

@@ -1,12 +1,13 @@
 """Launch-shape autotuner of the fused recurrence pass (no reference
 counterpart; the reference has no tuning knob).
 
-``autotune(H, NR, precision)`` times every candidate geometry of the kpm_1d
-hot-loop pass (rows in flight per lane x a persistent grid of 2..16
-workgroups per CU or a one-shot grid of 2..64 row tiles per workgroup) on
-the caller's operator and a probe-sized block of random-phase probes, and
-records the fastest for that shape in the library's table; every later engine
-call on the same shape uses it.  The blocks hold probes because the pass's
+``autotune(H, NR, precision)`` times every candidate geometry (rows in
+flight per lane x a persistent grid of 2..16 workgroups per CU or a one-shot
+grid of 2..64 row tiles per workgroup) of three passes: the kpm_1d hot-loop
+pass, its probe-summed form, and the plain pass without dots.  It runs them
+on the caller's operator and a probe-sized block of random-phase probes, and
+records the fastest of each for that shape in the library's table; every
+later engine call on the same shape uses them.  The blocks hold probes because the pass's
 speed depends on the values it moves: on all-zero blocks the schedules rank
 differently (graphene 2M sites, NR 64, FP32: per-XCD runs of 8 chunks time
 fastest on zeros, 561 us against 568-570 us for runs of 2 or none, but slowest
@@ -33,19 +34,22 @@ def autotune(H, NR: int, precision="f64", reps: int = 8) -> dict:
            "precision": prec_code(precision)}
     out.update({k: v for k, v in tune_get(Hd, NR, precision).items() if k in ("chunk", "xmap")})
     out["avg"] = tune_get(Hd, NR, precision, avg=True)
+    out["plain"] = tune_get(Hd, NR, precision, plain=True)
     return out
 
 
-def tune_get(H, NR: int, precision="f64", avg: bool = False) -> dict:
+def tune_get(H, NR: int, precision="f64", avg: bool = False, plain: bool = False) -> dict:
     """The launch shape the engines use for (H, NR, precision); chunk 0 is
     the persistent grid.  ``avg``: the probe-summed doubling pass of the
-    averaged kpm_1d instead of the per-probe passes."""
+    averaged kpm_1d instead of the per-probe passes; ``plain``: the passes
+    without dots (kpm_2d / kpm_3d chains, the action, the adjoint's source
+    steps)."""
     ctx = get_context()
     Hd = to_device(H, ctx)
     u, g, ch, xm = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
-    check(ctx.lib.kpm_tune_get_mode(ctx.handle, Hd.handle, int(NR), prec_code(precision),
-                                    1 if avg else 0, C.byref(u), C.byref(g), C.byref(ch),
-                                    C.byref(xm)), "tune_get")
+    mode = 2 if plain else 1 if avg else 0
+    check(ctx.lib.kpm_tune_get_mode(ctx.handle, Hd.handle, int(NR), prec_code(precision), mode,
+                                    C.byref(u), C.byref(g), C.byref(ch), C.byref(xm)), "tune_get")
     return {"unroll": u.value, "grid": g.value, "chunk": ch.value, "xmap": xm.value}
 
 
