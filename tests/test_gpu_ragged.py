@@ -7,10 +7,11 @@ and the left/right form with distinct probes and the probe-averaged form are
 checked the same way.
 CUSPARSE's SpMM (ext/KPMCUDAExt.jl:84-88) takes any CSR; the lattice models of
 the other tests have near-uniform rows, so this is where empty and long rows
-are exercised.
+are exercised, and banded operators fill each gather width exactly.
 """
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 import kpmhip as K
 import oracle
@@ -52,3 +53,52 @@ def test_ragged_rows_vs_oracle(w, NR, prec):
         np.testing.assert_allclose(avg, ref_avg, rtol=0, atol=1e-12)
     else:
         assert np.max(np.abs(avg - ref_avg)) < 2e-5
+
+
+def banded(n, offsets, seed):
+    """Hermitian operator whose interior rows hold exactly the given offsets
+    (0 = the diagonal), scaled into (-1, 1)."""
+    rng = np.random.default_rng(seed)
+    H = sp.lil_matrix((n, n), dtype=np.complex128)
+    for d in sorted({abs(o) for o in offsets if o != 0}):
+        v = rng.standard_normal(n - d) + 1j * rng.standard_normal(n - d)
+        H.setdiag(v, d)
+        H.setdiag(np.conj(v), -d)
+    if 0 in offsets:
+        H.setdiag(rng.uniform(0.5, 1.5, n))
+    H = H.tocsc()
+    bound = np.max(np.asarray(abs(H).sum(axis=1)).ravel())
+    return (H / (1.05 * bound)).tocsc()
+
+
+# the pass's gather widths (4, 6, 7, 8, 10, 12, 16), each hit exactly by the
+# operator's longest row
+WIDTHS = {
+    4: [-2, -1, 1, 2],
+    6: [-3, -2, -1, 1, 2, 3],
+    7: [-3, -2, -1, 0, 1, 2, 3],
+    8: [-4, -3, -2, -1, 1, 2, 3, 4],
+    10: [-5, -4, -3, -2, -1, 1, 2, 3, 4, 5],
+    12: [-6, -5, -4, -3, -2, -1, 1, 2, 3, 4, 5, 6],
+    16: [-8, -7, -6, -5, -4, -3, -2, -1, 1, 2, 3, 4, 5, 6, 7, 8],
+}
+
+
+@pytest.mark.parametrize("width", sorted(WIDTHS))
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_exact_gather_widths(width, prec):
+    """Operators whose longest row fills each gather width exactly, on rows of
+    1, 4 and 32 lanes: per-probe and probe-averaged moments against the
+    oracle."""
+    n, NC = 400, 30
+    H = banded(n, WIDTHS[width], seed=width)
+    assert np.diff(H.tocsr().indptr).max() == width
+    for NR in (1, 8, 64):
+        print(f"gather width {width} {prec} NR={NR}", flush=True)
+        psi = K.random_phase_vectors(Xoshiro256pp(3 + width + NR), n, NR)
+        ref = oracle.kpm_1d(H, NC, psi)
+        tol = 1e-12 if prec == "f64" else 2e-5
+        got = K.kpm_1d(H, NC, NR, psi_in=psi, avg_output=False, precision=prec)
+        assert np.max(np.abs(got - ref)) < tol
+        avg = K.kpm_1d(H, NC, NR, psi_in=psi, precision=prec)
+        assert np.max(np.abs(avg - ref.mean(axis=0).real)) < tol
