@@ -102,3 +102,25 @@ def test_small_blocks_keep_slot_path():
     ref = K.kpm_2d(H, Jx, Jy, NC, NR, psi_in=psi, gemm="f64", **kw)
     err = np.max(np.abs(on - ref)) / np.max(np.abs(ref))
     assert 1e-9 < err < 1e-4
+
+
+def test_kpm_3d_nested_form():
+    """kpm_3d's nested form (src/moment.jl:1106-1227) runs one kpm_2d per outer
+    order; with blocks of 128 slots (fewer left slots than NC, so not the
+    batched form) each of them takes the plane path: bitwise the FP64-slot
+    table."""
+    H, Jx, Jy = haldane(4)
+    NR, NC = 2, 256
+    psi = K.random_phase_vectors(Xoshiro256pp(77), H.shape[0], NR)
+    ctx = K.get_context()
+    outs = []
+    try:
+        for planes in (True, False):
+            print(f"bf16 planes kpm_3d nested planes={planes}", flush=True)
+            ctx.set_bf16_planes(planes)
+            outs.append(K.kpm_3d(H, Jx, Jy, Jx, NC, NR, psi_in=psi, gemm="bf16x3", arr_size=128,
+                                 right_block=128))
+    finally:
+        ctx.set_bf16_planes(True)
+    assert np.array_equal(outs[0], outs[1])
+    assert np.abs(outs[0]).max() > 0
