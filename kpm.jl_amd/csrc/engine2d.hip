@@ -154,7 +154,7 @@ static int dual_operator(kpm_ctx* ctx, const kpm_csr* A, const kpm_csr* B, int64
                          kpm_csr* M, const void** vals2, bool* ok) {
   *ok = false;
   if (!ctx->dual_pass || !A->sorted || !B->sorted || A->n != B->n || A->n < 1) return KPM_OK;
-  const int64_t n = A->n, cap = std::max<int64_t>(A->nnz + B->nnz, 1);
+  const int64_t n = A->n, cap = A->nnz + B->nnz + kCsrPad;  // union <= sum, plus the tail
   if (cap >= INT32_MAX) return KPM_OK;
   const size_t brp = ((size_t)(n + 1) * 4 + 255) / 256 * 256;
   const size_t bci = ((size_t)cap * 4 + 255) / 256 * 256;

@@ -153,6 +153,12 @@ struct kpm_accum {
   void* d = nullptr;
 };
 
+// Entries past nnz that every CSR column-index and value array carries
+// (zeros): a lane of the fused pass loads its share of a row's nonzeros with
+// one vector load per array, which may read up to this far past the row's
+// end (past the last row: past nnz).
+constexpr int64_t kCsrPad = 32;
+
 struct kpm_csr {
   kpm_ctx* ctx = nullptr;
   // position in this CSR of the k-th stored entry of the CSC arrays it was

@@ -102,7 +102,7 @@ __global__ void k_round_f32(const double* __restrict__ src, float* __restrict__ 
 }
 
 int attach_vals32(kpm_ctx* ctx, kpm_csr* A) {
-  const int64_t nreal = std::max<int64_t>(A->nnz, 1) * (A->is_real ? 1 : 2);
+  const int64_t nreal = (A->nnz + kCsrPad) * (A->is_real ? 1 : 2);
   hipError_t e = hipMalloc(&A->vals32, (size_t)nreal * sizeof(float));
   if (e != hipSuccess) {
     A->vals32 = nullptr;
@@ -110,6 +110,7 @@ int attach_vals32(kpm_ctx* ctx, kpm_csr* A) {
     return fail(KPM_ENOMEM, std::string("hipMalloc for FP32 operator values failed: ") +
                                 hipGetErrorString(e));
   }
+  KPM_HIP_TRY(hipMemsetAsync(A->vals32, 0, (size_t)nreal * sizeof(float), ctx->stream));  // the zero tail
   if (A->nnz > 0) {
     const int64_t n = A->nnz * (A->is_real ? 1 : 2);
     const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
@@ -146,10 +147,13 @@ int upload_csr(kpm_ctx* ctx, int64_t n, const std::vector<int64_t>& rp,
   A->is_real = is_real ? 1 : 0;
   A->max_row_nnz = maxrow;
   stamp_csr(A, rp32.data(), ci32.data());
-  const size_t vbytes = (size_t)std::max<int64_t>(nnz, 1) * (is_real ? sizeof(double) : 2 * sizeof(double));
+  const size_t vbytes = (size_t)(nnz + kCsrPad) * (is_real ? sizeof(double) : 2 * sizeof(double));
+  const size_t cbytes = sizeof(int) * (size_t)(nnz + kCsrPad);
   hipError_t e = hipMalloc(&A->rowptr, sizeof(int) * (n + 1));
-  if (e == hipSuccess) e = hipMalloc(&A->colind, sizeof(int) * std::max<int64_t>(nnz, 1));
+  if (e == hipSuccess) e = hipMalloc(&A->colind, cbytes);
   if (e == hipSuccess) e = hipMalloc(&A->vals, vbytes);
+  if (e == hipSuccess) e = hipMemset(A->colind, 0, cbytes);  // the zero tail
+  if (e == hipSuccess) e = hipMemset(A->vals, 0, vbytes);
   if (e != hipSuccess) {
     if (A->rowptr) (void)hipFree(A->rowptr);
     if (A->colind) (void)hipFree(A->colind);
@@ -215,6 +219,9 @@ int csr_from_csc_staged(kpm_ctx* ctx, int64_t n, const int64_t* colptr, const in
   const int64_t nz1 = std::max<int64_t>(nnz, 1);
   const size_t brp = (size_t)(n + 1) * sizeof(int), bci = (size_t)nz1 * sizeof(int);
   const size_t bv = (size_t)nz1 * (is_real ? sizeof(double) : 2 * sizeof(double));
+  // device arrays: the staged entries plus the zero tail
+  const size_t bci_dev = (size_t)(nnz + kCsrPad) * sizeof(int);
+  const size_t bv_dev = (size_t)(nnz + kCsrPad) * (is_real ? sizeof(double) : 2 * sizeof(double));
   const size_t off_ci = (brp + 255) / 256 * 256, off_v = off_ci + (bci + 255) / 256 * 256;
   KPM_TRY(ensure_pinned(ctx, off_v + bv));
   char* base_p = (char*)ctx->pinned.ptr;
@@ -264,8 +271,10 @@ int csr_from_csc_staged(kpm_ctx* ctx, int64_t n, const int64_t* colptr, const in
   A->max_row_nnz = maxrow;
   stamp_csr(A, rp, ci);
   hipError_t e = hipMalloc(&A->rowptr, brp);
-  if (e == hipSuccess) e = hipMalloc(&A->colind, bci);
-  if (e == hipSuccess) e = hipMalloc(&A->vals, bv);
+  if (e == hipSuccess) e = hipMalloc(&A->colind, bci_dev);
+  if (e == hipSuccess) e = hipMalloc(&A->vals, bv_dev);
+  if (e == hipSuccess) e = hipMemsetAsync(A->colind, 0, bci_dev, ctx->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(A->vals, 0, bv_dev, ctx->stream);
   if (e != hipSuccess) {
     if (A->rowptr) (void)hipFree(A->rowptr);
     if (A->colind) (void)hipFree(A->colind);

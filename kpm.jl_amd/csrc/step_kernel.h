@@ -345,9 +345,10 @@ struct ColMeta {
   int c[UNR][KB];
 };
 // Nonzeros spread over the lanes of a row: with G = min(LPR, 16) lanes per
-// row inside a DPP row, lane j holds nonzeros j, j + G, ... (E of them) of its
-// row, column index and value, loaded coalesced (E loads per array instead of
-// KB same-address loads per lane); group_bcast hands slot jj to every lane of
+// row inside a DPP row, lane j holds nonzeros jE .. jE + E - 1 of its row,
+// column index and value, one contiguous run per array (vector loads instead
+// of KB same-address loads per lane; on 2-lane rows of 7 entries, E = 4 words
+// in one load instead of four); group_bcast hands slot jj to every lane of
 // the row.  Rows wider than 16 lanes load the same nonzeros in each DPP row.
 // DUAL adds the values of a second operator on the same pattern.
 template <int UNR, int E, bool DUAL = false, class AT = double>
@@ -543,31 +544,38 @@ __global__ void __launch_bounds__(kBlock)
   // kLaneMeta also their values (invalid slots: weight 0)
   auto load_cols = [&](const RowMeta<UNR>& R, CMeta& Cm) {
     if constexpr (kLaneMeta) {
+      // lane j of the row holds its nonzeros j*ME .. j*ME + ME - 1, read as
+      // one run per array (the compiler merges the consecutive loads into
+      // vector loads; the arrays' zero tail, kCsrPad entries, covers the
+      // reads past a row's end, and rows past the range read from entry 0)
       const int j = lane & (MG - 1);
+      const int i0 = j * ME;
 #pragma unroll
-      for (int u = 0; u < UNR; ++u)
+      for (int u = 0; u < UNR; ++u) {
+        const int k0 = R.s[u] + i0;
+        int cw[ME];
+#pragma unroll
+        for (int e = 0; e < ME; ++e) cw[e] = colind[k0 + e];
 #pragma unroll
         for (int e = 0; e < ME; ++e) {
-          const int idx = e * MG + j;
-          const bool v = idx < R.cnt[u];
-          const int k = v ? R.s[u] + idx : 0;
-          const int c = colind[k];
-          Cm.c[u][e] = v ? c : 0;
+          const bool v = i0 + e < R.cnt[u];
+          Cm.c[u][e] = v ? cw[e] : 0;
           if constexpr (REALH) {
-            const RV h = rvals[k];
+            const RV h = rvals[k0 + e];
             Cm.hr[u][e] = v ? (AT)h : (AT)0;
             Cm.hi[u][e] = (AT)0;
           } else {
-            const CV h = cvals[k];
+            const CV h = cvals[k0 + e];
             Cm.hr[u][e] = v ? (AT)h.x : (AT)0;
             Cm.hi[u][e] = v ? (AT)h.y : (AT)0;
           }
           if constexpr (kDual) {
-            const double2 g = cvals2[k];
+            const double2 g = cvals2[k0 + e];
             Cm.gr[u][e] = v ? (AT)g.x : (AT)0;
             Cm.gi[u][e] = v ? (AT)g.y : (AT)0;
           }
         }
+      }
     } else {
 #pragma unroll
       for (int u = 0; u < UNR; ++u)
@@ -660,7 +668,7 @@ __global__ void __launch_bounds__(kBlock)
       for (int u = 0; u < UNR; ++u)
         static_for<0, KB>([&](auto J) {
           constexpr int jj = decltype(J)::value;
-          constexpr int e = jj / MG, src = jj % MG;
+          constexpr int e = jj % ME, src = jj / ME;  // nonzero jj: lane jj / ME, entry jj % ME
           const int c = group_bcast<MG, src>(C0.c[u][e]);
           if constexpr (kDbl) own[u][jj] = c == R0.row[u];
           hr[u][jj] = group_bcast<MG, src>(C0.hr[u][e]);
