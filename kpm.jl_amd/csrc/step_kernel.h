@@ -577,13 +577,14 @@ __global__ void __launch_bounds__(kBlock)
         }
       }
     } else {
+      // one lane per row: the row's first KB column indices as one run (the
+      // zero tail covers the reads past its end)
 #pragma unroll
       for (int u = 0; u < UNR; ++u)
 #pragma unroll
         for (int jj = 0; jj < KB; ++jj) {
-          const bool v = jj < R.cnt[u];
-          const int c = colind[v ? R.s[u] + jj : 0];
-          Cm.c[u][jj] = v ? c : 0;
+          const int c = colind[R.s[u] + jj];
+          Cm.c[u][jj] = jj < R.cnt[u] ? c : 0;
         }
     }
   };
@@ -685,7 +686,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
         for (int jj = 0; jj < KB; ++jj) {
           const bool v = jj < R0.cnt[u];
-          const int k = v ? R0.s[u] + jj : 0;
+          const int k = R0.s[u] + jj;  // one run per row (zero tail past its end)
           if constexpr (REALH) {
             const RV h = rvals[k];
             hr[u][jj] = v ? (AT)h : (AT)0;
