@@ -172,7 +172,7 @@ StepGeom step_geom(const kpm_csr* A, int64_t NR, int prec, const StepConfig& cfg
   g.lpr = std::min(g.RW, 64);
   g.chunks = g.RW / g.lpr;
   const int w = A->max_row_nnz;
-  g.kb = w <= 4 ? 4 : w <= 6 ? 6 : w <= 7 ? 7 : w <= 8 ? 8 : w <= 10 ? 10 : w <= 12 ? 12 : 16;
+  g.kb = w <= 4 ? 4 : w <= 5 ? 5 : w <= 6 ? 6 : w <= 7 ? 7 : w <= 8 ? 8 : w <= 10 ? 10 : w <= 12 ? 12 : 16;
   g.unr = (cfg.unroll == 1 || g.kb > 8) ? 1 : 2;
   g.has_tail = A->max_row_nnz > g.kb ? 1 : 0;
   g.TR = 4LL * (64 / g.lpr) * g.unr;

@@ -918,16 +918,19 @@ static StepLauncher pick_real(bool realh, int mode) {
   return realh ? pick_mode<PREC, LPR, UNR, KB, true>(mode) : pick_mode<PREC, LPR, UNR, KB, false>(mode);
 }
 
-// Gather slots per row: 4, 6, 7, 8, 10, 12 or 16, the smallest that holds the
-// operator's longest row (slots past a row's end still issue a gather of row
-// 0, so 5-, 7- and 9/10-entry lattices get their own widths; the cubic
-// lattice's 7 entries per row at width 7 instead of 8: batched ensemble of
-// 256 realisations 1.597e12 -> 1.642e12 moments/s).  KB 10, 12 and 16 keep
-// one row in flight per lane (register budget).
+// Gather slots per row: 4, 5, 6, 7, 8, 10, 12 or 16, the smallest that holds
+// the operator's longest row (slots past a row's end still issue a gather of
+// row 0, so the common lattices get their own widths: the cubic lattice's 7
+// entries per row at width 7 instead of 8, batched ensemble of 256
+// realisations 1.597e12 -> 1.642e12 moments/s; the disordered square
+// lattice's 5 at width 5 instead of 6, DOS-loss gradient on 512^2 133 -> 130
+// ms).  KB 10, 12 and 16 keep one row in flight per lane (register budget).
 template <int PREC, int LPR>
 static StepLauncher pick_kb(int unr, int kb, bool realh, int mode) {
   if (kb <= 4)
     return unr == 1 ? pick_real<PREC, LPR, 1, 4>(realh, mode) : pick_real<PREC, LPR, 2, 4>(realh, mode);
+  if (kb <= 5)
+    return unr == 1 ? pick_real<PREC, LPR, 1, 5>(realh, mode) : pick_real<PREC, LPR, 2, 5>(realh, mode);
   if (kb <= 6)
     return unr == 1 ? pick_real<PREC, LPR, 1, 6>(realh, mode) : pick_real<PREC, LPR, 2, 6>(realh, mode);
   if (kb <= 7)

@@ -71,10 +71,11 @@ def banded(n, offsets, seed):
     return (H / (1.05 * bound)).tocsc()
 
 
-# the pass's gather widths (4, 6, 7, 8, 10, 12, 16), each hit exactly by the
-# operator's longest row
+# the pass's gather widths (4, 5, 6, 7, 8, 10, 12, 16), each hit exactly by
+# the operator's longest row
 WIDTHS = {
     4: [-2, -1, 1, 2],
+    5: [-2, -1, 0, 1, 2],
     6: [-3, -2, -1, 1, 2, 3],
     7: [-3, -2, -1, 0, 1, 2, 3],
     8: [-4, -3, -2, -1, 1, 2, 3, 4],
