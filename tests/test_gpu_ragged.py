@@ -103,3 +103,25 @@ def test_exact_gather_widths(width, prec):
         assert np.max(np.abs(got - ref)) < tol
         avg = K.kpm_1d(H, NC, NR, psi_in=psi, precision=prec)
         assert np.max(np.abs(avg - ref.mean(axis=0).real)) < tol
+
+
+@pytest.mark.parametrize("n", [1, 2, 3])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_tiny_operators(n, prec):
+    """Operators of 1-3 rows (nnz from 1 to 9): a lane's run of nonzeros
+    reads far past nnz into the arrays' zero tail; the moments still match
+    the oracle on rows of 1, 4 and 32 lanes."""
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    D = (A + A.conj().T) / 2
+    D /= 1.05 * np.max(np.abs(np.linalg.eigvalsh(D)))
+    H = sp.csc_matrix(D)
+    for NR in (1, 8, 64):
+        print(f"tiny operator n={n} {prec} NR={NR}", flush=True)
+        psi = K.random_phase_vectors(Xoshiro256pp(40 + n + NR), n, NR)
+        ref = oracle.kpm_1d(H, 16, psi)
+        tol = 1e-12 if prec == "f64" else 2e-5
+        got = K.kpm_1d(H, 16, NR, psi_in=psi, avg_output=False, precision=prec)
+        assert np.max(np.abs(got - ref)) < tol
+        avg = K.kpm_1d(H, 16, NR, psi_in=psi, precision=prec)
+        assert np.max(np.abs(avg - ref.mean(axis=0).real)) < tol
