@@ -29,6 +29,7 @@
 #include "kpm_internal.h"
 
 #include <algorithm>
+#include <type_traits>
 
 #include "bf16_planes.h"
 
@@ -565,6 +566,16 @@ __global__ void __launch_bounds__(kT128Threads)
 // bitwise the same as from the FP64 (or FP32) slots the planes were split
 // from.  (Measured on the 512 x 512, K = 2.1M complex table: 11.0 ms against
 // 14.6 ms from FP64 slots, both on random data.)
+//
+// The stage words are loaded two stages ahead, into two named register pairs
+// that alternate between even and odd stages: stage s + 2 is loaded at the top
+// of stage s, and stage s + 1 (loaded one stage earlier) is stored to LDS at
+// its end, so each load has two stages of MFMAs to arrive behind (10.8 against
+// 11.2 ms for one stage ahead).  The loads and the stores are unconditional:
+// the load index is clamped to the last stage, and the store at the last
+// stage goes to the buffer that no wave reads again.  A load or store under a
+// branch is moved by the compiler next to its use, or makes it wait for every
+// load in flight.
 __global__ void __launch_bounds__(kT128Threads)
     k_contract_planes_t128(const uint4* __restrict__ L, int nL, const uint4* __restrict__ R, int nR,
                            int64_t K2, int64_t P2, int64_t kchunk, int tm, int ntiles, int splits,
@@ -612,15 +623,26 @@ __global__ void __launch_bounds__(kT128Threads)
     loff[q] = (wi >> 2) * PLANE + (op * TT + row) * LDB + (wi & 3) * 4;
   }
   uint32_t* const base = &lds[0][0][0][0];
-  uint4 w0, w1;  // (named words, not an array: an array of them went to scratch)
-  auto gload = [&](int stage) {
-    w0 = src[0][(int64_t)stage * PW];
-    w1 = src[1][(int64_t)stage * PW];
+  // stage words of even and odd stages (named registers, chosen at compile
+  // time: an array of them, or a reference to one pair, went to scratch)
+  uint4 e0, e1, o0, o1;
+  auto gload = [&](auto odd, int stage) {
+    const uint4 a = src[0][(int64_t)stage * PW], b = src[1][(int64_t)stage * PW];
+    if constexpr (decltype(odd)::value) {
+      o0 = a;
+      o1 = b;
+    } else {
+      e0 = a;
+      e1 = b;
+    }
   };
-  auto lstore = [&](int buf) {
-    *reinterpret_cast<uint4*>(base + buf * BUF + loff[0]) = w0;
-    *reinterpret_cast<uint4*>(base + buf * BUF + loff[1]) = w1;
+  auto lstore = [&](auto odd, int buf) {
+    const uint4 a = decltype(odd)::value ? o0 : e0, b = decltype(odd)::value ? o1 : e1;
+    *reinterpret_cast<uint4*>(base + buf * BUF + loff[0]) = a;
+    *reinterpret_cast<uint4*>(base + buf * BUF + loff[1]) = b;
   };
+  const std::integral_constant<bool, false> even_pair;
+  const std::integral_constant<bool, true> odd_pair;
 
   f4 are[WN][WM], aim[WN][WM];
 #pragma unroll
@@ -628,14 +650,14 @@ __global__ void __launch_bounds__(kT128Threads)
 #pragma unroll
     for (int b = 0; b < WM; ++b) are[a][b] = aim[a][b] = f4{0.f, 0.f, 0.f, 0.f};
 
-  if (nstage > 0) {
-    gload(0);
-    lstore(0);
-  }
-  __syncthreads();
-  for (int st = 0; st < nstage; ++st) {
+  // stage st (its words in LDS buffer st & 1): load stage st + 2 into this
+  // parity's pair, run the MFMAs, store stage st + 1 from the other pair
+  auto run_stage = [&](auto odd, int st) {
     const int buf = st & 1;
-    if (st + 1 < nstage) gload(st + 1);
+    gload(odd, min(st + 2, nstage - 1));
+    // (the scheduling barriers keep the loads at the top of the stage and the
+    // stores at its end: without them the loads were moved down to the stores)
+    __builtin_amdgcn_sched_barrier(0);
     uint4 ah[WN], al[WN];
 #pragma unroll
     for (int t = 0; t < WN; ++t) {
@@ -659,9 +681,23 @@ __global__ void __launch_bounds__(kT128Threads)
         aim[tn][tmi] = mfma_bf16(ah[tn], b2h, aim[tn][tmi]);
       }
     }
-    if (st + 1 < nstage) lstore(buf ^ 1);
+    __builtin_amdgcn_sched_barrier(0);
+    lstore(std::integral_constant<bool, !decltype(odd)::value>{}, buf ^ 1);
     __syncthreads();
+  };
+
+  if (nstage > 0) {
+    gload(even_pair, 0);
+    gload(odd_pair, min(1, nstage - 1));
+    lstore(even_pair, 0);
   }
+  __syncthreads();
+  int st = 0;
+  for (; st + 1 < nstage; st += 2) {
+    run_stage(even_pair, st);
+    run_stage(odd_pair, st + 1);
+  }
+  if (st < nstage) run_stage(even_pair, st);
   float2* out = part + (size_t)z * ldn * ldm;
 #pragma unroll
   for (int tn = 0; tn < WN; ++tn)
