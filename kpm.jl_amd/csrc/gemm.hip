@@ -29,7 +29,6 @@
 #include "kpm_internal.h"
 
 #include <algorithm>
-#include <type_traits>
 
 #include "bf16_planes.h"
 
@@ -562,31 +561,32 @@ __global__ void __launch_bounds__(kT128Threads)
 // words, exactly the LDS row of the kernel above without its pad), so the
 // stage is copied to LDS without a split, and the operand bytes read per
 // stage are half those of FP64 slots.  The LDS layout, the fragment reads and
-// the MFMA order are those of k_contract_bf16x3_t128, so the partials are
-// bitwise the same as from the FP64 (or FP32) slots the planes were split
-// from.  (Measured on the 512 x 512, K = 2.1M complex table: 11.0 ms against
-// 14.6 ms from FP64 slots, both on random data.)
+// the MFMA order per accumulator are those of k_contract_bf16x3_t128, so the
+// partials are bitwise the same as from the FP64 (or FP32) slots the planes
+// were split from.
 //
-// The stage words are loaded two stages ahead, into two named register pairs
-// that alternate between even and odd stages: stage s + 2 is loaded at the top
-// of stage s, and stage s + 1 (loaded one stage earlier) is stored to LDS at
-// its end, so each load has two stages of MFMAs to arrive behind (10.8 against
-// 11.2 ms for one stage ahead).  The loads and the stores are unconditional:
-// the load index is clamped to the last stage, and the store at the last
-// stage goes to the buffer that no wave reads again.  A load or store under a
-// branch is moved by the compiler next to its use, or makes it wait for every
-// load in flight.
-__global__ void __launch_bounds__(kT128Threads)
+// The 128 x 128 tile runs on eight waves of 32 (n) x 64 (m), so that two
+// workgroups (80 KB of LDS each) share a CU: while one waits at its per-stage
+// barrier for its stage loads, the other's MFMAs run.  Measured on the
+// 512 x 512, K = 2.1M complex table on random planes: 10.3-10.5 ms, against
+// 10.7-10.9 ms for sixteen waves of 32 x 32 in one workgroup per CU with loads
+// two stages ahead, and 11.0-11.2 ms with loads one stage ahead.  (The
+// register limit of four waves per SIMD leaves no room for a second set of
+// stage words: two-ahead loads on this layout spilled 40 registers.)
+constexpr int kPlaneThreads = 512;
+
+__global__ void __launch_bounds__(kPlaneThreads, 4)
     k_contract_planes_t128(const uint4* __restrict__ L, int nL, const uint4* __restrict__ R, int nR,
                            int64_t K2, int64_t P2, int64_t kchunk, int tm, int ntiles, int splits,
                            float2* __restrict__ part, int ldn, int ldm) {
   constexpr int TT = 128;
-  constexpr int NT = kT128Threads;
-  constexpr int WN = 2, WM = 2;
+  constexpr int NT = kPlaneThreads;
+  constexpr int WN = 2, WM = 4;        // MFMA tiles per wave along n (R rows) and m (L rows)
   constexpr int LDB = kStep / 2 + 4;   // LDS row stride in words (16 pairs + pad)
   constexpr int PW = 8;                // 16-byte words per row and stage: 4 hi, then 4 lo
   constexpr int PNW = 2 * TT * PW / NT;
-  static_assert(PNW == 2, "two plane words per thread and stage");
+  static_assert(PNW == 4, "four plane words per thread and stage");
+  static_assert((TT / (16 * WN)) * (TT / (16 * WM)) == NT / 64, "waves cover the tile");
   // [plane][buffer][operand][row * LDB]
   __shared__ uint32_t lds[2][2][2][TT * LDB];
   constexpr int PLANE = 2 * 2 * TT * LDB, BUF = 2 * TT * LDB;
@@ -609,7 +609,7 @@ __global__ void __launch_bounds__(kT128Threads)
   const int64_t kend = min(kbeg + kchunk, K2);
   const int nstage = kend > kbeg ? (int)((kend - kbeg) / kStep) : 0;
 
-  // the two words this thread moves per stage: global source, LDS offset
+  // the four words this thread moves per stage: global source, LDS offset
   const uint4* src[PNW];
   int loff[PNW];
 #pragma unroll
@@ -623,26 +623,21 @@ __global__ void __launch_bounds__(kT128Threads)
     loff[q] = (wi >> 2) * PLANE + (op * TT + row) * LDB + (wi & 3) * 4;
   }
   uint32_t* const base = &lds[0][0][0][0];
-  // stage words of even and odd stages (named registers, chosen at compile
-  // time: an array of them, or a reference to one pair, went to scratch)
-  uint4 e0, e1, o0, o1;
-  auto gload = [&](auto odd, int stage) {
-    const uint4 a = src[0][(int64_t)stage * PW], b = src[1][(int64_t)stage * PW];
-    if constexpr (decltype(odd)::value) {
-      o0 = a;
-      o1 = b;
-    } else {
-      e0 = a;
-      e1 = b;
-    }
+  uint4 w0, w1, w2, w3;  // (named words, not an array: an array of them went to scratch)
+  auto gload = [&](int stage) {
+    const int64_t o = (int64_t)stage * PW;
+    w0 = src[0][o];
+    w1 = src[1][o];
+    w2 = src[2][o];
+    w3 = src[3][o];
   };
-  auto lstore = [&](auto odd, int buf) {
-    const uint4 a = decltype(odd)::value ? o0 : e0, b = decltype(odd)::value ? o1 : e1;
-    *reinterpret_cast<uint4*>(base + buf * BUF + loff[0]) = a;
-    *reinterpret_cast<uint4*>(base + buf * BUF + loff[1]) = b;
+  auto lstore = [&](int buf) {
+    uint32_t* p = base + buf * BUF;
+    *reinterpret_cast<uint4*>(p + loff[0]) = w0;
+    *reinterpret_cast<uint4*>(p + loff[1]) = w1;
+    *reinterpret_cast<uint4*>(p + loff[2]) = w2;
+    *reinterpret_cast<uint4*>(p + loff[3]) = w3;
   };
-  const std::integral_constant<bool, false> even_pair;
-  const std::integral_constant<bool, true> odd_pair;
 
   f4 are[WN][WM], aim[WN][WM];
 #pragma unroll
@@ -650,26 +645,27 @@ __global__ void __launch_bounds__(kT128Threads)
 #pragma unroll
     for (int b = 0; b < WM; ++b) are[a][b] = aim[a][b] = f4{0.f, 0.f, 0.f, 0.f};
 
-  // stage st (its words in LDS buffer st & 1): load stage st + 2 into this
-  // parity's pair, run the MFMAs, store stage st + 1 from the other pair
-  auto run_stage = [&](auto odd, int st) {
+  if (nstage > 0) {
+    gload(0);
+    lstore(0);
+  }
+  __syncthreads();
+  for (int st = 0; st < nstage; ++st) {
     const int buf = st & 1;
-    gload(odd, min(st + 2, nstage - 1));
-    // (the scheduling barriers keep the loads at the top of the stage and the
-    // stores at its end: without them the loads were moved down to the stores)
-    __builtin_amdgcn_sched_barrier(0);
+    if (st + 1 < nstage) gload(st + 1);
+    const uint32_t* p = base + buf * BUF;
     uint4 ah[WN], al[WN];
 #pragma unroll
     for (int t = 0; t < WN; ++t) {
-      const int ra = buf * BUF + (n0 + 16 * t + i) * LDB + 4 * g;
-      ah[t] = *reinterpret_cast<const uint4*>(base + ra);
-      al[t] = *reinterpret_cast<const uint4*>(base + PLANE + ra);
+      const int ra = (n0 + 16 * t + i) * LDB + 4 * g;
+      ah[t] = *reinterpret_cast<const uint4*>(p + ra);
+      al[t] = *reinterpret_cast<const uint4*>(p + PLANE + ra);
     }
 #pragma unroll
     for (int tmi = 0; tmi < WM; ++tmi) {
-      const int rb = buf * BUF + (TT + m0 + 16 * tmi + i) * LDB + 4 * g;
-      const uint4 bh = *reinterpret_cast<const uint4*>(base + rb);
-      const uint4 bl = *reinterpret_cast<const uint4*>(base + PLANE + rb);
+      const int rb = (TT + m0 + 16 * tmi + i) * LDB + 4 * g;
+      const uint4 bh = *reinterpret_cast<const uint4*>(p + rb);
+      const uint4 bl = *reinterpret_cast<const uint4*>(p + PLANE + rb);
       const uint4 b2h = swap_neg(bh), b2l = swap_neg(bl);
 #pragma unroll
       for (int tn = 0; tn < WN; ++tn) {
@@ -681,23 +677,9 @@ __global__ void __launch_bounds__(kT128Threads)
         aim[tn][tmi] = mfma_bf16(ah[tn], b2h, aim[tn][tmi]);
       }
     }
-    __builtin_amdgcn_sched_barrier(0);
-    lstore(std::integral_constant<bool, !decltype(odd)::value>{}, buf ^ 1);
+    if (st + 1 < nstage) lstore(buf ^ 1);
     __syncthreads();
-  };
-
-  if (nstage > 0) {
-    gload(even_pair, 0);
-    gload(odd_pair, min(1, nstage - 1));
-    lstore(even_pair, 0);
   }
-  __syncthreads();
-  int st = 0;
-  for (; st + 1 < nstage; st += 2) {
-    run_stage(even_pair, st);
-    run_stage(odd_pair, st + 1);
-  }
-  if (st < nstage) run_stage(even_pair, st);
   float2* out = part + (size_t)z * ldn * ldm;
 #pragma unroll
   for (int tn = 0; tn < WN; ++tn)
@@ -783,7 +765,7 @@ static int launch_contract_bf16_t128(kpm_ctx* ctx, const void* Lslots, int nL, c
   const dim3 grid((unsigned)(ntiles * splits));
   const int sp = (int)splits;
   if (prec < 0)
-    hipLaunchKernelGGL(k_contract_planes_t128, grid, dim3(kT128Threads), 0, ctx->stream,
+    hipLaunchKernelGGL(k_contract_planes_t128, grid, dim3(kPlaneThreads), 0, ctx->stream,
                        (const uint4*)Lslots, nL, (const uint4*)Rslots, nR, K2, P2, kchunk, tm, ntiles, sp,
                        (float2*)ctx->gemm_part.ptr, ldn, ldm);
   else if (prec == KPM_VEC_F64)
