@@ -45,6 +45,9 @@ CASES = [
     (12, 4, 256, dict(arr_size=256, right_block=128, moment_parity="ODD")),
     (10, 6, 256, dict(arr_size=256, right_block=256, precision="f32")),
     (10, 16, 128, dict(arr_size=128, right_block=128)),
+    # 192 slots a side: the second 128-slot tile of each side is partial (its
+    # rows past the last slot are clamped on load and masked on store)
+    (12, 4, 384, dict(arr_size=192, right_block=192)),
 ]
 
 
